@@ -1,0 +1,232 @@
+"""Benchmark: particle-steps/s of the sfparticles step on one MI355X.
+
+Workload (the head-on collision the metric is quoted on): 1e7 electrons,
+gamma ~ N(1957, 5%), momentum along -x, x ~ N(12 um, 3 um), y, z ~ N(0, 1 um),
+against a linearly polarised paraxial Gaussian pulse (lambda 0.8 um, a0 100,
+w0 2 um, tau_FWHM 30 fs) starting at x0 = -12 um; dt = T/100; 3000 steps;
+rejection-sampling photon emission + Breit-Wheeler pairs (electrons,
+positrons, photons), photon threshold 2 m c^2, Philox seed 0; FP64.
+
+A step is one pass of the fused step kernels over every species.  Inputs are
+uploaded to HBM before the timed region; the timed region is exactly --steps
+steps starting at step 0 of the collision.  --warmup steps run first on a
+separate, smaller copy of the workload (kernel load, allocator warm-up).
+
+Prints one JSON line.  Also reported: the dominant kernel's achieved
+algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around each of
+its launches) and the CPU oracle timed on this machine's cores on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C_LIGHT = 299792458.0
+LAMBDA = 0.8e-6
+DT = LAMBDA / C_LIGHT / 100
+CTAU = C_LIGHT * 30e-15 / math.sqrt(2 * math.log(2))
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per particle-step of each step kernel: the persistent
+# state read once and written once, plus the id (8 B) and pruned flag (1 B);
+# per appended particle 57 B are written (x, y, z, u 48 B + id + flag)
+BYTES = {
+    "lepton_rs": 48 + 48 + 8 + 1,   # x,y,z,u read and written
+    "lepton_od": 48 + 48 + 8 + 1 + 16,
+    "photon_rs": 48 + 24 + 8 + 1,   # u read only
+    "photon_od": 48 + 24 + 8 + 1 + 16,
+    "spawn": 57,
+}
+
+
+def make_case(n, nsteps, seed=0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gamma = rng.normal(1957.0, 0.05 * 1957.0, n)
+    d = C_LIGHT * DT * nsteps / 2  # 12 um for 3000 steps: the pulse meets the bunch half-way
+    x = rng.normal(d, 3e-6, n)
+    y = rng.normal(0.0, 1e-6, n)
+    z = rng.normal(0.0, 1e-6, n)
+    ux = -np.sqrt(gamma ** 2 - 1)
+    return (x, y, z, ux, np.zeros(n), np.zeros(n)), [(2, (100.0, 2e-6, CTAU, 1, -d, LAMBDA, 0.0, 0.0))]
+
+
+def build_gpu(n, nsteps, reserve):
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed
+    from sfparticles_amd.fields import Fields
+    qed.set_model("rs")
+    props, field = make_case(n, nsteps)
+    e = sfa.Particles("electron", -1, 1, N=n, props=props)
+    p = sfa.Particles("positron", 1, 1)
+    g = sfa.Particles("photon", 0, 0)
+    e.set_photon(g)
+    p.set_photon(g)
+    g.set_pair(e, p)
+    sim = sfa.Simulation(e, p, g, dt=DT, fields=Fields(terms=field), print_every=None,
+                         photon_threshold=2.0, seed=0)
+    for s in (e, p, g):
+        s._ensure_device()
+    # preallocated SoA capacity (the reference's _extend), so the timed steps
+    # grow no buffer
+    for s, cap in zip((e, p, g), reserve):
+        if cap > s.buffer_size:
+            s._extend(cap - s.N_buffered)
+    sim._build()
+    return sim, (e, p, g)
+
+
+def gpu_run(n, steps, warmup, nsteps_config):
+    from sfparticles_amd import _lib as L
+    if warmup > 0:
+        wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0))
+        wsim.run_steps(warmup)
+        del wsim
+    reserve = (int(n * 1.3) + 4096, int(n * 0.3) + 4096, int(n * 9) + 4096)
+    sim, species = build_gpu(n, nsteps_config, reserve)
+    L.call("sfp_synchronize")
+    t0 = time.perf_counter()
+    sim.run_steps(steps, timing=True)
+    L.call("sfp_synchronize")
+    t1 = time.perf_counter()
+    stats = [sim.stats(i) for i in range(3)]
+    ktime = [sim.kernel_time(i) for i in range(3)]
+    return t1 - t0, stats, ktime, [s.N_buffered for s in species]
+
+
+def cpu_baseline(n, nsteps):
+    """The CPU oracle in reference-structured mode (one OpenMP pass per
+    reference function, serial counts/compaction) on a bounded sample."""
+    import oracle
+    from oracle.sim import OParticles, OSimulation
+    from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
+    props, field = make_case(n, nsteps)
+    e = OParticles("electron", -1, 1, N=n, props=[a.copy() for a in props])
+    p = OParticles("positron", 1, 1)
+    g = OParticles("photon", 0, 0)
+    e.set_photon(g)
+    p.set_photon(g)
+    g.set_pair(e, p)
+    sim = OSimulation(e, p, g, dt=DT, fields=field, photon_threshold=2.0, qed="rs", seed=0,
+                      rs_tables=load_rs_tables(), od_tables=load_od_tables())
+    psteps = 0
+
+    def cb(i):
+        nonlocal psteps
+        psteps += e.Npart + p.Npart + g.Npart
+
+    t0 = time.perf_counter()
+    sim.start(nsteps, call_back=cb, prune=False)
+    t1 = time.perf_counter()
+    return psteps / (t1 - t0), t1 - t0, oracle.num_threads()
+
+
+def load_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def calibrate_copy(n_doubles, reps):
+    """Plain streaming copy (for scaling the HBM byte counters)."""
+    from sfparticles_amd import _lib as L
+    src = L.DeviceArray(8 * n_doubles)
+    dst = L.DeviceArray(8 * n_doubles)
+    for _ in range(reps):
+        L.call("sfp_copy_f64", src.ptr, dst.ptr, n_doubles, None)
+    L.call("sfp_synchronize")
+    print(json.dumps({"copy_bytes_read": 8 * n_doubles, "copy_bytes_written": 8 * n_doubles,
+                      "reps": reps}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=float, default=1e7)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=float, default=1e6)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: the particle step is measured on one "
+              "MI355X (no multi-GPU path is built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(1 << 27, 5)
+        return
+
+    n = int(args.n)
+    nsteps_config = 3000
+    elapsed, stats, ktime, counts = gpu_run(n, args.steps, args.warmup, nsteps_config)
+    psteps = sum(s["psteps"] for s in stats)
+    value = psteps / elapsed
+
+    # dominant kernel: largest accumulated time
+    names = ["electron", "positron", "photon"]
+    kinds = ["lepton_rs", "lepton_rs", "photon_rs"]
+    dom = max(range(3), key=lambda i: ktime[i][0])
+    ms, launches = ktime[dom]
+    st = stats[dom]
+    alg_bytes = BYTES[kinds[dom]] * st["processed"] + BYTES["spawn"] * st["spawned"] * (2 if dom == 2 else 1)
+    per_launch = alg_bytes / max(launches, 1)
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    kname = f"k_advance<{'photon' if dom == 2 else 'lepton'}, RS> ({names[dom]})"
+    traffic = load_traffic(kinds[dom] + "_" + names[dom])
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": kname, "avg_launch_ms": round(ms / max(launches, 1), 4),
+                "alg_bytes_per_launch": int(per_launch),
+                "kernel_time_share": round(ms / max(1e-9, sum(k[0] for k in ktime)), 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cn = int(args.cpu_n)
+        rate, secs, threads = cpu_baseline(cn, args.cpu_steps)
+        cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"CPU oracle, reference-structured passes (OpenMP), FP64; same head-on "
+                         f"workload scaled to {cn} electrons over the {args.cpu_steps} steps "
+                         f"around the collision ({secs:.1f} s)"}
+
+    line = {
+        "metric": "particle-steps/s (1e7 e-, QED on): FP64 tables vs mixed+MLP at <=1% yield err",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"head-on collision, {n} e-, a0 100, gaussian pulse, RS tables, "
+                               f"FP64, {nsteps_config}-step geometry",
+                   "path": "fp64-tables", "electrons": n, "steps_timed": args.steps,
+                   "final_counts": dict(zip(names, counts)),
+                   "particle_steps": {names[i]: stats[i]["psteps"] for i in range(3)},
+                   "kernel_ms": {names[i]: round(ktime[i][0], 2) for i in range(3)}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,310 @@
+/*
+ * sfp.h -- C ABI of libsfp_hip.so, the MI355X (gfx950) implementation of the
+ * sfparticles per-particle step.
+ *
+ * Three groups of entry points:
+ *
+ *  1. Per-operation kernels, one per callable of the reference's three
+ *     backend seams, on caller-owned DEVICE arrays (struct-of-arrays, FP64,
+ *     bool arrays as uint8):
+ *       - backend module  sfparticles/particles.py:14-25 (cpu.py / gpu.py)
+ *       - QED model       sfparticles/particles.py:8-12  (qed/ rejection_sampling.py, optical_depth.py)
+ *       - field kernel    sfparticles/fields.py:26-46 via particles.py:299-323
+ *  2. Species stores: library-owned device buffers with capacity, the
+ *     device-resident counterpart of a Particles object's attributes
+ *     (particles.py:98-146, 476-516).
+ *  3. The fused simulation: the step loop of sfparticles/simulation.py:102-144
+ *     as one fused HIP kernel per species and step (fields, chi, QED sampling,
+ *     hard-photon cut, recoil, stable append of new particles, push).
+ *
+ * Every function returns SFP_OK (0) or an error code; sfp_last_error() gives
+ * a message.  Pointers named *_dev are device pointers; all work is issued on
+ * the HIP stream passed in (NULL = the default stream) and is asynchronous
+ * unless stated otherwise.
+ */
+#ifndef SFP_H
+#define SFP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFP_OK 0
+#define SFP_E_HIP 1      /* a HIP runtime call failed */
+#define SFP_E_ARG 2      /* invalid argument */
+#define SFP_E_CAPACITY 3 /* a store ran out of capacity inside a step */
+#define SFP_E_STATE 4    /* invalid object state */
+
+typedef void* sfp_stream_t; /* hipStream_t */
+
+/* analytic field term kinds (sfparticles/fields.py:69-271) */
+#define SFP_FIELD_STATIC 0
+#define SFP_FIELD_SIMPLE 1
+#define SFP_FIELD_GAUSSIAN 2
+#define SFP_FIELD_FOCUSED 3
+
+/* One field term with the user-level arguments of the reference builder:
+ *  STATIC   (fields.py:268)  p = Ex, Ey, Ez, Bx, By, Bz
+ *  SIMPLE   (fields.py:69)   p = a0, w0, ctau, t_order, direction, x0, wavelength, pol_angle, cep
+ *  GAUSSIAN (fields.py:120)  p = a0, w0, ctau, direction, x0, l0, pol_angle, cep
+ *  FOCUSED  (fields.py:173)  p = a0, w0, ctau, direction, xfoc, x0, l0, pol_angle, cep
+ * A field is a sum of terms, accumulated left to right (fields.py:52-61). */
+typedef struct {
+  int32_t kind;
+  int32_t pad;
+  double p[12];
+} sfp_field;
+
+/* Rejection-sampling tables, HOST pointers (qed/rejection_sampling_tables.py:14-26) */
+typedef struct {
+  const double* int_Ai; /* [z_N] */
+  const double* Aip;    /* [z_N] */
+  int64_t z_N;
+  double z_min, z_max, z_delta;
+} sfp_rs_tables;
+
+/* Optical-depth tables, HOST pointers (qed/optical_depth_tables.py:14-32) */
+typedef struct {
+  const double* photon_total; /* [chi_N] */
+  const double* pair_total;   /* [chi_N] */
+  const double* photon_cdf;   /* [chi_N][delta_N] */
+  const double* pair_cdf;     /* [chi_N][delta_N] */
+  int64_t chi_N;
+  double log_chi_min, log_chi_max, log_chi_delta;
+  int64_t delta_N;
+  double log_delta_min, log_delta_max, log_delta_delta;
+  const double* chi_grid;   /* [chi_N]   10**(log_chi_min + i*log_chi_delta) */
+  const double* delta_grid; /* [delta_N] 10**(log_delta_min + j*log_delta_delta) */
+  double chi_min;           /* 10**log_chi_min */
+  double chi_max;           /* 10**log_chi_max */
+} sfp_od_tables;
+
+const char* sfp_last_error(void);
+int sfp_version(void);
+/* number of compute units of the current device and its name (name may be NULL) */
+int sfp_device_info(int32_t* num_cus, char* name, int32_t name_len);
+
+/* Device memory helpers for callers without their own HIP allocator (the
+ * Python host layer uses these; no counterpart in the reference, whose arrays
+ * are numpy/cupy).  Copies are synchronous. */
+int sfp_malloc(void** dev, int64_t bytes);
+int sfp_free(void* dev);
+int sfp_memcpy_htod(void* dst_dev, const void* src_host, int64_t bytes);
+int sfp_memcpy_dtoh(void* dst_host, const void* src_dev, int64_t bytes);
+int sfp_synchronize(void);
+
+/* Upload QED tables into library-owned device memory; either pointer may be NULL. */
+int sfp_tables_create(const sfp_rs_tables* rs, const sfp_od_tables* od, void** tables);
+int sfp_tables_destroy(void* tables);
+
+/* ---------------------------------------------------------------------------
+ * 1. Per-operation kernels
+ * ------------------------------------------------------------------------- */
+
+/* seam 3: fields.py:26-46 (Fields.field_func), evaluated for [0, n) skipping pruned */
+int sfp_eval_field(const sfp_field* terms, int32_t nterms, const double* x_dev,
+                   const double* y_dev, const double* z_dev, double t, int64_t n,
+                   const uint8_t* pruned_dev, double* Ex_dev, double* Ey_dev, double* Ez_dev,
+                   double* Bx_dev, double* By_dev, double* Bz_dev, sfp_stream_t stream);
+
+/* seam 1: cpu.py:5-19 */
+int sfp_push_position(double* x_dev, double* y_dev, double* z_dev, const double* ux_dev,
+                      const double* uy_dev, const double* uz_dev, const double* inv_gamma_dev,
+                      int64_t n, const uint8_t* pruned_dev, double dt, sfp_stream_t stream);
+/* cpu.py:22-31 */
+int sfp_boris(double* ux_dev, double* uy_dev, double* uz_dev, double* inv_gamma_dev,
+              const double* Ex_dev, const double* Ey_dev, const double* Ez_dev,
+              const double* Bx_dev, const double* By_dev, const double* Bz_dev, double q,
+              int64_t n, const uint8_t* pruned_dev, double dt, sfp_stream_t stream);
+/* cpu.py:33-42 */
+int sfp_boris_tbmt(double* ux_dev, double* uy_dev, double* uz_dev, double* inv_gamma_dev,
+                   double* sx_dev, double* sy_dev, double* sz_dev, const double* Ex_dev,
+                   const double* Ey_dev, const double* Ez_dev, const double* Bx_dev,
+                   const double* By_dev, const double* Bz_dev, double q, double ae, int64_t n,
+                   const uint8_t* pruned_dev, double dt, sfp_stream_t stream);
+/* cpu.py:44-51 */
+int sfp_LL_push(double* ux_dev, double* uy_dev, double* uz_dev, double* inv_gamma_dev,
+                const double* chi_dev, int64_t n, const uint8_t* pruned_dev, double dt,
+                sfp_stream_t stream);
+/* cpu.py:53-60 */
+int sfp_CLL_push(double* ux_dev, double* uy_dev, double* uz_dev, double* inv_gamma_dev,
+                 const double* chi_dev, int64_t n, const uint8_t* pruned_dev, double dt,
+                 sfp_stream_t stream);
+/* cpu.py:62-75 */
+int sfp_update_chi(const double* Ex_dev, const double* Ey_dev, const double* Ez_dev,
+                   const double* Bx_dev, const double* By_dev, const double* Bz_dev,
+                   const double* ux_dev, const double* uy_dev, const double* uz_dev,
+                   const double* inv_gamma_dev, double* chi_dev, int64_t n,
+                   const uint8_t* pruned_dev, sfp_stream_t stream);
+/* cpu.py:227-232 */
+int sfp_pick_hard_photon(uint8_t* event_dev, const double* delta_dev,
+                         const double* inv_gamma_dev, double threshold, int64_t n,
+                         sfp_stream_t stream);
+/* cpu.py:78-88 */
+int sfp_photon_recoil(double* ux_dev, double* uy_dev, double* uz_dev, double* inv_gamma_dev,
+                      const uint8_t* event_dev, const double* delta_dev, int64_t n,
+                      const uint8_t* pruned_dev, sfp_stream_t stream);
+/* cpu.py:90-118; additionally writes id next_id+k and zeroes chi/tau of the
+ * new slot when those pointers are non-NULL */
+int sfp_create_photon(const double* x_src, const double* y_src, const double* z_src,
+                      const double* ux_src, const double* uy_src, const double* uz_src,
+                      double* x_dst, double* y_dst, double* z_dst, double* ux_dst,
+                      double* uy_dst, double* uz_dst, double* inv_gamma_dst,
+                      uint8_t* photon_pruned_dst, const int64_t* event_index_dev,
+                      const double* photon_delta_dev, int64_t n_buffered, int64_t n_photon,
+                      uint64_t* id_dst, uint64_t next_id, double* chi_dst, double* tau_dst,
+                      sfp_stream_t stream);
+/* cpu.py:120-191; same additions, plus spin of the new lepton set to 0 */
+int sfp_create_pair(const double* x_src, const double* y_src, const double* z_src,
+                    const double* ux_src, const double* uy_src, const double* uz_src,
+                    uint8_t* photon_pruned, double* x_dst, double* y_dst, double* z_dst,
+                    double* ux_dst, double* uy_dst, double* uz_dst, double* inv_gamma_dst,
+                    uint8_t* pair_pruned_dst, const int64_t* event_index_dev,
+                    const double* pair_delta_dev, int64_t n_buffered, int64_t n_pair,
+                    int32_t inverse_delta, uint64_t* id_dst, uint64_t next_id, double* chi_dst,
+                    double* tau_dst, double* sx_dst, double* sy_dst, double* sz_dst,
+                    sfp_stream_t stream);
+/* cpu.py:193-216: ascending indices of the true flags into event_index_dev
+ * (room for n entries); the count goes to *count_host (synchronous). */
+int sfp_find_event_index(const uint8_t* event_dev, int64_t n, int64_t* event_index_dev,
+                         int64_t* count_host, sfp_stream_t stream);
+/* cpu.py:218-225: number of true flags in [0, n) (synchronous) */
+int sfp_bool_sum(const uint8_t* flags_dev, int64_t n, int64_t* count_host, sfp_stream_t stream);
+
+/* 1/gamma from the momentum, [0, n) skipping pruned: 1/sqrt(1+u^2) for
+ * leptons (photon == 0), 1/|u| for photons (particles.py:118-123).  No
+ * counterpart in the reference, which stores inv_gamma; the fused step keeps
+ * it in registers and this rebuilds the array on request. */
+int sfp_update_inv_gamma(const double* ux_dev, const double* uy_dev, const double* uz_dev,
+                         double* inv_gamma_dev, int64_t n, const uint8_t* pruned_dev,
+                         int32_t photon, sfp_stream_t stream);
+
+/* seam 2: qed/rejection_sampling.py:68-85 and :87-101.  Draws are
+ * Philox4x32-10 keyed by seed with counter (id, step, tag, slot). */
+int sfp_photon_from_rejection_sampling(const double* inv_gamma_dev, const double* chi_dev,
+                                       double dt, int64_t n, const uint8_t* pruned_dev,
+                                       uint8_t* event_dev, double* delta_dev,
+                                       const uint64_t* id_dev, uint64_t seed, uint64_t step,
+                                       uint32_t tag, uint32_t slot, void* tables,
+                                       sfp_stream_t stream);
+int sfp_pair_from_rejection_sampling(const double* inv_gamma_dev, const double* chi_dev,
+                                     double dt, int64_t n, const uint8_t* pruned_dev,
+                                     uint8_t* event_dev, double* delta_dev, const uint64_t* id_dev,
+                                     uint64_t seed, uint64_t step, uint32_t tag, uint32_t slot,
+                                     void* tables, sfp_stream_t stream);
+/* qed/optical_depth.py:15-62 and :64-110 */
+int sfp_update_tau_e(double* tau_dev, const double* inv_gamma_dev, const double* chi_dev,
+                     double dt, int64_t n, const uint8_t* pruned_dev, uint8_t* event_dev,
+                     double* delta_dev, const uint64_t* id_dev, uint64_t seed, uint64_t step,
+                     uint32_t tag, uint32_t slot, void* tables, sfp_stream_t stream);
+int sfp_update_tau_gamma(double* tau_dev, const double* inv_gamma_dev, const double* chi_dev,
+                         double dt, int64_t n, const uint8_t* pruned_dev, uint8_t* event_dev,
+                         double* delta_dev, const uint64_t* id_dev, uint64_t seed, uint64_t step,
+                         uint32_t tag, uint32_t slot, void* tables, sfp_stream_t stream);
+
+/* Elementary functions of the device math library on n device values, for
+ * checking that the device and the host compute them identically (no
+ * counterpart in the reference).  fn: 0 exp, 1 log, 2 sin, 3 cos, 4 atan,
+ * 5 pow(x, y), 6 exp10, 7 log10, 8 sqrt, 9 x / y. */
+int sfp_math_eval(int32_t fn, const double* x_dev, const double* y_dev, double* out_dev,
+                  int64_t n, sfp_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * 2. Species stores (device SoA with capacity)
+ * ------------------------------------------------------------------------- */
+enum sfp_attr {
+  SFP_X = 0, SFP_Y, SFP_Z, SFP_UX, SFP_UY, SFP_UZ, /* float64 */
+  SFP_ID,                                          /* uint64 */
+  SFP_PRUNED,                                      /* uint8 */
+  SFP_TAU, SFP_SX, SFP_SY, SFP_SZ,                 /* float64, optional */
+  SFP_INV_GAMMA, SFP_CHI,                          /* float64, auxiliary */
+  SFP_EX, SFP_EY, SFP_EZ, SFP_BX, SFP_BY, SFP_BZ,  /* float64, auxiliary */
+  SFP_EVENT,                                       /* uint8, auxiliary */
+  SFP_DELTA,                                       /* float64, auxiliary */
+  SFP_NATTR
+};
+#define SFP_STORE_TAU 1  /* optical-depth attribute */
+#define SFP_STORE_SPIN 2 /* sx, sy, sz */
+
+int sfp_store_create(int32_t flags, int64_t capacity, void** store);
+int sfp_store_destroy(void* store);
+/* grow to at least `capacity` (content kept; new slots pruned, events false) */
+int sfp_store_reserve(void* store, int64_t capacity);
+int sfp_store_capacity(void* store, int64_t* capacity);
+/* device pointer of an attribute; auxiliary attributes are allocated on first use */
+int sfp_store_ptr(void* store, int32_t attr, void** dev_ptr);
+int sfp_store_set_count(void* store, int64_t n_buffered, uint64_t next_id);
+/* synchronous */
+int sfp_store_get_count(void* store, int64_t* n_buffered, uint64_t* next_id);
+/* synchronous host<->device copies of elements [offset, offset+n) */
+int sfp_store_upload(void* store, int32_t attr, const void* host, int64_t offset, int64_t n);
+int sfp_store_download(void* store, int32_t attr, void* host, int64_t offset, int64_t n);
+/* particles.py:503-516: stable compaction of every allocated attribute over
+ * the non-pruned particles; the new count is returned (synchronous) */
+int sfp_store_prune(void* store, int64_t* n_kept);
+
+/* ---------------------------------------------------------------------------
+ * 3. Fused simulation (simulation.py:12-161)
+ * ------------------------------------------------------------------------- */
+#define SFP_RR_NONE 0
+#define SFP_RR_PHOTON 1
+#define SFP_RR_LL 2
+#define SFP_RR_CLL 3
+
+#define SFP_QED_RS 0 /* per-step probability, rejection sampling */
+#define SFP_QED_OD 1 /* optical depth */
+
+typedef struct {
+  void* store;
+  double q;        /* charge in C (particles.py:79) */
+  double m;        /* mass in units of m_e; 0 for photons */
+  double ae;       /* anomalous moment (spin species) */
+  int32_t rr;      /* SFP_RR_* */
+  int32_t push;    /* particles.py:82 */
+  int32_t spin;    /* SpinParticles */
+  int32_t photon_dst; /* index of the photon species this lepton emits into, or -1 */
+  int32_t pair_e_dst; /* photon: index of the electron species, or -1 */
+  int32_t pair_p_dst; /* photon: index of the positron species, or -1 */
+  int32_t pad[2];
+} sfp_species_desc;
+
+typedef struct {
+  double dt;
+  double photon_threshold; /* simulation.py:19 */
+  uint64_t seed;
+  int32_t qed_model;       /* SFP_QED_* */
+  int32_t pad;
+} sfp_sim_params;
+
+int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies, const sfp_field* terms,
+                   int32_t nterms, void* tables, const sfp_sim_params* params, void** sim);
+int sfp_sim_destroy(void* sim);
+/* Advance nsteps steps from time *t (updated: t += dt per step, as
+ * simulation.py:143) and step index step0 (the Philox counter).  Counts stay
+ * on the device; capacities grow as needed between steps. */
+#define SFP_RUN_TIMING 1 /* record HIP events around each species kernel */
+int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t flags);
+/* accumulated kernel time (ms) and launch count of one species' step kernel
+ * since the last reset (requires SFP_RUN_TIMING); synchronous */
+int sfp_sim_kernel_time(void* sim, int32_t species, double* ms, int64_t* launches);
+int sfp_sim_reset_timing(void* sim);
+/* Counters of one species since the last reset (synchronous):
+ * stats[0] particle-steps (live particles after each step, the reference's
+ *          ns/particle denominator, simulation.py:150-151)
+ * stats[1] particles read by the species' step kernel
+ * stats[2] particles that kernel appended (to each destination)
+ * stats[3] current buffered count */
+int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
+int sfp_sim_reset_stats(void* sim);
+
+/* dst[i] = src[i] for n doubles: a plain streaming copy kernel used to
+ * calibrate HBM byte counters (no counterpart in the reference) */
+int sfp_copy_f64(const double* src_dev, double* dst_dev, int64_t n, sfp_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFP_H */

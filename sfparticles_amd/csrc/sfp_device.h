@@ -1,0 +1,518 @@
+/*
+ * sfp_device.h -- per-particle physics of the particle step for gfx950.
+ *
+ * Every formula follows the reference's Python expression operator by
+ * operator (same association, no contraction: compile with
+ * -ffp-contract=off) so that FP64 results are reproducible on the host.
+ * References are to the sfparticles source tree.
+ */
+#ifndef SFP_DEVICE_H
+#define SFP_DEVICE_H
+
+#include <stdint.h>
+
+#include "sfp_math.h"
+#include "../../include/sfp.h"
+
+/* CODATA 2018 (see sfparticles_amd/constants.py) */
+#define SFP_C 299792458.0
+#define SFP_ME 9.1093837015e-31
+#define SFP_E 1.602176634e-19
+#define SFP_HBAR 1.0545718176461565e-34
+#define SFP_EPS0 8.8541878128e-12
+#define SFP_ALPHA 0.0072973525693
+#define SFP_PI 3.141592653589793
+
+#define SFP_MAX_TERMS 8
+
+/* ------------------------------------------------------------- fields -- */
+/* Device form of one field term: the closure constants of the reference's
+ * builders, precomputed once on the host with the same expressions. */
+typedef struct {
+  int32_t kind;  /* SFP_FIELD_* */
+  int32_t n;     /* t_order (simple) */
+  double c[24];
+} sfp_term_dev;
+
+typedef struct {
+  int32_t nterms;
+  int32_t pad;
+  sfp_term_dev t[SFP_MAX_TERMS];
+} sfp_fieldset;
+
+/* term constant slots */
+enum {
+  /* simple (fields.py:69-118) */
+  FS_E0 = 0, FS_K0, FS_W0SQ, FS_KCTN, FS_DIR, FS_DX0, FS_CEP, FS_COSP, FS_SINP,
+};
+enum {
+  /* gaussian (fields.py:120-170) */
+  FG_E0 = 0, FG_K0, FG_ZR, FG_W0, FG_KCT2, FG_X0, FG_DC, FG_CEP, FG_COSP, FG_SINP, FG_DIR,
+};
+enum {
+  /* focused (fields.py:173-265) */
+  FF_E0 = 0, FF_K0, FF_ZR, FF_W0, FF_EPS, FF_E2, FF_E3, FF_E4, FF_E5, FF_KCT2, FF_DIR, FF_XFOC,
+  FF_DX0, FF_CEP, FF_COSP, FF_SINP,
+};
+
+/* host: user arguments -> closure constants */
+static inline void sfp_compile_term(const sfp_field* u, sfp_term_dev* d) {
+  const double* p = u->p;
+  for (int i = 0; i < 24; ++i) d->c[i] = 0.0;
+  d->kind = u->kind;
+  d->n = 0;
+  switch (u->kind) {
+    case SFP_FIELD_STATIC:
+      for (int i = 0; i < 6; ++i) d->c[i] = p[i];
+      break;
+    case SFP_FIELD_SIMPLE: {
+      double a0 = p[0], w0 = p[1], ctau = p[2];
+      int t_order = (int)p[3];
+      double direction = p[4], x0 = p[5], wavelength = p[6], pol_angle = p[7], cep = p[8];
+      double omega0 = 2 * SFP_PI * SFP_C / wavelength;
+      double k0 = omega0 / SFP_C;
+      double a0_norm = SFP_E / (SFP_ME * SFP_C * omega0);
+      d->n = t_order;
+      d->c[FS_E0] = a0 / a0_norm;
+      d->c[FS_K0] = k0;
+      d->c[FS_W0SQ] = w0 * w0;
+      d->c[FS_KCTN] = sfp_powi(k0 * ctau, t_order);
+      d->c[FS_DIR] = direction;
+      d->c[FS_DX0] = direction * x0;
+      d->c[FS_CEP] = cep;
+      d->c[FS_COSP] = sfp_cos(pol_angle);
+      d->c[FS_SINP] = sfp_sin(pol_angle);
+      break;
+    }
+    case SFP_FIELD_GAUSSIAN: {
+      double a0 = p[0], w0 = p[1], ctau = p[2], direction = p[3], x0 = p[4], l0 = p[5];
+      double pol_angle = p[6], cep = p[7];
+      double omega0 = 2 * SFP_PI * SFP_C / l0;
+      double k0 = 2 * SFP_PI / l0;
+      double zR = SFP_PI * (w0 * w0) / l0;
+      double a0_norm = SFP_E / (SFP_ME * SFP_C * omega0);
+      double kct = k0 * ctau;
+      d->c[FG_E0] = a0 / a0_norm;
+      d->c[FG_K0] = k0;
+      d->c[FG_ZR] = zR;
+      d->c[FG_W0] = w0;
+      d->c[FG_KCT2] = kct * kct;
+      d->c[FG_X0] = x0;
+      d->c[FG_DC] = direction * SFP_C;
+      d->c[FG_CEP] = cep;
+      d->c[FG_COSP] = sfp_cos(pol_angle);
+      d->c[FG_SINP] = sfp_sin(pol_angle);
+      d->c[FG_DIR] = direction;
+      break;
+    }
+    default: { /* focused */
+      double a0 = p[0], w0 = p[1], ctau = p[2], direction = p[3], xfoc = p[4], x0 = p[5];
+      double l0 = p[6], pol_angle = p[7], cep = p[8];
+      double omega0 = 2 * SFP_PI * SFP_C / l0;
+      double k0 = 2 * SFP_PI / l0;
+      double zR = SFP_PI * (w0 * w0) / l0;
+      double eps = w0 / zR;
+      double a0_norm = SFP_E / (SFP_ME * SFP_C * omega0);
+      double kct = k0 * ctau;
+      d->c[FF_E0] = a0 / a0_norm;
+      d->c[FF_K0] = k0;
+      d->c[FF_ZR] = zR;
+      d->c[FF_W0] = w0;
+      d->c[FF_EPS] = eps;
+      d->c[FF_E2] = sfp_powi(eps, 2);
+      d->c[FF_E3] = sfp_powi(eps, 3);
+      d->c[FF_E4] = sfp_powi(eps, 4);
+      d->c[FF_E5] = sfp_powi(eps, 5);
+      d->c[FF_KCT2] = kct * kct;
+      d->c[FF_DIR] = direction;
+      d->c[FF_XFOC] = xfoc;
+      d->c[FF_DX0] = direction * x0;
+      d->c[FF_CEP] = cep;
+      d->c[FF_COSP] = sfp_cos(pol_angle);
+      d->c[FF_SINP] = sfp_sin(pol_angle);
+      break;
+    }
+  }
+}
+
+/* fields.py:69-118 (per point) */
+SFP_HD void sfp_field_simple(const sfp_term_dev* d, double x, double y, double z, double ct,
+                             double* F) {
+  const double* c = d->c;
+  double direction = c[FS_DIR];
+  x = direction * x;
+  double r2 = y * y + z * z;
+  double phi = c[FS_K0] * (x - c[FS_DX0] - ct);
+  double E = c[FS_E0] * sfp_cos(phi + c[FS_CEP]) * sfp_exp(-r2 / c[FS_W0SQ]) *
+             sfp_exp(-sfp_powi(phi, d->n) / c[FS_KCTN]);
+  double Ey = E * c[FS_COSP];
+  double Ez = E * c[FS_SINP];
+  F[0] = 0.0;
+  F[1] = Ey;
+  F[2] = Ez;
+  F[3] = 0.0;
+  F[4] = -Ez / SFP_C * direction;
+  F[5] = Ey / SFP_C * direction;
+}
+
+/* fields.py:120-170; the curvature term is 0 on the focal plane x == 0 */
+SFP_HD void sfp_field_gaussian(const sfp_term_dev* d, double x, double y, double z, double t,
+                               double* F) {
+  const double* c = d->c;
+  double zR = c[FG_ZR], k0 = c[FG_K0], w0 = c[FG_W0];
+  double r2 = y * y + z * z;
+  double xz = x / zR;
+  double wx = w0 * sfp_sqrt(1 + xz * xz);
+  double gouy = sfp_atan(x / zR);
+  double curv;
+  if (x == 0.0) {
+    curv = 0.0;
+  } else {
+    double zx = zR / x;
+    double Rx = x * (1 + zx * zx);
+    curv = k0 * r2 / 2 / Rx;
+  }
+  double phi = k0 * (x - c[FG_X0] - c[FG_DC] * t) + curv - gouy;
+  double E = c[FG_E0] * sfp_cos(phi + c[FG_CEP]) * w0 / wx * sfp_exp(-r2 / (wx * wx)) *
+             sfp_exp(-(phi * phi) / c[FG_KCT2]);
+  double Ey = E * c[FG_COSP];
+  double Ez = E * c[FG_SINP];
+  double direction = c[FG_DIR];
+  F[0] = 0.0;
+  F[1] = Ey;
+  F[2] = Ez;
+  F[3] = 0.0;
+  F[4] = -Ez / SFP_C * direction;
+  F[5] = Ey / SFP_C * direction;
+}
+
+/* fields.py:173-265 */
+SFP_HD void sfp_field_focused(const sfp_term_dev* d, double x, double y, double z, double ct,
+                              double* F) {
+  const double* c = d->c;
+  double zR = c[FF_ZR], k0 = c[FF_K0], w0 = c[FF_W0], eps = c[FF_EPS];
+  double e2 = c[FF_E2], e3 = c[FF_E3], e4 = c[FF_E4], e5 = c[FF_E5];
+  double direction = c[FF_DIR];
+  x = direction * (x - c[FF_XFOC]);
+  double r2 = y * y + z * z;
+  double xz = x / zR;
+  double wx = w0 * sfp_sqrt(1 + xz * xz);
+  double psiG = sfp_atan(x / zR);
+  double psiR = k0 * x * r2 / 2 / (x * x + zR * zR);
+  double psiP = k0 * (x - c[FF_DX0] - ct);
+  double cep = c[FF_CEP];
+  double psi = psiG + psiR + psiP + cep;
+  double rho = sfp_sqrt(r2) / w0;
+  double xi = y / w0;
+  double nu = z / w0;
+  double eta = 1 / sfp_sqrt(1 + xz * xz);
+  double rho2 = sfp_powi(rho, 2), rho4 = sfp_powi(rho, 4), rho6 = sfp_powi(rho, 6),
+         rho8 = sfp_powi(rho, 8);
+  double pc = psi - cep;
+  double E = c[FF_E0] * w0 / wx * sfp_exp(-r2 / (wx * wx)) * sfp_exp(-(pc * pc) / c[FF_KCT2]);
+  double S0 = sfp_sin(psi);
+  double S2 = sfp_powi(eta, 2) * sfp_sin(psi + 2 * psiG);
+  double S3 = sfp_powi(eta, 3) * sfp_sin(psi + 3 * psiG);
+  double S4 = sfp_powi(eta, 4) * sfp_sin(psi + 4 * psiG);
+  double S5 = sfp_powi(eta, 5) * sfp_sin(psi + 5 * psiG);
+  double S6 = sfp_powi(eta, 6) * sfp_sin(psi + 6 * psiG);
+  double C1 = sfp_powi(eta, 1) * sfp_cos(psi + 1 * psiG);
+  double C2 = sfp_powi(eta, 2) * sfp_cos(psi + 2 * psiG);
+  double C3 = sfp_powi(eta, 3) * sfp_cos(psi + 3 * psiG);
+  double C4 = sfp_powi(eta, 4) * sfp_cos(psi + 4 * psiG);
+  double C5 = sfp_powi(eta, 5) * sfp_cos(psi + 5 * psiG);
+  double C6 = sfp_powi(eta, 6) * sfp_cos(psi + 6 * psiG);
+  double C7 = sfp_powi(eta, 7) * sfp_cos(psi + 7 * psiG);
+  double xi2 = xi * xi;
+  double Ex_ = E * xi *
+               (eps * C1 + e3 * (-C2 / 2 + rho2 * C3 - rho4 * C4 / 4) +
+                e5 * (-3.0 / 8 * C3 - 3.0 / 8 * rho2 * C4 + 17.0 / 16 * rho4 * C5 - 3.0 / 8 * rho6 * C6 +
+                      rho8 * C7 / 32));
+  double Ey_ = E * (S0 + e2 * (xi2 * S2 - rho4 * S3 / 4) +
+                    e4 * (S2 / 8 - rho2 * S3 / 4 - rho2 * (rho2 - 16 * xi2) * S4 / 16 -
+                          rho4 * (rho2 + 2 * xi2) * S5 / 8 + rho8 * S6 / 32));
+  double Ez_ = E * xi * nu * (e2 * S2 + e4 * (rho2 * S4 - rho4 * S5 / 4));
+  double Bx_ = E / SFP_C * nu *
+               (eps * C1 + e3 * (C2 / 2 + rho2 * C3 / 2 - rho4 * C4 / 4) +
+                e5 * (3.0 / 8 * C3 + 3.0 / 8 * rho2 * C4 + 3.0 / 16 * rho4 * C5 - rho6 * C6 / 4 +
+                      rho8 * C7 / 32));
+  double By_ = 0.0;
+  double Bz_ = E / SFP_C *
+               (S0 + e2 * (rho2 * S2 / 2 - rho4 * S3 / 4) +
+                e4 * (-S2 / 8 + rho2 * S3 / 4 + 5.0 / 16 * rho4 * S4 - rho6 * S5 / 4 + rho8 * S6 / 32));
+  double cp = c[FF_COSP], sp = c[FF_SINP];
+  F[0] = Ex_;
+  F[1] = Ey_ * cp - Ez_ * sp;
+  F[2] = Ey_ * sp + Ez_ * cp;
+  F[3] = Bx_;
+  F[4] = (By_ * cp - Bz_ * sp) * direction;
+  F[5] = (By_ * sp + Bz_ * cp) * direction;
+}
+
+/* Fields.__add__ (fields.py:52-61): left fold over the terms */
+SFP_HD void sfp_field_eval(const sfp_fieldset* fs, double x, double y, double z, double t,
+                           double* F) {
+  double ct = SFP_C * t;
+  for (int j = 0; j < fs->nterms; ++j) {
+    double G[6];
+    const sfp_term_dev* d = &fs->t[j];
+    switch (d->kind) {
+      case SFP_FIELD_STATIC:
+        for (int i = 0; i < 6; ++i) G[i] = d->c[i];
+        break;
+      case SFP_FIELD_SIMPLE: sfp_field_simple(d, x, y, z, ct, G); break;
+      case SFP_FIELD_GAUSSIAN: sfp_field_gaussian(d, x, y, z, t, G); break;
+      default: sfp_field_focused(d, x, y, z, ct, G); break;
+    }
+    if (j == 0) {
+      for (int i = 0; i < 6; ++i) F[i] = G[i];
+    } else {
+      for (int i = 0; i < 6; ++i) F[i] = F[i] + G[i];
+    }
+  }
+  if (fs->nterms == 0)
+    for (int i = 0; i < 6; ++i) F[i] = 0.0;
+}
+
+/* ------------------------------------------------------------- pusher -- */
+/* 1/gamma of a lepton from its normalised momentum (particles.py:119) */
+SFP_HD double sfp_inv_gamma_lepton(double ux, double uy, double uz) {
+  return 1 / sfp_sqrt(1 + ux * ux + uy * uy + uz * uz);
+}
+
+/* 1/|u| of a photon (particles.py:122, cpu.py:116) */
+SFP_HD double sfp_inv_gamma_photon(double ux, double uy, double uz) {
+  return 1.0 / sfp_sqrt(ux * ux + uy * uy + uz * uz);
+}
+
+/* cpu.py:5-19 (one particle); cdt = c * dt */
+SFP_HD void sfp_push_position(double* x, double* y, double* z, double ux, double uy, double uz,
+                              double inv_gamma, double cdt) {
+  *x += cdt * inv_gamma * ux;
+  *y += cdt * inv_gamma * uy;
+  *z += cdt * inv_gamma * uz;
+}
+
+/* inline.py:4-36 */
+SFP_HD void sfp_boris(double* pux, double* puy, double* puz, double* pig, double Ex, double Ey,
+                      double Ez, double Bx, double By, double Bz, double q, double dt) {
+  double efactor = q * dt / (2 * SFP_ME * SFP_C);
+  double bfactor = q * dt / (2 * SFP_ME);
+  double ux_minus = *pux + efactor * Ex;
+  double uy_minus = *puy + efactor * Ey;
+  double uz_minus = *puz + efactor * Ez;
+  double inv_gamma_minus =
+      1 / sfp_sqrt(1 + ux_minus * ux_minus + uy_minus * uy_minus + uz_minus * uz_minus);
+  double Tx = bfactor * Bx * inv_gamma_minus;
+  double Ty = bfactor * By * inv_gamma_minus;
+  double Tz = bfactor * Bz * inv_gamma_minus;
+  double ux_prime = ux_minus + uy_minus * Tz - uz_minus * Ty;
+  double uy_prime = uy_minus + uz_minus * Tx - ux_minus * Tz;
+  double uz_prime = uz_minus + ux_minus * Ty - uy_minus * Tx;
+  double Tfactor = 2 / (1 + Tx * Tx + Ty * Ty + Tz * Tz);
+  double Sx = Tfactor * Tx, Sy = Tfactor * Ty, Sz = Tfactor * Tz;
+  double ux_plus = ux_minus + uy_prime * Sz - uz_prime * Sy;
+  double uy_plus = uy_minus + uz_prime * Sx - ux_prime * Sz;
+  double uz_plus = uz_minus + ux_prime * Sy - uy_prime * Sx;
+  double ux_ = ux_plus + efactor * Ex;
+  double uy_ = uy_plus + efactor * Ey;
+  double uz_ = uz_plus + efactor * Ez;
+  *pux = ux_;
+  *puy = uy_;
+  *puz = uz_;
+  *pig = 1 / sfp_sqrt(1 + ux_ * ux_ + uy_ * uy_ + uz_ * uz_);
+}
+
+/* inline.py:39-100 (the reference's B_eff coefficients, kept as written) */
+SFP_HD void sfp_boris_tbmt(double* pux, double* puy, double* puz, double* pig, double* psx,
+                           double* psy, double* psz, double Ex, double Ey, double Ez, double Bx,
+                           double By, double Bz, double q, double ae, double dt) {
+  double efactor = q * dt / (2 * SFP_ME * SFP_C);
+  double bfactor = q * dt / (2 * SFP_ME);
+  double ux_minus = *pux + efactor * Ex;
+  double uy_minus = *puy + efactor * Ey;
+  double uz_minus = *puz + efactor * Ez;
+  double u_dot_B = ux_minus * Bx + uy_minus * By + uz_minus * Bz;
+  double gamma_minus = sfp_sqrt(1.0 + ux_minus * ux_minus + uy_minus * uy_minus + uz_minus * uz_minus);
+  double inv_gamma_minus = 1. / gamma_minus;
+  double Bx_eff = (ae + inv_gamma_minus) * Bx - ae * gamma_minus / (gamma_minus + 1) * u_dot_B * ux_minus -
+                  (ae + 1 / (gamma_minus + 1)) * (uy_minus * Ez - uz_minus * Ey);
+  double By_eff = (ae + inv_gamma_minus) * By - ae * gamma_minus / (gamma_minus + 1) * u_dot_B * uy_minus -
+                  (ae + 1 / (gamma_minus + 1)) * (uz_minus * Ex - ux_minus * Ez);
+  double Bz_eff = (ae + inv_gamma_minus) * Bz - ae * gamma_minus / (gamma_minus + 1) * u_dot_B * uz_minus -
+                  (ae + 1 / (gamma_minus + 1)) * (ux_minus * Ey - uy_minus * Ex);
+  double Tx = bfactor * Bx_eff, Ty = bfactor * By_eff, Tz = bfactor * Bz_eff;
+  double T2 = Tx * Tx + Ty * Ty + Tz * Tz;
+  double Sx = 2 * Tx / (1. + T2), Sy = 2 * Ty / (1. + T2), Sz = 2 * Tz / (1. + T2);
+  double sx = *psx, sy = *psy, sz = *psz;
+  double sx_plus = sx + sy * Tz - sz * Ty;
+  double sy_plus = sy + sz * Tx - sx * Tz;
+  double sz_plus = sz + sx * Ty - sy * Tx;
+  double sx_new = sx + sy_plus * Sz - sz_plus * Sy;
+  double sy_new = sy + sz_plus * Sx - sx_plus * Sz;
+  double sz_new = sz + sx_plus * Sy - sy_plus * Sx;
+  Tx = bfactor * Bx * inv_gamma_minus;
+  Ty = bfactor * By * inv_gamma_minus;
+  Tz = bfactor * Bz * inv_gamma_minus;
+  double ux_prime = ux_minus + uy_minus * Tz - uz_minus * Ty;
+  double uy_prime = uy_minus + uz_minus * Tx - ux_minus * Tz;
+  double uz_prime = uz_minus + ux_minus * Ty - uy_minus * Tx;
+  double Tfactor = 2 / (1 + Tx * Tx + Ty * Ty + Tz * Tz);
+  Sx = Tfactor * Tx;
+  Sy = Tfactor * Ty;
+  Sz = Tfactor * Tz;
+  double ux_plus = ux_minus + uy_prime * Sz - uz_prime * Sy;
+  double uy_plus = uy_minus + uz_prime * Sx - ux_prime * Sz;
+  double uz_plus = uz_minus + ux_prime * Sy - uy_prime * Sx;
+  double ux_new = ux_plus + efactor * Ex;
+  double uy_new = uy_plus + efactor * Ey;
+  double uz_new = uz_plus + efactor * Ez;
+  *pux = ux_new;
+  *puy = uy_new;
+  *puz = uz_new;
+  *pig = 1 / sfp_sqrt(1 + ux_new * ux_new + uy_new * uy_new + uz_new * uz_new);
+  *psx = sx_new;
+  *psy = sy_new;
+  *psz = sz_new;
+}
+
+/* inline.py:105: -(2/3) e^2/(4 pi eps0) m c / hbar^2 * dt */
+SFP_HD double sfp_ll_factor(double dt) {
+  return -2.0 / 3 / (4 * SFP_PI * SFP_EPS0) * (SFP_E * SFP_E) * SFP_ME * SFP_C /
+         (SFP_HBAR * SFP_HBAR) * dt;
+}
+
+/* inline.py:104-111 (factor = sfp_ll_factor) and, with factor multiplied by
+ * g(chi), inline.py:114-123 */
+SFP_HD void sfp_ll_apply(double* pux, double* puy, double* puz, double* pig, double chi,
+                         double factor) {
+  double c2 = chi * chi;
+  double ig = *pig;
+  double ux_ = *pux + factor * c2 * *pux * ig;
+  double uy_ = *puy + factor * c2 * *puy * ig;
+  double uz_ = *puz + factor * c2 * *puz * ig;
+  *pux = ux_;
+  *puy = uy_;
+  *puz = uz_;
+  *pig = 1 / sfp_sqrt(1 + ux_ * ux_ + uy_ * uy_ + uz_ * uz_);
+}
+
+/* inline.py:115: quantum correction g(chi) */
+SFP_HD double sfp_cll_g(double chi) {
+  return sfp_pow(1 + 4.8 * (1 + chi) * sfp_log(1 + 1.7 * chi) + 2.44 * (chi * chi), -2.0 / 3);
+}
+
+/* inline.py:126-134 */
+SFP_HD double sfp_chi(double Ex, double Ey, double Ez, double Bx, double By, double Bz, double ux,
+                      double uy, double uz, double inv_gamma) {
+  double factor = SFP_E * SFP_HBAR / ((SFP_ME * SFP_ME) * (SFP_C * SFP_C * SFP_C));
+  double gamma = 1.0 / inv_gamma;
+  double a = gamma * Ex + (uy * Bz - uz * By) * SFP_C;
+  double b = gamma * Ey + (uz * Bx - ux * Bz) * SFP_C;
+  double c = gamma * Ez + (ux * By - uy * Bx) * SFP_C;
+  double d = ux * Ex + uy * Ey + uz * Ez;
+  return factor * sfp_sqrt(a * a + b * b + c * c - d * d);
+}
+
+/* ------------------------------------------------------- QED lookups -- */
+/* One rejection-sampling table interval: values at the left node and the
+ * slopes (T[i+1]-T[i])/dz, for int Ai and Ai'. */
+typedef struct {
+  double ia, ka, ap, kp;
+} sfp_rs_node;
+
+/* rejection_sampling_tables.py:32-53 (photon, sign = +1) and :56-78 (pair,
+ * sign = -1).  z > z_max -> 0 (the reference reads past the table for photons). */
+SFP_HD double sfp_rs_rate(const sfp_rs_node* nodes, int64_t z_N, double z_min, double z_max,
+                          double z_delta, double chi_parent, double delta, int photon) {
+  /* photon: chi_a = chi_gamma, chi_b = chi_e', z = (chi_gamma/chi_e/chi_e')^(2/3)
+   * pair:   chi_a = chi_e,     chi_b = chi_e', z = (chi_gamma/chi_e/chi_e')^(2/3) */
+  double chi_a = delta * chi_parent;
+  double chi_b = chi_parent - chi_a;
+  double z = photon ? sfp_pow(chi_a / chi_parent / chi_b, 2.0 / 3)
+                    : sfp_pow(chi_parent / chi_a / chi_b, 2.0 / 3);
+  double factor = photon ? (-SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR)
+                         : (SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR);
+  if (!(z <= z_max)) return (z != z) ? z : 0.0;
+  int64_t idx = (int64_t)((z - z_min) / z_delta);
+  if (idx > z_N - 2) idx = z_N - 2;
+  double z_left = z_min + (double)idx * z_delta;
+  sfp_rs_node nd = nodes[idx];
+  double int_Ai_ = nd.ia + nd.ka * (z - z_left);
+  double Aip_ = nd.ap + nd.kp * (z - z_left);
+  if (photon) return factor * (int_Ai_ + (2.0 / z + chi_a * sfp_sqrt(z)) * Aip_);
+  return factor * (int_Ai_ + (2.0 / z - chi_parent * sfp_sqrt(z)) * Aip_);
+}
+
+/* optical-depth tables as the kernels read them */
+typedef struct {
+  const double* photon_total; /* [chi_N] */
+  const double* pair_total;
+  const double* photon_cdf;   /* [chi_N][delta_N] */
+  const double* pair_cdf;
+  const double* chi_grid;     /* [chi_N] */
+  const double* delta_grid;   /* [delta_N] */
+  int64_t chi_N, delta_N;
+  double log_chi_min, log_chi_max, log_chi_delta, chi_min, chi_max;
+} sfp_od_dev;
+
+/* optical_depth_tables.py:46-56 on the total-rate table; chi above the table
+ * is clamped to its top node. */
+SFP_HD double sfp_od_total(const double* table1d, const sfp_od_dev* T, double chi) {
+  double log_chi = sfp_log10(chi);
+  if (log_chi > T->log_chi_max) log_chi = T->log_chi_max;
+  if (log_chi < T->log_chi_min) return 0.0;
+  double fidx = (log_chi - T->log_chi_min) / T->log_chi_delta;
+  int64_t idx = (int64_t)sfp_floor(fidx);
+  if (idx > T->chi_N - 2) idx = T->chi_N - 2;
+  double log_chi_left = T->log_chi_min + (double)idx * T->log_chi_delta;
+  double k = (table1d[idx + 1] - table1d[idx]) / T->log_chi_delta;
+  return table1d[idx] + k * (log_chi - log_chi_left);
+}
+
+/* optical_depth_tables.py:59-101: inverse CDF by bisection, lower bound with
+ * ties to the left, bracket clamped into the table. */
+SFP_HD double sfp_od_inverse_cdf(const double* table2d, const sfp_od_dev* T, double chi, double u) {
+  int64_t DN = T->delta_N;
+  if (chi > T->chi_max) chi = T->chi_max;
+  double log_chi = sfp_log10(chi);
+  if (log_chi > T->log_chi_max) log_chi = T->log_chi_max;
+  double fidx = log_chi < T->log_chi_min ? 0.0 : (log_chi - T->log_chi_min) / T->log_chi_delta;
+  int64_t lo_i = (int64_t)sfp_floor(fidx);
+  if (lo_i > T->chi_N - 2) lo_i = T->chi_N - 2;
+  int64_t hi_i = lo_i + 1;
+  double chi_low = T->chi_grid[lo_i];
+  double chi_high = T->chi_grid[hi_i];
+  const double* rl = table2d + lo_i * DN;
+  const double* rh = table2d + hi_i * DN;
+  double dchi = chi_high - chi_low;
+  double w = chi - chi_low;
+  double k = (rh[0] - rl[0]) / dchi;
+  double ymin = rl[0] + k * w;
+  k = (rh[DN - 1] - rl[DN - 1]) / dchi;
+  double ymax = rl[DN - 1] + k * w;
+  double r = u * (ymax - ymin) + ymin;
+  int64_t low = 0, high = DN - 1;
+  while (low <= high) {
+    int64_t mid = (low + high) / 2;
+    k = (rh[mid] - rl[mid]) / dchi;
+    double mid_delta = rl[mid] + k * w;
+    if (mid_delta < r)
+      low = mid + 1;
+    else
+      high = mid - 1;
+  }
+  int64_t di = high;
+  if (di < 0) di = 0;
+  if (di > DN - 2) di = DN - 2;
+  k = (rh[di] - rl[di]) / dchi;
+  double y1 = rl[di] + k * w;
+  k = (rh[di + 1] - rl[di + 1]) / dchi;
+  double y2 = rl[di + 1] + k * w;
+  double delta_left = T->delta_grid[di];
+  double delta_right = T->delta_grid[di + 1];
+  k = (delta_right - delta_left) / (y2 - y1);
+  return delta_left + k * (r - y1);
+}
+
+#endif /* SFP_DEVICE_H */

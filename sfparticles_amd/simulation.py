@@ -1,0 +1,175 @@
+"""Simulation driver (sfparticles/simulation.py) on the fused HIP step.
+
+``Simulation.start`` advances every species with one fused kernel per species
+and step (``sfp_sim_run``).  The order of work inside a step -- QED at time t
+for every species, appends in species order, then the leapfrog push of old
+and new particles -- is the reference's (simulation.py:102-144); particles
+created in a step are pushed in that step and are QED-active from the next.
+
+Callbacks and progress updates are served between chunks of steps, so a run
+without a callback advances ``print_every`` steps per library call.
+"""
+import ctypes as C
+from time import perf_counter_ns
+from typing import Callable
+
+from . import _lib as L
+from . import qed
+from .fields import Fields, terms_to_c
+from .particles import Particles, RadiationReactionType, _RR_CODE
+
+try:
+    from tqdm.auto import tqdm
+except Exception:  # pragma: no cover
+    tqdm = None
+
+
+class Simulation(object):
+    """simulation.py:12-161"""
+
+    def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
+                 t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None) -> None:
+        self.all_particles = all_particles
+        self.particles_push = []
+        self.particles_bw = []
+        self.particles_rad = []
+        self.dt = dt
+        self.fields = fields
+        self.print_every = print_every or None  # 0 behaves like None (reference divides by it)
+        self.photon_threshold = photon_threshold
+        self.seed = qed.seed if seed is None else int(seed)
+        self.qed_model = qed.model() if qed_model is None else qed_model
+        self.t = t0
+        self.step = 0
+        self.progress_bar = None
+        self._sim = None
+        self._sim_stores = None
+        self.last_run_ns = 0
+
+        for i, p in enumerate(all_particles):
+            p.tag = i
+            if p.bw:
+                assert p.bw_electron_ref() in all_particles, \
+                    f"{p.bw_electron_ref().name} not included in simulation."
+                assert p.bw_positron_ref() in all_particles, \
+                    f"{p.bw_positron_ref().name} not included in simulation."
+                self.particles_bw.append(p)
+            if p.radiating:
+                assert p.photon_ref() in all_particles, f"{p.photon_ref().name} not included in simulation."
+                self.particles_rad.append(p)
+            if p.push:
+                self.particles_push.append(p)
+
+    # ------------------------------------------------------------ native --
+    def _build(self):
+        """Create (or re-create after a store change) the native simulation."""
+        stores = tuple(p._store for p in self.all_particles)
+        if self._sim is not None and stores == self._sim_stores:
+            return
+        self._destroy()
+        terms = self.fields.require_terms()
+        idx = {id(p): i for i, p in enumerate(self.all_particles)}
+        n = len(self.all_particles)
+        descs = (L.SpeciesDesc * n)()
+        for i, p in enumerate(self.all_particles):
+            d = descs[i]
+            d.store = p._store
+            d.q = float(p.q)
+            d.m = float(p.m / 9.1093837015e-31) if p.m else 0.0
+            d.ae = float(p.ae)
+            d.rr = _RR_CODE[p.RR]
+            d.push = int(bool(p.push))
+            d.spin = int(bool(p.spin))
+            d.photon_dst = idx[id(p.photon_ref())] if p.radiating else -1
+            d.pair_e_dst = idx[id(p.bw_electron_ref())] if p.bw else -1
+            d.pair_p_dst = idx[id(p.bw_positron_ref())] if p.bw else -1
+        params = L.SimParams(float(self.dt), float(self.photon_threshold), int(self.seed),
+                             L.QED_OD if self.qed_model == "od" else L.QED_RS, 0)
+        arr = terms_to_c(terms)
+        h = C.c_void_p()
+        L.call("sfp_sim_create", descs, n, arr, len(terms), qed.device_tables(), C.byref(params),
+               C.byref(h))
+        self._sim = h.value
+        self._sim_stores = stores
+
+    def _destroy(self):
+        if self._sim is not None and L._lib is not None:
+            L._lib.sfp_sim_destroy(self._sim)
+        self._sim = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
+
+    def run_steps(self, nsteps: int, timing: bool = False) -> None:
+        """Advance nsteps without callbacks, progress or pruning."""
+        for p in self.all_particles:
+            p._ensure_device()
+            p._push_count_to_device()
+        self._build()
+        t = C.c_double(self.t)
+        L.call("sfp_sim_run", self._sim, C.byref(t), int(self.step), int(nsteps),
+               L.RUN_TIMING if timing else 0)
+        self.t = t.value
+        self.step += int(nsteps)
+        for p in self.all_particles:
+            p._sync_count_from_device()
+            object.__setattr__(p, "_ig_stale", True)
+
+    def kernel_time(self, species: int):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        L.call("sfp_sim_kernel_time", self._sim, int(species), C.byref(ms), C.byref(n))
+        return ms.value, int(n.value)
+
+    def reset_timing(self):
+        L.call("sfp_sim_reset_timing", self._sim)
+
+    def stats(self, species: int) -> dict:
+        """Counters of one species since the last reset: particle-steps (live
+        particles after each step), particles read by its step kernel,
+        particles that kernel appended, current buffered count."""
+        v = (C.c_int64 * 4)()
+        L.call("sfp_sim_stats", self._sim, int(species), v)
+        return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]))
+
+    def reset_stats(self):
+        L.call("sfp_sim_reset_stats", self._sim)
+
+    # ------------------------------------------------------------ driver --
+    def start(self, nstep: int, total: int = None, call_back: Callable[[int], None] = None):
+        """simulation.py:74-161"""
+        if total is None:
+            total = nstep
+        if self.progress_bar is None and self.print_every and tqdm is not None:
+            self.progress_bar = tqdm(total=total, unit="step", smoothing=1)
+
+        end = self.step + nstep
+        tic = perf_counter_ns()
+        while self.step < end:
+            if call_back is not None:
+                chunk = 1
+            elif self.print_every:
+                chunk = min(end - self.step, self.print_every - self.step % self.print_every)
+            else:
+                chunk = end - self.step
+            istep = self.step
+            self.run_steps(chunk)
+            if call_back is not None:
+                call_back(istep)
+            if self.print_every and self.step % self.print_every == 0:
+                elapsed = perf_counter_ns() - tic
+                tic = perf_counter_ns()
+                Ntotal = sum(p.Npart for p in self.all_particles)
+                ns_per_particle = elapsed / max(Ntotal, 1) / self.print_every
+                if self.progress_bar is not None:
+                    particle_num_str = ",".join(f"{p.Npart} {p.name}" for p in self.all_particles)
+                    self.progress_bar.set_postfix_str(
+                        f"{particle_num_str}, {ns_per_particle:.0f}ns/particle")
+                    self.progress_bar.update(self.print_every)
+
+        for p in self.all_particles:
+            p._to_host()
+            p._prune()

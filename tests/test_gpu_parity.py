@@ -1,0 +1,268 @@
+"""GPU parity: the HIP implementation against the CPU oracle on the same
+seeded inputs (Philox draws replayed on the CPU).
+
+Bar: bit-exact particle counts, ids (= compaction order) and FP64 state for
+the fused step and the per-operation kernels; QED-off trajectories of the
+oracle's libm build within 1e-10 relative (the reference's own criterion for
+deterministic trajectories).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.cases import (head_on_case, cascade_case, run_gpu_case, run_oracle_case,
+                         compare_final)
+
+pytestmark = pytest.mark.gpu
+
+
+def _names(out):
+    return {k: v["n"] for k, v in out.items()}
+
+
+def test_device_math_matches_host():
+    """sfp_math.h on the device gives the same bits as on the host."""
+    import oracle
+    from sfparticles_amd import _lib as L
+    rng = np.random.default_rng(1)
+    n = 200000
+    cases = {
+        "exp": (rng.uniform(-740, 705, n), None),
+        "log": (np.exp(rng.uniform(-700, 700, n)), None),
+        "sin": (rng.uniform(-3000, 3000, n), None),
+        "cos": (rng.uniform(-3000, 3000, n), None),
+        "atan": (np.sinh(rng.uniform(-30, 30, n)), None),
+        "pow": (np.exp(rng.uniform(-50, 50, n)), rng.uniform(-3, 3, n)),
+        "exp10": (rng.uniform(-300, 300, n), None),
+        "log10": (np.exp(rng.uniform(-700, 700, n)), None),
+    }
+    names = ["exp", "log", "sin", "cos", "atan", "pow", "exp10", "log10"]
+    for fn, (x, y) in cases.items():
+        yv = np.zeros_like(x) if y is None else y
+        dx, dy, do = L.DeviceArray.of(x), L.DeviceArray.of(yv), L.DeviceArray(x.nbytes)
+        L.call("sfp_math_eval", names.index(fn), dx.ptr, dy.ptr, do.ptr, n, None)
+        got = np.empty_like(x)
+        do.download(got)
+        want = oracle.math(fn, x, yv)
+        assert np.array_equal(got, want), f"{fn}: {np.sum(got != want)} values differ"
+    # IEEE sqrt and division are correctly rounded on both sides
+    x = np.exp(rng.uniform(-300, 300, n))
+    y = np.exp(rng.uniform(-300, 300, n))
+    for code, want in ((8, np.sqrt(x)), (9, x / y)):
+        dx, dy, do = L.DeviceArray.of(x), L.DeviceArray.of(y), L.DeviceArray(x.nbytes)
+        L.call("sfp_math_eval", code, dx.ptr, dy.ptr, do.ptr, n, None)
+        got = np.empty_like(x)
+        do.download(got)
+        assert np.array_equal(got, want), f"op {code}: {np.sum(got != want)} differ"
+
+
+def test_head_on_rs_bit_exact():
+    case = head_on_case(n=3000, nsteps=240, seed=11)
+    gpu, _ = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    print("counts", _names(gpu))
+    assert ref["photon"]["n"] > 1000  # the collision happened
+    compare_final(gpu, ref)
+
+
+def test_head_on_rs_chunked_equals_single():
+    """start() called in several chunks (prune in between) replays the same draws."""
+    case = head_on_case(n=1500, nsteps=160, seed=5)
+    g1, _ = run_gpu_case(case, chunks=1)
+    g4, _ = run_gpu_case(case, chunks=4)
+    r4 = run_oracle_case(case, chunks=4)
+    compare_final(g4, r4)
+    # pruning between chunks does not change what survives
+    for name in g1:
+        assert g1[name]["n"] == g4[name]["n"]
+        assert np.array_equal(g1[name]["id"], g4[name]["id"])
+        assert np.array_equal(g1[name]["x"], g4[name]["x"])
+
+
+def test_head_on_od_bit_exact():
+    case = head_on_case(n=3000, nsteps=240, seed=12, qed="od")
+    gpu, _ = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    print("counts", _names(gpu))
+    assert ref["photon"]["n"] > 1000
+    compare_final(gpu, ref)
+
+
+def test_cascade_od_bit_exact():
+    # a0 = 500 and pulses starting 1 ctau from the node keep the cascade at
+    # ~1e4 particles (64 seeds -> ~230 e-, ~170 e+, ~7000 photons by step 1600)
+    case = cascade_case(n=64, nsteps=1600, seed=2, a0=500.0, x0_ctau=1.0)
+    gpu, _ = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    print("counts", _names(gpu))
+    assert ref["positron"]["n"] > 0  # pairs were created
+    compare_final(gpu, ref)
+
+
+def test_qed_off_spin_ll_trajectories():
+    """QED off (Boris + LL + T-BMT): bit-exact against the oracle, and within
+    1e-10 relative of the oracle built on the platform libm."""
+    case = head_on_case(n=2000, nsteps=300, seed=13, species=("e",), rr="LL", spin=True)
+    gpu, _ = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    compare_final(gpu, ref)
+    ref_libm = run_oracle_case(case, libm=True)
+    compare_final(gpu, ref_libm, rtol=1e-10)
+
+
+def test_focused_and_simple_fields_push():
+    """Focused Gaussian and a sum of plane pulses through the fused push."""
+    case = head_on_case(n=1000, nsteps=120, seed=21, species=("e",), rr="CLL")
+    c = head_on_case(n=1000, nsteps=120, seed=21)["field"][0][1]
+    case["field"] = [(3, (100.0, 2e-6, c[2], 1, 0.0, c[4], 0.8e-6, 0.3, 0.1)),
+                     (1, (30.0, 3e-6, c[2], 2, -1, 1e-6, 0.8e-6, 0.7, 0.2))]
+    gpu, _ = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    compare_final(gpu, ref)
+
+
+def test_per_op_methods_match_oracle():
+    """Particles._eval_field/_calculate_chi/_photon_event/_pick_hard_photon/
+    _create_photon/_push_* on the GPU against the oracle's passes."""
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed as Q
+    from sfparticles_amd.fields import Fields
+    import oracle
+    from oracle.sim import OParticles, OSimulation
+    from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
+
+    Q.set_model("rs")
+    case = head_on_case(n=4000, nsteps=2, seed=7)
+    case["props"] = (np.zeros(4000) + 1e-7,) + tuple(case["props"][1:])
+    e = sfa.Particles("e", -1, 1, N=4000, props=case["props"])
+    g = sfa.Particles("g", 0, 0)
+    e.set_photon(g)
+    F = Fields(terms=case["field"])
+    oe = OParticles("e", -1, 1, N=4000, props=[a.copy() for a in case["props"]])
+    og = OParticles("g", 0, 0)
+    oe.set_photon(og)
+    osim = OSimulation(oe, og, dt=case["dt"], fields=case["field"], rs_tables=load_rs_tables(),
+                       od_tables=load_od_tables())
+    t = 40e-15
+    for it in range(3):
+        e._eval_field(F, t)
+        e._calculate_chi()
+        e._photon_event(case["dt"] * 20)
+        e._pick_hard_photon(2.0)
+        e._create_photon(g)
+        e._push_position(case["dt"])
+        osim.eval_field(oe, t)
+        osim.calculate_chi(oe)
+        osim.qed_event(oe, case["dt"] * 20, it, slot_base=Q.PER_OP_SLOT)
+        osim.pick_hard_photon(oe, 2.0)
+        osim.create_photon(oe, og)
+        osim.push_position(oe, case["dt"])
+        assert g.N_buffered == og.N_buffered
+        for a in ("x", "ux", "chi", "Ey", "Bz", "inv_gamma"):
+            assert np.array_equal(getattr(e, a)[:e.N_buffered], getattr(oe, a)[:oe.N_buffered]), a
+        for a in ("x", "ux", "uy", "id"):
+            assert np.array_equal(getattr(g, a)[:g.N_buffered], getattr(og, a)[:og.N_buffered]), a
+    assert g.N_buffered > 0
+
+
+def test_reference_pusher_tests():
+    """tests/test_pusher.py of the reference, on the GPU backend functions."""
+    from sfparticles_amd.particles import boris, LL_push
+    from sfparticles_amd.constants import e, m_e, c, pi
+    r = 1e-6
+    ux0, uz0 = 100, 10
+    u0 = np.sqrt(ux0 ** 2 + uz0 ** 2)
+    v = ux0 / np.sqrt(1 + ux0 ** 2) * c
+    T = 2 * pi * r / v
+    bz = m_e * c / e * ux0 / r
+    ux = np.asarray([ux0], dtype=float)
+    uy = np.asarray([0], dtype=float)
+    uz = np.asarray([uz0], dtype=float)
+    inv_gamma = 1 / np.sqrt(1 + ux ** 2 + uy ** 2 + uz ** 2)
+    z = np.zeros(1)
+    Bz = np.asarray([bz], dtype=float)
+    pr = np.array([False])
+    for _ in range(20):
+        boris(ux, uy, uz, inv_gamma, z, z, z, z, z, Bz, -e, 1, pr, T / 20)
+    assert abs(np.sqrt(ux ** 2 + uy ** 2 + uz ** 2) - u0) / u0 < 1e-10
+
+    ux = np.asarray([1000.0])
+    uy = np.zeros(1)
+    uz = np.zeros(1)
+    inv_gamma = 1 / np.sqrt(1 + ux ** 2)
+    LL_push(ux, uy, uz, inv_gamma, np.asarray([1.0]), 1, pr, 2.67e-17)
+    assert ux[0] < 1000.0
+    assert round((1000.0 - ux[0]) / 1000.0 - 0.1, 2) == 0
+
+
+def test_photon_number_statistics():
+    """tests/test_qed.py TestPhotonNumber: count and delta bins within 3 sigma
+    of the closed-form rate (RS and OD models), N = 1e6 electrons."""
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed as Q
+    from sfparticles_amd.fields import static_field
+    from sfparticles_amd.constants import m_e, c, e, hbar
+    from sfparticles_amd.qed.tables import photon_rate_for_delta, load_od_tables
+    od = load_od_tables()
+    gamma = 10000
+    ux = np.sqrt(gamma ** 2 - 1)
+    N = 1_000_000
+    for model in ("rs", "od"):
+        Q.set_model(model)
+        for chi_e in (0.1, 1.0, 5.0):
+            interval = 1e-17 / chi_e
+            tau = interval / gamma
+            ele = sfa.Particles("ele", -1, 1, N=N, props=(np.zeros(N), np.zeros(N), np.zeros(N),
+                                                         np.full(N, ux), np.zeros(N), np.zeros(N)),
+                                RR=sfa.RadiationReactionType.NONE)
+            pho = sfa.Particles("pho", 0, 0)
+            ele.set_photon(pho)
+            ele._eval_field(static_field(Bz=chi_e * m_e ** 2 * c ** 2 / e / hbar / ux), 0)
+            ele._calculate_chi()
+            ele._photon_event(interval)
+            ele._create_photon(pho)
+            n_photon = pho.Npart
+            # total rate: the optical-depth table at this chi (pinned to the KAT)
+            W = np.interp(np.log10(chi_e), np.linspace(-3, 2, 256), od.photon_total)
+            p = W * tau
+            expect = p * N
+            sigma = np.sqrt(expect * (1 - p))
+            assert abs(n_photon - expect) < 3 * sigma + 1e-3 * expect, (model, chi_e, n_photon, expect)
+            P = photon_rate_for_delta(chi_e)
+            for delta in (0.01, 0.1):
+                nb = np.histogram(pho.ux[:pho.N_buffered] / gamma, bins=[delta, delta * 1.1])[0][0]
+                pb = (P(delta) + P(delta * 1.1)) * (delta * 0.1) / 2 * tau
+                eb = pb * N
+                assert abs(nb - eb) < 3 * np.sqrt(eb * (1 - pb)) + 0.01 * eb, (model, chi_e, delta, nb, eb)
+
+
+def test_pair_bookkeeping_and_momentum():
+    """tests/test_qed.py TestPairNumber: bookkeeping and momentum conservation."""
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed as Q
+    from sfparticles_amd.fields import static_field
+    from sfparticles_amd.constants import m_e, c, e, hbar
+    for model in ("rs", "od"):
+        Q.set_model(model)
+        gamma = 10000
+        ux = np.sqrt(gamma ** 2 - 1)
+        N = 200_000
+        pho = sfa.Particles("pho", 0, 0, N=N, props=(np.zeros(N), np.zeros(N), np.zeros(N),
+                                                    np.full(N, ux), np.zeros(N), np.zeros(N)))
+        ele = sfa.Particles("ele", -1, 1)
+        pos = sfa.Particles("pos", 1, 1)
+        pho.set_pair(ele, pos)
+        pho._eval_field(static_field(Bz=m_e ** 2 * c ** 2 / e / hbar / ux), 0)
+        pho._calculate_chi()
+        ev, _ = pho._pair_event(1e-17)
+        ev = np.array(ev[:N])
+        ux_pho = np.array(pho.ux[:N])[ev]
+        pho._create_pair(ele, pos)
+        assert pos.N_buffered == pos.Npart == ele.Npart == ev.sum() > 0
+        assert N - pho.Npart == pos.Npart
+        assert (np.abs(ele.ux[:ele.N_buffered] + pos.ux[:pos.N_buffered] - ux_pho) < 1e-10).all()
+        pho._pair_event(1e-17)
+        pho._create_pair(ele, pos)
+        assert pos.N_buffered == pos.Npart
+        assert N - pho.Npart == pos.Npart
