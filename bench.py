@@ -35,16 +35,22 @@ DT = LAMBDA / C_LIGHT / 100
 CTAU = C_LIGHT * 30e-15 / math.sqrt(2 * math.log(2))
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes per particle-step of each step kernel: the persistent
-# state read once and written once, plus the id (8 B) and pruned flag (1 B);
-# per appended particle 57 B are written (x, y, z, u 48 B + id + flag)
-BYTES = {
-    "lepton_rs": 48 + 48 + 8 + 1,   # x,y,z,u read and written
-    "lepton_od": 48 + 48 + 8 + 1 + 16,
-    "photon_rs": 48 + 24 + 8 + 1,   # u read only
-    "photon_od": 48 + 24 + 8 + 1 + 16,
-    "spawn": 57,
-}
+# Algorithmic bytes of the two step kernels of a species (the persistent
+# state read once and written once; transients stay in registers):
+#   sweep: reads x, y, z, u and the pruned flag of every buffered particle
+#          (49 B); writes x, y, z (photon, 24 B) or x, y, z, u (lepton, 48 B)
+#          of every particle it pushes, and a 4-byte index for every particle
+#          it puts on the QED list
+#   hot:   per listed particle reads the index, x, y, z, u and the id (60 B,
+#          + 8 B tau read and written under the optical-depth model), writes
+#          x, y, z (+ u for leptons); per appended particle writes x, y, z, u,
+#          id and flag (57 B)
+def kernel_bytes(photon, phase, st, od=False):
+    pushed_w = 24 if photon else 48
+    if phase == 0:
+        return 49 * st["processed"] + pushed_w * (st["processed"] - st["hot"]) + 4 * st["hot"]
+    return ((60 + (16 if od else 0)) * st["hot"] + pushed_w * st["hot"]
+            + 57 * st["spawned"] * (2 if photon else 1))
 
 
 def make_case(n, nsteps, seed=0):
@@ -89,15 +95,21 @@ def gpu_run(n, steps, warmup, nsteps_config):
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0))
         wsim.run_steps(warmup)
         del wsim
-    reserve = (int(n * 1.3) + 4096, int(n * 0.3) + 4096, int(n * 9) + 4096)
+    reserve = (2 * n + 4096, n + 4096, 26 * n + 4096)
     sim, species = build_gpu(n, nsteps_config, reserve)
     L.call("sfp_synchronize")
     t0 = time.perf_counter()
-    sim.run_steps(steps, timing=True)
+    done = 0
+    while done < steps:  # chunks, so that progress is visible during long runs
+        m = min(250, steps - done)
+        sim.run_steps(m, timing=True)
+        done += m
+        print(f"[bench] {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
+              f"{[s.N_buffered for s in species]}", file=sys.stderr, flush=True)
     L.call("sfp_synchronize")
     t1 = time.perf_counter()
     stats = [sim.stats(i) for i in range(3)]
-    ktime = [sim.kernel_time(i) for i in range(3)]
+    ktime = [[sim.kernel_time(i, ph) for ph in (0, 1)] for i in range(3)]
     return t1 - t0, stats, ktime, [s.N_buffered for s in species]
 
 
@@ -174,24 +186,24 @@ def main():
     psteps = sum(s["psteps"] for s in stats)
     value = psteps / elapsed
 
-    # dominant kernel: largest accumulated time
+    # dominant kernel: largest accumulated time over (species, phase)
     names = ["electron", "positron", "photon"]
-    kinds = ["lepton_rs", "lepton_rs", "photon_rs"]
-    dom = max(range(3), key=lambda i: ktime[i][0])
-    ms, launches = ktime[dom]
-    st = stats[dom]
-    alg_bytes = BYTES[kinds[dom]] * st["processed"] + BYTES["spawn"] * st["spawned"] * (2 if dom == 2 else 1)
+    dom, ph = max(((i, p) for i in range(3) for p in (0, 1)), key=lambda ip: ktime[ip[0]][ip[1]][0])
+    ms, launches = ktime[dom][ph]
+    alg_bytes = kernel_bytes(dom == 2, ph, stats[dom])
     per_launch = alg_bytes / max(launches, 1)
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    kname = f"k_advance<{'photon' if dom == 2 else 'lepton'}, RS> ({names[dom]})"
-    traffic = load_traffic(kinds[dom] + "_" + names[dom])
+    kname = (f"k_{'sweep' if ph == 0 else 'hot'}<{'photon' if dom == 2 else 'lepton'}, RS, "
+             f"gaussian field> ({names[dom]})")
+    traffic = load_traffic(f"{'sweep' if ph == 0 else 'hot'}_{names[dom]}_rs")
+    total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
                 "kernel": kname, "avg_launch_ms": round(ms / max(launches, 1), 4),
                 "alg_bytes_per_launch": int(per_launch),
-                "kernel_time_share": round(ms / max(1e-9, sum(k[0] for k in ktime)), 3)}
+                "kernel_time_share": round(ms / max(1e-9, total_ms), 3)}
 
     cpu = None
     if not args.no_cpu:
@@ -221,7 +233,10 @@ def main():
                    "path": "fp64-tables", "electrons": n, "steps_timed": args.steps,
                    "final_counts": dict(zip(names, counts)),
                    "particle_steps": {names[i]: stats[i]["psteps"] for i in range(3)},
-                   "kernel_ms": {names[i]: round(ktime[i][0], 2) for i in range(3)}},
+                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
+                   "kernel_ms": {f"{names[i]}_{'sweep' if p == 0 else 'hot'}": round(ktime[i][p][0], 2)
+                                 for i in range(3) for p in (0, 1)},
+                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

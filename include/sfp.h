@@ -283,19 +283,25 @@ int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies, const sfp_
 int sfp_sim_destroy(void* sim);
 /* Advance nsteps steps from time *t (updated: t += dt per step, as
  * simulation.py:143) and step index step0 (the Philox counter).  Counts stay
- * on the device; capacities grow as needed between steps. */
+ * on the device; the host synchronises every 64 steps.  A step whose appends
+ * would overflow a store halts on the device before changing anything; the
+ * store grows and the step is replayed from the halted kernel. */
 #define SFP_RUN_TIMING 1 /* record HIP events around each species kernel */
 int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t flags);
-/* accumulated kernel time (ms) and launch count of one species' step kernel
- * since the last reset (requires SFP_RUN_TIMING); synchronous */
-int sfp_sim_kernel_time(void* sim, int32_t species, double* ms, int64_t* launches);
+/* Accumulated time (ms) and launch count of one species' step kernel since
+ * the last reset (requires SFP_RUN_TIMING; synchronous).  phase 0 is the
+ * sweep kernel (envelope bound, push of particles that cannot undergo a QED
+ * event, QED list), phase 1 the QED kernel over that list. */
+int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
+                        int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
-/* Counters of one species since the last reset (synchronous):
- * stats[0] particle-steps (live particles after each step, the reference's
- *          ns/particle denominator, simulation.py:150-151)
- * stats[1] particles read by the species' step kernel
- * stats[2] particles that kernel appended (to each destination)
- * stats[3] current buffered count */
+/* Counters of one species since the last reset (synchronous), stats[8]:
+ * [0] particle-steps (live particles after each step, the reference's
+ *     ns/particle denominator, simulation.py:150-151)
+ * [1] particles read by the sweep kernel   [2] particles appended by the species
+ * [3] current buffered count               [4] particles on QED lists (sum over steps)
+ * [5] host synchronisations of the run     [6] replays after a capacity halt
+ * [7] current store capacity */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 

@@ -119,7 +119,7 @@ SIGNATURES = {
     "sfp_sim_create": [P, I32, P, I32, P, P, C.POINTER(P)],
     "sfp_sim_destroy": [P],
     "sfp_sim_run": [P, C.POINTER(D), U64, I64, I32],
-    "sfp_sim_kernel_time": [P, I32, C.POINTER(D), C.POINTER(I64)],
+    "sfp_sim_kernel_time": [P, I32, I32, C.POINTER(D), C.POINTER(I64)],
     "sfp_sim_reset_timing": [P],
     "sfp_sim_stats": [P, I32, C.POINTER(I64)],
     "sfp_sim_reset_stats": [P],

@@ -1,12 +1,19 @@
 #!/bin/bash
-# Build libsfp_hip.so for gfx950 in-tree (next to this script's package).
+# Build libsfp_hip.so for gfx950 in-tree: each translation unit is compiled in
+# parallel, then linked.
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="${1:-$HERE/../libsfp_hip.so}"
-/opt/rocm/bin/hipcc --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
-  -ffp-contract=off -fno-fast-math \
-  -Xclang -target-feature -Xclang -packed-fp32-ops \
-  -Wall -Wno-unused-function \
-  -I"$HERE" -I"$HERE/../../include" \
-  -o "$OUT.tmp" "$HERE/sfp_lib.hip"
+B="$HERE/_build"
+mkdir -p "$B"
+FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -fno-fast-math
+       -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-function
+       -I"$HERE" -I"$HERE/../../include")
+pids=()
+for src in sfp_lib sfp_step; do
+  /opt/rocm/bin/hipcc "${FLAGS[@]}" -c -o "$B/$src.o" "$HERE/$src.hip" &
+  pids+=($!)
+done
+for p in "${pids[@]}"; do wait "$p"; done
+/opt/rocm/bin/hipcc --offload-arch=gfx950 -shared -fPIC -o "$OUT.tmp" "$B/sfp_lib.o" "$B/sfp_step.o"
 mv "$OUT.tmp" "$OUT"

@@ -249,20 +249,48 @@ SFP_HD void sfp_field_focused(const sfp_term_dev* d, double x, double y, double 
   F[5] = (By_ * sp + Bz_ * cp) * direction;
 }
 
+/* Which term kinds a kernel instance handles: the fused step kernels are
+ * instantiated per mode so that a field made only of Gaussian (or only of
+ * plane/static) terms does not pay the registers of the focused expansion. */
+enum { SFP_FM_ANY = 0, SFP_FM_GAUSSIAN = 1, SFP_FM_PLANE = 2 };
+
+static inline int sfp_field_mode(const sfp_fieldset* fs) {
+  int gauss = 1, plane = 1;
+  for (int j = 0; j < fs->nterms; ++j) {
+    int k = fs->t[j].kind;
+    if (k != SFP_FIELD_GAUSSIAN) gauss = 0;
+    if (k != SFP_FIELD_SIMPLE && k != SFP_FIELD_STATIC) plane = 0;
+  }
+  if (fs->nterms > 0 && gauss) return SFP_FM_GAUSSIAN;
+  if (fs->nterms > 0 && plane) return SFP_FM_PLANE;
+  return SFP_FM_ANY;
+}
+
 /* Fields.__add__ (fields.py:52-61): left fold over the terms */
+template <int FM = SFP_FM_ANY>
 SFP_HD void sfp_field_eval(const sfp_fieldset* fs, double x, double y, double z, double t,
                            double* F) {
   double ct = SFP_C * t;
   for (int j = 0; j < fs->nterms; ++j) {
     double G[6];
     const sfp_term_dev* d = &fs->t[j];
-    switch (d->kind) {
-      case SFP_FIELD_STATIC:
+    if (FM == SFP_FM_GAUSSIAN) {
+      sfp_field_gaussian(d, x, y, z, t, G);
+    } else if (FM == SFP_FM_PLANE) {
+      if (d->kind == SFP_FIELD_SIMPLE) {
+        sfp_field_simple(d, x, y, z, ct, G);
+      } else {
         for (int i = 0; i < 6; ++i) G[i] = d->c[i];
-        break;
-      case SFP_FIELD_SIMPLE: sfp_field_simple(d, x, y, z, ct, G); break;
-      case SFP_FIELD_GAUSSIAN: sfp_field_gaussian(d, x, y, z, t, G); break;
-      default: sfp_field_focused(d, x, y, z, ct, G); break;
+      }
+    } else {
+      switch (d->kind) {
+        case SFP_FIELD_STATIC:
+          for (int i = 0; i < 6; ++i) G[i] = d->c[i];
+          break;
+        case SFP_FIELD_SIMPLE: sfp_field_simple(d, x, y, z, ct, G); break;
+        case SFP_FIELD_GAUSSIAN: sfp_field_gaussian(d, x, y, z, t, G); break;
+        default: sfp_field_focused(d, x, y, z, ct, G); break;
+      }
     }
     if (j == 0) {
       for (int i = 0; i < 6; ++i) F[i] = G[i];
@@ -272,6 +300,50 @@ SFP_HD void sfp_field_eval(const sfp_fieldset* fs, double x, double y, double z,
   }
   if (fs->nterms == 0)
     for (int i = 0; i < 6; ++i) F[i] = 0.0;
+}
+
+/* Upper bound of |E| + c|B| at (x, y, z, t), from the envelopes only (no
+ * carrier phase, no Gouy/curvature trigonometry).  Used to prove that no QED
+ * event is possible for a particle: chi <= (e hbar/(m^2 c^3)) gamma (|E| + c|B|).
+ *  simple:   |E| = c|B| <= E0 exp(-r^2/w0^2 - phi^n/(k ctau)^n)  (phi exact)
+ *  gaussian: |E| = c|B| <= E0 exp(-r^2/w^2(x) - phi_lo^2/(k ctau)^2) with
+ *            phi_lo = max(0, |k(x - x0 - d c t)| - pi/2 - k r^2/(4 zR)), since the
+ *            Gouy phase is within pi/2, the curvature phase k r^2/(2R) within
+ *            k r^2/(4 zR), w0/w <= 1
+ *  static:   |E| + c|B|
+ *  focused:  no bound (+inf). */
+template <int FM = SFP_FM_ANY>
+SFP_HD double sfp_field_bound(const sfp_fieldset* fs, double x, double y, double z, double t) {
+  double S = 0.0;
+  double ct = SFP_C * t;
+  double r2 = y * y + z * z;
+  for (int j = 0; j < fs->nterms; ++j) {
+    const sfp_term_dev* d = &fs->t[j];
+    const double* c = d->c;
+    int kind = (FM == SFP_FM_GAUSSIAN) ? SFP_FIELD_GAUSSIAN : d->kind;
+    if (FM == SFP_FM_PLANE && kind != SFP_FIELD_SIMPLE) kind = SFP_FIELD_STATIC;
+    if (kind == SFP_FIELD_SIMPLE) {
+      double xd = c[FS_DIR] * x;
+      double phi = c[FS_K0] * (xd - c[FS_DX0] - ct);
+      double a = r2 / c[FS_W0SQ] + sfp_powi(phi, d->n) / c[FS_KCTN];
+      S += 2.0 * c[FS_E0] * sfp_exp(-a);
+    } else if (kind == SFP_FIELD_GAUSSIAN) {
+      double zR = c[FG_ZR], k0 = c[FG_K0];
+      double xz = x / zR;
+      double w2 = c[FG_W0] * c[FG_W0] * (1 + xz * xz);
+      double phl = k0 * (x - c[FG_X0] - c[FG_DC] * t);
+      phl = (phl < 0 ? -phl : phl) - 1.5707963267948966 - k0 * r2 / (4.0 * zR);
+      if (phl < 0) phl = 0;
+      double a = r2 / w2 + phl * phl / c[FG_KCT2];
+      S += 2.0 * c[FG_E0] * sfp_exp(-a);
+    } else if (kind == SFP_FIELD_STATIC) {
+      S += sfp_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) +
+           SFP_C * sfp_sqrt(c[3] * c[3] + c[4] * c[4] + c[5] * c[5]);
+    } else {
+      return __builtin_inf();
+    }
+  }
+  return S;
 }
 
 /* ------------------------------------------------------------- pusher -- */

@@ -1,0 +1,132 @@
+// sfp_internal.h -- declarations shared by the translation units of libsfp_hip.so.
+#ifndef SFP_INTERNAL_H
+#define SFP_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sfp_device.h"
+
+// errors: set the thread's last-error message and return the code
+int sfp_set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+#define set_err sfp_set_err
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(SFP_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                              \
+  } while (0)
+
+#define CHK(expr)          \
+  do {                     \
+    int rc_ = (expr);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
+static inline hipStream_t S(sfp_stream_t s) { return (hipStream_t)s; }
+
+#define KLOOP(n) for (int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ip < (n); ip += (int64_t)gridDim.x * blockDim.x)
+
+static constexpr int OPB = 256;
+
+static inline int op_grid(int64_t n) {
+  int64_t g = (n + OPB - 1) / OPB;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+__device__ inline int wave_lane() { return (int)__lane_id(); }
+
+// QED tables in device memory
+struct Tables {
+  sfp_rs_node* rs_nodes = nullptr;  // [z_N-1]
+  // decreasing envelopes of the table rates over brackets j.. (see sfp_tables_create)
+  double2* rs_env = nullptr;        // [z_N-1] {E1, E2}
+  int64_t z_N = 0;
+  double z_min = 0, z_max = 0, z_delta = 0;
+  bool has_rs = false;
+  double* od_mem = nullptr;
+  sfp_od_dev od{};
+  double2* od_photon_nodes = nullptr;  // [chi_N-1] {value, slope}
+  double2* od_pair_nodes = nullptr;
+  bool has_od = false;
+};
+
+// per-species device counters
+struct Counts {
+  int64_t n;          // buffered particles (live + pruned holes)
+  int64_t next_id;    // id of the next appended particle
+  int64_t n_old;      // n at the start of the current step
+  int64_t overflow;   // appends that did not fit (per-operation path)
+  int64_t psteps;     // particle-steps advanced (live after each step)
+  int64_t processed;  // particles read by this species' sweep kernel
+  int64_t spawned;    // particles this species appended (per destination)
+  int64_t n_hot;      // particles on this step's QED list
+  int64_t hot_total;  // sum of n_hot over steps
+  int64_t pad[7];
+};
+
+extern const size_t kAttrSize[SFP_NATTR];
+
+struct Store {
+  int32_t flags = 0;
+  int64_t cap = 0;
+  void* p[SFP_NATTR] = {};
+  Counts* cnt = nullptr;  // device
+};
+
+int sfp_compile_fields(const sfp_field* terms, int32_t nterms, sfp_fieldset* fs);
+int sfp_compact_indices(const uint8_t* f, int64_t n, int invert, int64_t* out, int64_t* count_host,
+                        hipStream_t st);
+
+// one optical-depth update (optical_depth.py:42-62); returns the event flag
+__device__ inline bool od_update(double* tau, double ig, double chi, double dt, uint64_t seed,
+                                 uint64_t id, uint64_t step, uint32_t tag, uint32_t slot,
+                                 const double2* nodes, const double* cdf, const sfp_od_dev& od,
+                                 double* delta) {
+  // total rate (optical_depth_tables.py:46-56) from {value, slope} nodes
+  double rate;
+  {
+    double log_chi = sfp_log10(chi);
+    if (log_chi > od.log_chi_max) log_chi = od.log_chi_max;
+    if (log_chi < od.log_chi_min) {
+      rate = 0.0;
+    } else {
+      double fidx = (log_chi - od.log_chi_min) / od.log_chi_delta;
+      int64_t idx = (int64_t)sfp_floor(fidx);
+      if (idx > od.chi_N - 2) idx = od.chi_N - 2;
+      double left = od.log_chi_min + (double)idx * od.log_chi_delta;
+      double2 nd = nodes[idx];
+      rate = nd.x + nd.y * (log_chi - left);
+    }
+  }
+  double dtau = dt * ig;
+  double t = *tau;
+  if (t == 0.0) {
+    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, slot + 1);
+    t = -sfp_log(1 - sfp_u01(d.v[0], d.v[1]));
+  }
+  t -= rate * dtau;
+  bool ev = false;
+  if (t < 0) {
+    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, slot + 2);
+    t = -sfp_log(1 - sfp_u01(d.v[0], d.v[1]));
+    ev = true;
+    *delta = sfp_od_inverse_cdf(cdf, &od, chi, sfp_u01(d.v[2], d.v[3]));
+  } else {
+    *delta = 0.0;
+  }
+  *tau = t;
+  return ev;
+}
+
+#endif  // SFP_INTERNAL_H

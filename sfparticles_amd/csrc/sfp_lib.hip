@@ -1,39 +1,23 @@
-// sfp_lib.hip -- libsfp_hip.so: the sfparticles per-particle step on MI355X (gfx950).
+// sfp_lib.hip -- libsfp_hip.so, part 1: the per-operation kernels of the
+// reference's three backend seams, QED table upload, device species stores and
+// small device-memory helpers.  The fused simulation step is in sfp_step.hip.
 //
 // Layout in HBM: one struct-of-arrays store per species (x, y, z, ux, uy, uz
 // as float64, a uint64 id and a uint8 pruned flag; tau and spin when the
 // species needs them).  inv_gamma, chi, E, B, the event flag and delta are
-// transients of the fused step and live in registers; the per-operation
-// entry points materialise them as auxiliary store attributes.
-//
-// The fused step launches one persistent kernel per species and step.  Each
-// workgroup takes 256-particle tiles from an atomic tile counter; per tile it
-// evaluates the QED front (field at t, chi, emission/pair draw, hard-photon
-// cut), computes the stable rank of every event with a wave64 ballot +
-// workgroup scan + decoupled look-back across tiles, appends the new
-// particles (already pushed, as the reference pushes them in the step they
-// are created), applies recoil and pushes the tile.
+// transients of the fused step; the per-operation entry points materialise
+// them as auxiliary store attributes.
 //
 // All physics is compiled with -ffp-contract=off and uses the project's own
 // elementary functions (sfp_math.h), so every FP64 result is bit-identical
 // to the CPU restatement used for validation.
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "sfp_device.h"
+#include "sfp_internal.h"
 
 // ----------------------------------------------------------------- errors --
 static thread_local std::string g_err;
 
-static int set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int set_err(int code, const char* fmt, ...) {
+int sfp_set_err(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -43,54 +27,7 @@ static int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return set_err(SFP_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                     __FILE__, __LINE__);                                              \
-  } while (0)
-
-#define CHK(expr)          \
-  do {                     \
-    int rc_ = (expr);      \
-    if (rc_) return rc_;   \
-  } while (0)
-
-static inline hipStream_t S(sfp_stream_t s) { return (hipStream_t)s; }
-
-static inline int grid_for(int64_t n, int block) {
-  int64_t g = (n + block - 1) / block;
-  if (g < 1) g = 1;
-  if (g > (int64_t)INT32_MAX) g = INT32_MAX;
-  return (int)g;
-}
-
-// --------------------------------------------------------------- tables ---
-struct Tables {
-  sfp_rs_node* rs_nodes = nullptr;  // [z_N-1]
-  int64_t z_N = 0;
-  double z_min = 0, z_max = 0, z_delta = 0;
-  bool has_rs = false;
-  // optical depth
-  double* od_mem = nullptr;
-  sfp_od_dev od{};
-  double2* od_photon_nodes = nullptr;  // [chi_N-1] {value, slope}
-  double2* od_pair_nodes = nullptr;
-  bool has_od = false;
-};
-
 // ------------------------------------------------------------ per-op API --
-#define KLOOP(n) for (int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ip < (n); ip += (int64_t)gridDim.x * blockDim.x)
-
-static constexpr int OPB = 256;
-
-static int op_grid(int64_t n) {
-  int64_t g = (n + OPB - 1) / OPB;
-  if (g < 1) g = 1;
-  if (g > 65536) g = 65536;
-  return (int)g;
-}
 
 __global__ void k_eval_field(const sfp_fieldset* fs, const double* x, const double* y,
                              const double* z, double t, int64_t n, const uint8_t* pruned,
@@ -105,7 +42,7 @@ __global__ void k_eval_field(const sfp_fieldset* fs, const double* x, const doub
   }
 }
 
-static int compile_fields(const sfp_field* terms, int32_t nterms, sfp_fieldset* fs) {
+int sfp_compile_fields(const sfp_field* terms, int32_t nterms, sfp_fieldset* fs) {
   if (nterms < 0 || nterms > SFP_MAX_TERMS)
     return set_err(SFP_E_ARG, "field has %d terms; at most %d are supported", nterms,
                    SFP_MAX_TERMS);
@@ -144,7 +81,7 @@ extern "C" int sfp_eval_field(const sfp_field* terms, int32_t nterms, const doub
                               double* Bx, double* By, double* Bz, sfp_stream_t stream) {
   if (n <= 0) return SFP_OK;
   sfp_fieldset fs;
-  CHK(compile_fields(terms, nterms, &fs));
+  CHK(sfp_compile_fields(terms, nterms, &fs));
   CHK(g_fieldbuf.ensure(sizeof(fs)));
   HIPCHK(hipMemcpyAsync(g_fieldbuf.p, &fs, sizeof(fs), hipMemcpyHostToDevice, S(stream)));
   hipLaunchKernelGGL(k_eval_field, dim3(op_grid(n)), dim3(OPB), 0, S(stream),
@@ -437,7 +374,6 @@ extern "C" int sfp_math_eval(int32_t fn, const double* x, const double* y, doubl
 // ---- flag compaction for the per-op API: block counts -> scan -> scatter --
 static constexpr int CB = 1024;  // flags per block
 
-__device__ inline int wave_lane() { return (int)__lane_id(); }
 
 __global__ void k_block_count(const uint8_t* f, int64_t n, int invert, int64_t* counts) {
   __shared__ int wsum[CB / 64];
@@ -502,13 +438,10 @@ __global__ void __launch_bounds__(CB) k_scatter_idx(const uint8_t* f, int64_t n,
   if (on) out[offsets[blockIdx.x] + woff + rank] = i;
 }
 
-struct ScratchI64 {
-  DevBuf buf;
-};
 static thread_local DevBuf g_cnt_buf;
 
 // indices of flags (== 1 ^ invert) into out (device); count to *count_host
-static int compact_indices(const uint8_t* f, int64_t n, int invert, int64_t* out, int64_t* count_host,
+int sfp_compact_indices(const uint8_t* f, int64_t n, int invert, int64_t* out, int64_t* count_host,
                            hipStream_t st) {
   if (n <= 0) {
     *count_host = 0;
@@ -530,12 +463,12 @@ static int compact_indices(const uint8_t* f, int64_t n, int invert, int64_t* out
 
 extern "C" int sfp_find_event_index(const uint8_t* event, int64_t n, int64_t* event_index,
                                     int64_t* count_host, sfp_stream_t stream) {
-  return compact_indices(event, n, 0, event_index, count_host, S(stream));
+  return sfp_compact_indices(event, n, 0, event_index, count_host, S(stream));
 }
 
 extern "C" int sfp_bool_sum(const uint8_t* flags, int64_t n, int64_t* count_host,
                             sfp_stream_t stream) {
-  return compact_indices(flags, n, 0, nullptr, count_host, S(stream));
+  return sfp_compact_indices(flags, n, 0, nullptr, count_host, S(stream));
 }
 
 // ------------------------------------------------------ QED per-op kernels --
@@ -600,47 +533,6 @@ extern "C" int sfp_pair_from_rejection_sampling(const double* ig, const double* 
                                                 sfp_stream_t stream) {
   return rs_launch(ig, chi, dt, n, pruned, event, delta, ids, seed, step, tag, slot, tables,
                    stream, 0);
-}
-
-// one optical-depth update (optical_depth.py:42-62); returns event
-__device__ inline bool od_update(double* tau, double ig, double chi, double dt, uint64_t seed,
-                                 uint64_t id, uint64_t step, uint32_t tag, uint32_t slot,
-                                 const double2* nodes, const double* cdf, const sfp_od_dev& od,
-                                 double* delta) {
-  // total rate (optical_depth_tables.py:46-56) from {value, slope} nodes
-  double rate;
-  {
-    double log_chi = sfp_log10(chi);
-    if (log_chi > od.log_chi_max) log_chi = od.log_chi_max;
-    if (log_chi < od.log_chi_min) {
-      rate = 0.0;
-    } else {
-      double fidx = (log_chi - od.log_chi_min) / od.log_chi_delta;
-      int64_t idx = (int64_t)sfp_floor(fidx);
-      if (idx > od.chi_N - 2) idx = od.chi_N - 2;
-      double left = od.log_chi_min + (double)idx * od.log_chi_delta;
-      double2 nd = nodes[idx];
-      rate = nd.x + nd.y * (log_chi - left);
-    }
-  }
-  double dtau = dt * ig;
-  double t = *tau;
-  if (t == 0.0) {
-    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, slot + 1);
-    t = -sfp_log(1 - sfp_u01(d.v[0], d.v[1]));
-  }
-  t -= rate * dtau;
-  bool ev = false;
-  if (t < 0) {
-    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, slot + 2);
-    t = -sfp_log(1 - sfp_u01(d.v[0], d.v[1]));
-    ev = true;
-    *delta = sfp_od_inverse_cdf(cdf, &od, chi, sfp_u01(d.v[2], d.v[3]));
-  } else {
-    *delta = 0.0;
-  }
-  *tau = t;
-  return ev;
 }
 
 __global__ void k_od(double* tau, const double* ig, const double* chi, double dt, int64_t n,
@@ -709,6 +601,29 @@ extern "C" int sfp_tables_create(const sfp_rs_tables* rs, const sfp_od_tables* o
     HIPCHK(hipMalloc(&T->rs_nodes, sizeof(sfp_rs_node) * nodes.size()));
     HIPCHK(hipMemcpy(T->rs_nodes, nodes.data(), sizeof(sfp_rs_node) * nodes.size(),
                      hipMemcpyHostToDevice));
+    // Envelopes used to prove that an emission/pair rate is negligible.  On
+    // bracket j (z in [z_j, z_j+1], linear interpolation of the tables) the
+    // interpolated int Ai and |Ai'| are bounded by their node maxima, so for
+    // every z' >= z_j and c >= 0:
+    //   |int Ai(z') + (2/z' +- c sqrt z') Ai'(z')| <= E1[j] + c E2[j]
+    // with E1[j] = max_{k>=j} (max int Ai + (2/z_k) max|Ai'|) and
+    //      E2[j] = max_{k>=j} sqrt(z_k+1) max|Ai'|  (suffix maxima).
+    // Bracket 0 (z_0 = 0) has no bound.
+    std::vector<double2> env(rs->z_N - 1);
+    double e1 = 0.0, e2 = 0.0;
+    for (int64_t j = rs->z_N - 2; j >= 0; --j) {
+      double zj = rs->z_min + (double)j * rs->z_delta;
+      double zj1 = rs->z_min + (double)(j + 1) * rs->z_delta;
+      double ia = std::max(rs->int_Ai[j], rs->int_Ai[j + 1]);
+      double ap = std::max(fabs(rs->Aip[j]), fabs(rs->Aip[j + 1]));
+      double a = (zj > 0.0) ? ia + 2.0 / zj * ap : __builtin_inf();
+      double b = sqrt(zj1) * ap;
+      e1 = std::max(e1, a);
+      e2 = std::max(e2, b);
+      env[j] = make_double2(e1, e2);
+    }
+    HIPCHK(hipMalloc(&T->rs_env, sizeof(double2) * env.size()));
+    HIPCHK(hipMemcpy(T->rs_env, env.data(), sizeof(double2) * env.size(), hipMemcpyHostToDevice));
     T->z_N = rs->z_N;
     T->z_min = rs->z_min;
     T->z_max = rs->z_max;
@@ -764,6 +679,7 @@ extern "C" int sfp_tables_destroy(void* tables) {
   Tables* T = (Tables*)tables;
   if (!T) return SFP_OK;
   if (T->rs_nodes) (void)hipFree(T->rs_nodes);
+  if (T->rs_env) (void)hipFree(T->rs_env);
   if (T->od_mem) (void)hipFree(T->od_mem);
   if (T->od_photon_nodes) (void)hipFree(T->od_photon_nodes);
   if (T->od_pair_nodes) (void)hipFree(T->od_pair_nodes);
@@ -772,26 +688,8 @@ extern "C" int sfp_tables_destroy(void* tables) {
 }
 
 // --------------------------------------------------------------- stores ---
-struct Counts {
-  int64_t n;         // buffered particles (live + pruned holes)
-  int64_t next_id;   // id of the next appended particle
-  int64_t n_old;     // n at the start of the current step
-  int64_t overflow;  // number of appends that did not fit
-  int64_t psteps;     // particle-steps advanced (live after each step)
-  int64_t processed;  // particles read by this species' step kernel
-  int64_t spawned;    // particles this species' step kernel appended (per destination)
-  int64_t pad;
-};
-
-static const size_t kAttrSize[SFP_NATTR] = {8, 8, 8, 8, 8, 8, 8, 1, 8, 8, 8, 8,
-                                            8, 8, 8, 8, 8, 8, 8, 8, 1, 8};
-
-struct Store {
-  int32_t flags = 0;
-  int64_t cap = 0;
-  void* p[SFP_NATTR] = {};
-  Counts* cnt = nullptr;  // device
-};
+const size_t kAttrSize[SFP_NATTR] = {8, 8, 8, 8, 8, 8, 8, 1, 8, 8, 8, 8,
+                                     8, 8, 8, 8, 8, 8, 8, 8, 1, 8};
 
 static bool attr_core(int a) { return a <= SFP_PRUNED; }
 
@@ -938,7 +836,7 @@ extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
   int64_t* idx = nullptr;
   if (n > 0) {
     HIPCHK(hipMalloc(&idx, sizeof(int64_t) * n));
-    CHK(compact_indices((const uint8_t*)s->p[SFP_PRUNED], n, 1, idx, &kept, nullptr));
+    CHK(sfp_compact_indices((const uint8_t*)s->p[SFP_PRUNED], n, 1, idx, &kept, nullptr));
     for (int a = 0; a < SFP_NATTR; ++a) {
       if (!s->p[a] || kept == 0) continue;
       void* tmp = nullptr;
@@ -963,797 +861,6 @@ extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
   c.n_old = kept;
   HIPCHK(hipMemcpy(s->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   *n_kept = kept;
-  return SFP_OK;
-}
-
-// ------------------------------------------------------- fused simulation --
-static constexpr int TILE = 256;      // particles per tile = workgroup size
-static constexpr int WAVES = TILE / 64;
-
-struct SpeciesDev {
-  double *x, *y, *z, *ux, *uy, *uz;
-  uint64_t* id;
-  uint8_t* pruned;
-  double* tau;
-  double *sx, *sy, *sz;
-  Counts* cnt;
-  int64_t cap;
-  double q, ae;
-  int32_t rr, push, spin, photon, tag;
-  int32_t photon_dst, pair_e_dst, pair_p_dst;
-};
-
-struct ScanState {
-  uint32_t tile_ctr;
-  uint32_t done_ctr;
-  uint32_t epoch;
-  uint32_t pad;
-};
-
-#define SFP_MAX_SPECIES 8
-
-struct Plan {
-  SpeciesDev sp[SFP_MAX_SPECIES];
-  ScanState* scan[SFP_MAX_SPECIES];
-  uint64_t* flags[SFP_MAX_SPECIES];
-  sfp_fieldset fields;
-  const sfp_rs_node* rs_nodes;
-  int64_t z_N;
-  double z_min, z_max, z_delta;
-  const double2* od_photon_nodes;
-  const double2* od_pair_nodes;
-  sfp_od_dev od;
-  double threshold;
-  uint64_t seed;
-  int32_t nspecies;
-  int32_t qed;
-};
-
-struct StepArgs {
-  double t, t_half, dt, cdt_half, ll_factor;
-  uint64_t step;
-};
-
-enum { KIND_PUSH = 0, KIND_RS = 1, KIND_OD = 2 };
-
-__device__ inline uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// flag word: [epoch:24][status:2][value:38]; status 1 = aggregate, 2 = inclusive prefix
-static constexpr int FLAG_VAL_BITS = 38;
-static constexpr uint64_t FLAG_VAL_MASK = (1ull << FLAG_VAL_BITS) - 1ull;
-
-__device__ inline uint64_t flag_word(uint32_t epoch, uint32_t status, uint64_t v) {
-  return ((uint64_t)epoch << 40) | ((uint64_t)status << FLAG_VAL_BITS) | (v & FLAG_VAL_MASK);
-}
-
-// Decoupled look-back (one full wave): publish the tile aggregate, accumulate
-// predecessors until an inclusive prefix is found, publish the inclusive
-// prefix and return the exclusive prefix (uniform across the wave).
-__device__ uint64_t lookback(uint64_t* flags, int64_t tile, uint64_t agg, uint32_t epoch) {
-  int lane = wave_lane();
-  if (tile == 0) {
-    if (lane == 0) st_release(&flags[0], flag_word(epoch, 2, agg));
-    return 0;
-  }
-  if (lane == 0) st_release(&flags[tile], flag_word(epoch, 1, agg));
-  uint64_t prefix = 0;
-  int64_t j = tile - 1;
-  while (true) {
-    int64_t idx = j - lane;
-    uint64_t w;
-    uint32_t st;
-    while (true) {
-      w = (idx < 0) ? flag_word(epoch, 2, 0) : ld_acquire(&flags[idx]);
-      st = ((uint32_t)(w >> 40) == epoch) ? (uint32_t)((w >> FLAG_VAL_BITS) & 3u) : 0u;
-      if (__all(st != 0)) break;
-      __builtin_amdgcn_s_sleep(1);
-    }
-    uint64_t pm = __ballot(st == 2);
-    uint64_t v = w & FLAG_VAL_MASK;
-    int stop = pm ? (int)__builtin_ctzll(pm) : 63;
-    uint64_t contrib = (lane <= stop) ? v : 0;
-    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
-    prefix += contrib;
-    if (pm) break;
-    j -= 64;
-  }
-  if (lane == 0) st_release(&flags[tile], flag_word(epoch, 2, agg + prefix));
-  return prefix;
-}
-
-// Push of a lepton from the first position half-step to the second
-// (simulation.py:128-137 with particles.py:244-296); ig is 1/gamma on entry.
-__device__ inline void lepton_push(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
-                                   double& x, double& y, double& z, double& ux, double& uy,
-                                   double& uz, double& sx, double& sy, double& sz, double ig) {
-  sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-  double F[6];
-  sfp_field_eval(&P.fields, x, y, z, a.t_half, F);
-  if (sp.spin) {
-    sfp_boris_tbmt(&ux, &uy, &uz, &ig, &sx, &sy, &sz, F[0], F[1], F[2], F[3], F[4], F[5], sp.q,
-                   sp.ae, a.dt);
-  } else {
-    sfp_boris(&ux, &uy, &uz, &ig, F[0], F[1], F[2], F[3], F[4], F[5], sp.q, a.dt);
-  }
-  if (sp.rr == SFP_RR_LL || (sp.rr == SFP_RR_CLL && !sp.spin)) {
-    double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
-    double factor = (sp.rr == SFP_RR_CLL) ? a.ll_factor * sfp_cll_g(chi) : a.ll_factor;
-    sfp_ll_apply(&ux, &uy, &uz, &ig, chi, factor);
-  }
-  sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-}
-
-// Append one new lepton (pair member) at slot d of species `sp`, pushed.
-__device__ inline void spawn_lepton(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
-                                    int64_t d, uint64_t id, double x, double y, double z,
-                                    double ux, double uy, double uz) {
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  if (sp.push) {
-    double ig = 1.0 / sfp_sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-    lepton_push(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
-  }
-  sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
-  sp.ux[d] = ux; sp.uy[d] = uy; sp.uz[d] = uz;
-  sp.id[d] = id;
-  sp.pruned[d] = 0;
-  if (sp.tau) sp.tau[d] = 0.0;
-  if (sp.spin) { sp.sx[d] = 0.0; sp.sy[d] = 0.0; sp.sz[d] = 0.0; }
-}
-
-// Append one new photon at slot d of species `sp`, pushed.
-__device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int64_t d, uint64_t id,
-                                    double x, double y, double z, double ux, double uy,
-                                    double uz) {
-  if (sp.push) {
-    double ig = sfp_inv_gamma_photon(ux, uy, uz);
-    sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-    sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-  }
-  sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
-  sp.ux[d] = ux; sp.uy[d] = uy; sp.uz[d] = uz;
-  sp.id[d] = id;
-  sp.pruned[d] = 0;
-  if (sp.tau) sp.tau[d] = 0.0;
-}
-
-template <int KIND, bool PHOTON>
-__global__ void __launch_bounds__(TILE) k_advance(const Plan* __restrict__ plan, int s, StepArgs a) {
-  const Plan& P = *plan;
-  const SpeciesDev& sp = P.sp[s];
-  constexpr bool QED = KIND != KIND_PUSH;
-
-  __shared__ sfp_rs_node lds_rs[KIND == KIND_RS ? 1023 : 1];
-  __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
-  __shared__ int64_t sh_tile;
-  __shared__ uint32_t sh_wcnt[WAVES];
-  __shared__ uint32_t sh_alive[WAVES];
-  __shared__ uint64_t sh_prefix;
-
-  const int64_t n_old = sp.cnt->n_old;
-  const int64_t ntiles = (n_old + TILE - 1) / TILE;
-  ScanState* scan = P.scan[s];
-  uint32_t epoch = 0;
-  if (QED) {
-    epoch = (scan->epoch + 1u) & 0xffffffu;
-    if (epoch == 0) epoch = 1;
-  }
-
-  // destinations of appended particles
-  const SpeciesDev* dA = nullptr;  // photon dst, or electron dst
-  const SpeciesDev* dB = nullptr;  // positron dst
-  int64_t baseA = 0, baseB = 0, idA = 0, idB = 0;
-  if (QED) {
-    if (!PHOTON) {
-      if (sp.photon_dst >= 0) dA = &P.sp[sp.photon_dst];
-    } else {
-      if (sp.pair_e_dst >= 0) dA = &P.sp[sp.pair_e_dst];
-      if (sp.pair_p_dst >= 0) dB = &P.sp[sp.pair_p_dst];
-    }
-    if (dA) { baseA = dA->cnt->n; idA = dA->cnt->next_id; }
-    if (dB) { baseB = dB->cnt->n; idB = dB->cnt->next_id; }
-  }
-
-  if (KIND == KIND_RS) {
-    int64_t nn = P.z_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) lds_rs[k] = P.rs_nodes[k];
-  }
-  if (KIND == KIND_OD) {
-    const double2* src = PHOTON ? P.od_pair_nodes : P.od_photon_nodes;
-    int64_t nn = P.od.chi_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = src[k];
-  }
-  __syncthreads();
-
-  const int lane = wave_lane();
-  const int wv = threadIdx.x / 64;
-  uint32_t alive_acc = 0;
-  uint32_t proc_acc = 0;
-
-  while (true) {
-    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
-    __syncthreads();
-    const int64_t tile = sh_tile;
-    __syncthreads();
-    if (tile >= ntiles) break;
-
-    const int64_t i = tile * TILE + threadIdx.x;
-    bool active = false;
-    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
-    uint64_t pid = 0;
-    if (i < n_old) {
-      active = sp.pruned[i] == 0;
-      if (active) {
-        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-        if (KIND == KIND_OD) tau = sp.tau[i];
-        if (!PHOTON && sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
-        if (QED) pid = sp.id[i];
-      }
-    }
-    double ig = 0.0;
-    if (active) {
-      ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
-      proc_acc += 1;
-    }
-
-    // ---- QED front at time t (simulation.py:103-114) ------------------
-    bool event = false;
-    double delta = 0.0;
-    if (QED && active && sp.push) {
-      double F[6];
-      sfp_field_eval(&P.fields, x, y, z, a.t, F);
-      double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
-      if (KIND == KIND_RS) {
-        if (chi != 0.0) {
-          sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
-          double r1 = sfp_u01(d.v[0], d.v[1]);
-          double r2 = sfp_u01(d.v[2], d.v[3]);
-          double dtau = a.dt * ig;
-          if (!PHOTON) {
-            double r1_3 = sfp_powi(r1, 3);
-            double p = 3 * sfp_powi(r1, 2) *
-                       sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1) * dtau;
-            if (r2 < p) { event = true; delta = r1_3; }
-          } else {
-            double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
-            if (r2 < p) { event = true; delta = r1; }
-          }
-        }
-      } else if (KIND == KIND_OD) {
-        if (chi >= P.od.chi_min) {  // below the table or NaN: no event
-          event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
-                            PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
-        }
-      }
-      // cpu.py:227-232 (radiating leptons only)
-      if (!PHOTON && event) {
-        if (delta / ig < P.threshold) event = false;
-      }
-    }
-    if (!dA && !dB) event = false;  // nothing to append to: the event has no effect
-
-    // ---- stable rank of the events (wave ballot + tile scan + look-back) --
-    if (QED) {
-      uint64_t m = __ballot(event);
-      uint32_t lrank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(m);
-      __syncthreads();
-      uint32_t woff = 0, agg = 0;
-      for (int k = 0; k < WAVES; ++k) {
-        uint32_t c = sh_wcnt[k];
-        if (k < wv) woff += c;
-        agg += c;
-      }
-      if (wv == 0) {
-        uint64_t pre = lookback(P.flags[s], tile, agg, epoch);
-        if (lane == 0) sh_prefix = pre;
-      }
-      __syncthreads();
-      if (event) {
-        int64_t rank = (int64_t)sh_prefix + woff + lrank;
-        if (!PHOTON) {
-          // cpu.py:90-118: photon at x(t) with momentum delta*u(t)
-          int64_t d = baseA + rank;
-          if (d < dA->cap)
-            spawn_photon(*dA, a, d, (uint64_t)(idA + rank), x, y, z, delta * ux, delta * uy,
-                         delta * uz);
-          else
-            atomicAdd((unsigned long long*)&dA->cnt->overflow, 1ull);
-        } else {
-          // cpu.py:120-191: electron takes delta, positron 1 - delta
-          double dp = 1.0 - delta;
-          if (dA) {
-            int64_t d = baseA + rank;
-            if (d < dA->cap)
-              spawn_lepton(P, *dA, a, d, (uint64_t)(idA + rank), x, y, z, delta * ux, delta * uy,
-                           delta * uz);
-            else
-              atomicAdd((unsigned long long*)&dA->cnt->overflow, 1ull);
-          }
-          if (dB) {
-            int64_t d = baseB + rank;
-            if (d < dB->cap)
-              spawn_lepton(P, *dB, a, d, (uint64_t)(idB + rank), x, y, z, dp * ux, dp * uy,
-                           dp * uz);
-            else
-              atomicAdd((unsigned long long*)&dB->cnt->overflow, 1ull);
-          }
-        }
-      }
-    }
-
-    // ---- recoil (cpu.py:78-88) and push (simulation.py:128-137) -------
-    if (active) {
-      if (PHOTON) {
-        if (event) {
-          sp.pruned[i] = 1;  // the converted photon becomes a hole (cpu.py:189-190)
-        } else {
-          if (sp.push) {
-            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-            sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-          }
-          if (KIND == KIND_OD) sp.tau[i] = tau;
-          alive_acc += 1;
-        }
-      } else {
-        if (event && sp.rr == SFP_RR_PHOTON) {
-          ux *= 1 - delta;
-          uy *= 1 - delta;
-          uz *= 1 - delta;
-          ig = sfp_inv_gamma_lepton(ux, uy, uz);
-        }
-        if (sp.push) {
-          lepton_push(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
-          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-          sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
-          if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
-        } else if (event && sp.rr == SFP_RR_PHOTON) {
-          sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
-        }
-        if (KIND == KIND_OD) sp.tau[i] = tau;
-        alive_acc += 1;
-      }
-    }
-  }
-
-  // ---- workgroup exit: live count; the last workgroup publishes totals ---
-  {
-    uint32_t c = alive_acc, pc = proc_acc;
-    for (int off = 32; off > 0; off >>= 1) {
-      c += __shfl_xor(c, off);
-      pc += __shfl_xor(pc, off);
-    }
-    if (lane == 0) {
-      sh_alive[wv] = c;
-      sh_wcnt[wv] = pc;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint64_t tot = 0, ptot = 0;
-      for (int k = 0; k < WAVES; ++k) {
-        tot += sh_alive[k];
-        ptot += sh_wcnt[k];
-      }
-      if (tot) atomicAdd((unsigned long long*)&sp.cnt->psteps, (unsigned long long)tot);
-      if (ptot) atomicAdd((unsigned long long*)&sp.cnt->processed, (unsigned long long)ptot);
-      if (QED) {
-        __threadfence();
-        uint32_t prev = atomicAdd(&scan->done_ctr, 1u);
-        if (prev == gridDim.x - 1) {
-          __threadfence();
-          uint64_t total = 0;
-          if (ntiles > 0) total = ld_acquire(&P.flags[s][ntiles - 1]) & FLAG_VAL_MASK;
-          if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
-          if (dA) {
-            dA->cnt->n = baseA + (int64_t)total < dA->cap ? baseA + (int64_t)total : dA->cap;
-            dA->cnt->next_id = idA + (int64_t)total;
-            atomicAdd((unsigned long long*)&dA->cnt->psteps, (unsigned long long)total);
-          }
-          if (dB) {
-            dB->cnt->n = baseB + (int64_t)total < dB->cap ? baseB + (int64_t)total : dB->cap;
-            dB->cnt->next_id = idB + (int64_t)total;
-            atomicAdd((unsigned long long*)&dB->cnt->psteps, (unsigned long long)total);
-          }
-          scan->tile_ctr = 0;
-          scan->done_ctr = 0;
-          scan->epoch = epoch;
-          __threadfence();
-        }
-      } else {
-        __threadfence();
-        uint32_t prev = atomicAdd(&scan->done_ctr, 1u);
-        if (prev == gridDim.x - 1) {
-          scan->tile_ctr = 0;
-          scan->done_ctr = 0;
-          __threadfence();
-        }
-      }
-    }
-  }
-}
-
-// prologue of a step: snapshot the counts the species kernels iterate over
-__global__ void k_step_prologue(const Plan* plan) {
-  int s = threadIdx.x;
-  if (s < plan->nspecies) {
-    Counts* c = plan->sp[s].cnt;
-    c->n_old = c->n;
-  }
-}
-
-static const void* pick_kernel(int kind, bool photon) {
-  if (photon) {
-    if (kind == KIND_RS) return (const void*)k_advance<KIND_RS, true>;
-    if (kind == KIND_OD) return (const void*)k_advance<KIND_OD, true>;
-    return (const void*)k_advance<KIND_PUSH, true>;
-  }
-  if (kind == KIND_RS) return (const void*)k_advance<KIND_RS, false>;
-  if (kind == KIND_OD) return (const void*)k_advance<KIND_OD, false>;
-  return (const void*)k_advance<KIND_PUSH, false>;
-}
-
-static void launch_advance(int kind, bool photon, int grid, hipStream_t st, const Plan* plan,
-                           int s, StepArgs a) {
-  dim3 g(grid), b(TILE);
-  if (photon) {
-    if (kind == KIND_RS) hipLaunchKernelGGL((k_advance<KIND_RS, true>), g, b, 0, st, plan, s, a);
-    else if (kind == KIND_OD) hipLaunchKernelGGL((k_advance<KIND_OD, true>), g, b, 0, st, plan, s, a);
-    else hipLaunchKernelGGL((k_advance<KIND_PUSH, true>), g, b, 0, st, plan, s, a);
-  } else {
-    if (kind == KIND_RS) hipLaunchKernelGGL((k_advance<KIND_RS, false>), g, b, 0, st, plan, s, a);
-    else if (kind == KIND_OD) hipLaunchKernelGGL((k_advance<KIND_OD, false>), g, b, 0, st, plan, s, a);
-    else hipLaunchKernelGGL((k_advance<KIND_PUSH, false>), g, b, 0, st, plan, s, a);
-  }
-}
-
-struct Sim {
-  int nsp = 0;
-  std::vector<sfp_species_desc> desc;
-  std::vector<Store*> stores;
-  std::vector<int> kind;           // KIND_*
-  std::vector<int> grid;           // persistent workgroups per species kernel
-  std::vector<int64_t> flags_cap;  // tiles the flag array can hold
-  std::vector<int64_t> n_ub;       // host-side upper bounds of the counts
-  ScanState* scan_mem = nullptr;
-  std::vector<uint64_t*> flags;
-  Plan plan{};
-  Plan* plan_dev = nullptr;
-  Tables* tables = nullptr;
-  sfp_sim_params params{};
-  bool plan_dirty = true;
-  int cus = 0;
-  // timing
-  std::vector<std::vector<hipEvent_t>> ev_pending;  // per species: start/stop pairs
-  std::vector<double> ms_acc;
-  std::vector<int64_t> launches;
-  std::vector<hipEvent_t> ev_pool;
-};
-
-static int sim_sync_counts(Sim* sim) {
-  HIPCHK(hipDeviceSynchronize());
-  for (int s = 0; s < sim->nsp; ++s) {
-    Counts c{};
-    HIPCHK(hipMemcpy(&c, sim->stores[s]->cnt, sizeof(c), hipMemcpyDeviceToHost));
-    if (c.overflow)
-      return set_err(SFP_E_CAPACITY,
-                     "species %d: %lld appended particles did not fit its capacity %lld", s,
-                     (long long)c.overflow, (long long)sim->stores[s]->cap);
-    sim->n_ub[s] = c.n;
-  }
-  return SFP_OK;
-}
-
-static int sim_refresh_plan(Sim* sim) {
-  Plan& P = sim->plan;
-  for (int s = 0; s < sim->nsp; ++s) {
-    Store* st = sim->stores[s];
-    SpeciesDev& d = P.sp[s];
-    d.x = (double*)st->p[SFP_X];
-    d.y = (double*)st->p[SFP_Y];
-    d.z = (double*)st->p[SFP_Z];
-    d.ux = (double*)st->p[SFP_UX];
-    d.uy = (double*)st->p[SFP_UY];
-    d.uz = (double*)st->p[SFP_UZ];
-    d.id = (uint64_t*)st->p[SFP_ID];
-    d.pruned = (uint8_t*)st->p[SFP_PRUNED];
-    d.tau = (double*)st->p[SFP_TAU];
-    d.sx = (double*)st->p[SFP_SX];
-    d.sy = (double*)st->p[SFP_SY];
-    d.sz = (double*)st->p[SFP_SZ];
-    d.cnt = st->cnt;
-    d.cap = st->cap;
-    // look-back flags: one word per tile of the capacity
-    int64_t need = st->cap / TILE + 2;
-    if (need > sim->flags_cap[s]) {
-      if (sim->flags[s]) HIPCHK(hipFree(sim->flags[s]));
-      int64_t want = need + need / 2;
-      HIPCHK(hipMalloc(&sim->flags[s], sizeof(uint64_t) * want));
-      HIPCHK(hipMemset(sim->flags[s], 0, sizeof(uint64_t) * want));
-      sim->flags_cap[s] = want;
-    }
-    P.flags[s] = sim->flags[s];
-    P.scan[s] = sim->scan_mem + s;
-  }
-  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
-  sim->plan_dirty = false;
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
-                              const sfp_field* terms, int32_t nterms, void* tables,
-                              const sfp_sim_params* params, void** out) {
-  if (nspecies < 1 || nspecies > SFP_MAX_SPECIES)
-    return set_err(SFP_E_ARG, "between 1 and %d species are supported", SFP_MAX_SPECIES);
-  Sim* sim = new Sim();
-  sim->nsp = nspecies;
-  sim->params = *params;
-  sim->tables = (Tables*)tables;
-  sim->desc.assign(species, species + nspecies);
-  Plan& P = sim->plan;
-  memset(&P, 0, sizeof(P));
-  CHK(compile_fields(terms, nterms, &P.fields));
-  P.nspecies = nspecies;
-  P.seed = params->seed;
-  P.threshold = params->photon_threshold;
-  P.qed = params->qed_model;
-  bool need_rs = false, need_od = false;
-  for (int s = 0; s < nspecies; ++s) {
-    const sfp_species_desc& d = species[s];
-    if (!d.store) return set_err(SFP_E_ARG, "species %d has no store", s);
-    Store* st = (Store*)d.store;
-    sim->stores.push_back(st);
-    bool photon = d.m == 0.0;
-    bool qed = photon ? (d.pair_e_dst >= 0 || d.pair_p_dst >= 0) : (d.photon_dst >= 0);
-    int kind = !qed ? KIND_PUSH : (params->qed_model == SFP_QED_OD ? KIND_OD : KIND_RS);
-    if (kind == KIND_RS) need_rs = true;
-    if (kind == KIND_OD) {
-      need_od = true;
-      if (!(st->flags & SFP_STORE_TAU) || !st->p[SFP_TAU])
-        return set_err(SFP_E_ARG, "species %d uses the optical-depth model but has no tau", s);
-    }
-    if (!photon && d.spin && !st->p[SFP_SX])
-      return set_err(SFP_E_ARG, "species %d is a spin species without spin storage", s);
-    for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst})
-      if (dst >= nspecies) return set_err(SFP_E_ARG, "species %d: destination out of range", s);
-    if (!photon && (d.pair_e_dst >= 0 || d.pair_p_dst >= 0))
-      return set_err(SFP_E_ARG, "species %d: only photons create pairs", s);
-    if (photon && d.photon_dst >= 0)
-      return set_err(SFP_E_ARG, "species %d: photons do not radiate photons", s);
-    sim->kind.push_back(kind);
-    SpeciesDev& sd = P.sp[s];
-    sd.q = d.q;
-    sd.ae = d.ae;
-    sd.rr = d.rr;
-    sd.push = d.push;
-    sd.spin = photon ? 0 : d.spin;
-    sd.photon = photon;
-    sd.tag = s;
-    sd.photon_dst = d.photon_dst;
-    sd.pair_e_dst = d.pair_e_dst;
-    sd.pair_p_dst = d.pair_p_dst;
-  }
-  if (need_rs) {
-    if (!sim->tables || !sim->tables->has_rs)
-      return set_err(SFP_E_ARG, "rejection-sampling tables required");
-    if (sim->tables->z_N - 1 > 1023)
-      return set_err(SFP_E_ARG, "rejection-sampling table larger than 1024 nodes");
-    P.rs_nodes = sim->tables->rs_nodes;
-    P.z_N = sim->tables->z_N;
-    P.z_min = sim->tables->z_min;
-    P.z_max = sim->tables->z_max;
-    P.z_delta = sim->tables->z_delta;
-  }
-  if (need_od) {
-    if (!sim->tables || !sim->tables->has_od)
-      return set_err(SFP_E_ARG, "optical-depth tables required");
-    if (sim->tables->od.chi_N - 1 > 255)
-      return set_err(SFP_E_ARG, "optical-depth table larger than 256 chi nodes");
-    P.od = sim->tables->od;
-    P.od_photon_nodes = sim->tables->od_photon_nodes;
-    P.od_pair_nodes = sim->tables->od_pair_nodes;
-  }
-  HIPCHK(hipMalloc(&sim->plan_dev, sizeof(Plan)));
-  HIPCHK(hipMalloc(&sim->scan_mem, sizeof(ScanState) * nspecies));
-  HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * nspecies));
-  sim->flags.assign(nspecies, nullptr);
-  sim->flags_cap.assign(nspecies, 0);
-  sim->n_ub.assign(nspecies, 0);
-  sim->ms_acc.assign(nspecies, 0.0);
-  sim->launches.assign(nspecies, 0);
-  sim->ev_pending.resize(nspecies);
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, dev));
-  sim->cus = prop.multiProcessorCount;
-  for (int s = 0; s < nspecies; ++s) {
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, pick_kernel(sim->kind[s], P.sp[s].photon), TILE, 0));
-    if (per_cu < 1) per_cu = 1;
-    sim->grid.push_back(per_cu * sim->cus);
-  }
-  CHK(sim_refresh_plan(sim));
-  CHK(sim_sync_counts(sim));
-  *out = sim;
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_destroy(void* p) {
-  Sim* sim = (Sim*)p;
-  if (!sim) return SFP_OK;
-  (void)hipDeviceSynchronize();
-  for (auto f : sim->flags)
-    if (f) (void)hipFree(f);
-  if (sim->scan_mem) (void)hipFree(sim->scan_mem);
-  if (sim->plan_dev) (void)hipFree(sim->plan_dev);
-  for (auto& v : sim->ev_pending)
-    for (auto e : v) (void)hipEventDestroy(e);
-  for (auto e : sim->ev_pool) (void)hipEventDestroy(e);
-  delete sim;
-  return SFP_OK;
-}
-
-// make sure every append of the next step fits: worst case every source
-// particle creates one particle in each of its destinations
-static int sim_ensure_capacity(Sim* sim) {
-  std::vector<int64_t> need(sim->n_ub);
-  for (int s = 0; s < sim->nsp; ++s) {
-    const sfp_species_desc& d = sim->desc[s];
-    if (sim->kind[s] == KIND_PUSH) continue;
-    for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst})
-      if (dst >= 0) need[dst] += sim->n_ub[s];
-  }
-  bool fits = true;
-  for (int s = 0; s < sim->nsp; ++s)
-    if (need[s] > sim->stores[s]->cap) fits = false;
-  if (fits) {
-    sim->n_ub = need;
-    return SFP_OK;
-  }
-  // exact counts, then grow what still does not fit
-  CHK(sim_sync_counts(sim));
-  need = sim->n_ub;
-  for (int s = 0; s < sim->nsp; ++s) {
-    const sfp_species_desc& d = sim->desc[s];
-    if (sim->kind[s] == KIND_PUSH) continue;
-    for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst})
-      if (dst >= 0) need[dst] += sim->n_ub[s];
-  }
-  for (int s = 0; s < sim->nsp; ++s) {
-    Store* st = sim->stores[s];
-    if (need[s] > st->cap) {
-      int64_t want = need[s] + need[s] / 2 + 1024;
-      CHK(sfp_store_reserve(st, want));
-      sim->plan_dirty = true;
-    }
-  }
-  sim->n_ub = need;
-  if (sim->plan_dirty) CHK(sim_refresh_plan(sim));
-  return SFP_OK;
-}
-
-static hipEvent_t sim_event(Sim* sim) {
-  if (!sim->ev_pool.empty()) {
-    hipEvent_t e = sim->ev_pool.back();
-    sim->ev_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {
-  Sim* sim = (Sim*)p;
-  if (!sim) return set_err(SFP_E_ARG, "null simulation");
-  // stores may have been resized or refilled by the caller since the last run
-  for (int s = 0; s < sim->nsp; ++s)
-    if (sim->plan.sp[s].cap != sim->stores[s]->cap || sim->plan.sp[s].x != sim->stores[s]->p[SFP_X])
-      sim->plan_dirty = true;
-  if (sim->plan_dirty) CHK(sim_refresh_plan(sim));
-  CHK(sim_sync_counts(sim));
-  const double dt = sim->params.dt;
-  const bool timing = flags & SFP_RUN_TIMING;
-  for (int64_t k = 0; k < nsteps; ++k) {
-    CHK(sim_ensure_capacity(sim));
-    StepArgs a;
-    a.t = *t;
-    a.t_half = *t + 0.5 * dt;
-    a.dt = dt;
-    a.cdt_half = SFP_C * (0.5 * dt);
-    a.ll_factor = sfp_ll_factor(dt);
-    a.step = step0 + (uint64_t)k;
-    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, 0, (const Plan*)sim->plan_dev);
-    for (int s = 0; s < sim->nsp; ++s) {
-      int64_t tiles_ub = (sim->n_ub[s] + TILE - 1) / TILE;
-      int g = (int)std::max<int64_t>(1, std::min<int64_t>(sim->grid[s], tiles_ub));
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (timing) {
-        e0 = sim_event(sim);
-        e1 = sim_event(sim);
-        if (e0) HIPCHK(hipEventRecord(e0, 0));
-      }
-      launch_advance(sim->kind[s], sim->plan.sp[s].photon, g, 0, (const Plan*)sim->plan_dev, s, a);
-      HIPCHK(hipGetLastError());
-      if (timing && e1) {
-        HIPCHK(hipEventRecord(e1, 0));
-        sim->ev_pending[s].push_back(e0);
-        sim->ev_pending[s].push_back(e1);
-      }
-    }
-    *t = *t + dt;
-  }
-  CHK(sim_sync_counts(sim));
-  return SFP_OK;
-}
-
-static int sim_collect_timing(Sim* sim) {
-  HIPCHK(hipDeviceSynchronize());
-  for (int s = 0; s < sim->nsp; ++s) {
-    auto& v = sim->ev_pending[s];
-    for (size_t k = 0; k + 1 < v.size(); k += 2) {
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, v[k], v[k + 1]));
-      sim->ms_acc[s] += ms;
-      sim->launches[s] += 1;
-      sim->ev_pool.push_back(v[k]);
-      sim->ev_pool.push_back(v[k + 1]);
-    }
-    v.clear();
-  }
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_kernel_time(void* p, int32_t species, double* ms, int64_t* launches) {
-  Sim* sim = (Sim*)p;
-  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
-  CHK(sim_collect_timing(sim));
-  *ms = sim->ms_acc[species];
-  *launches = sim->launches[species];
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_reset_timing(void* p) {
-  Sim* sim = (Sim*)p;
-  if (!sim) return set_err(SFP_E_ARG, "null simulation");
-  CHK(sim_collect_timing(sim));
-  for (int s = 0; s < sim->nsp; ++s) {
-    sim->ms_acc[s] = 0.0;
-    sim->launches[s] = 0;
-  }
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
-  Sim* sim = (Sim*)p;
-  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
-  Counts c{};
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipMemcpy(&c, sim->stores[species]->cnt, sizeof(c), hipMemcpyDeviceToHost));
-  stats[0] = c.psteps;
-  stats[1] = c.processed;
-  stats[2] = c.spawned;
-  stats[3] = c.n;
-  return SFP_OK;
-}
-
-extern "C" int sfp_sim_reset_stats(void* p) {
-  Sim* sim = (Sim*)p;
-  if (!sim) return set_err(SFP_E_ARG, "null simulation");
-  HIPCHK(hipDeviceSynchronize());
-  for (int s = 0; s < sim->nsp; ++s) {
-    Counts c{};
-    HIPCHK(hipMemcpy(&c, sim->stores[s]->cnt, sizeof(c), hipMemcpyDeviceToHost));
-    c.psteps = c.processed = c.spawned = 0;
-    HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
-  }
   return SFP_OK;
 }
 

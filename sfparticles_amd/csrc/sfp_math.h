@@ -192,28 +192,24 @@ SFP_HD double sfp_reduce_pio2(double x, int* q) {
   return r;
 }
 
+/* Both polynomials are evaluated and the quadrant selects (no divergent
+ * branches across a wavefront). */
 SFP_HD double sfp_sin(double x) {
   if (x != x || x == __builtin_inf() || x == -__builtin_inf()) return __builtin_nan("");
   int q;
   double r = sfp_reduce_pio2(x, &q);
-  switch (q & 3) {
-    case 0: return sfp_sin_poly(r);
-    case 1: return sfp_cos_poly(r);
-    case 2: return -sfp_sin_poly(r);
-    default: return -sfp_cos_poly(r);
-  }
+  double s = sfp_sin_poly(r), c = sfp_cos_poly(r);
+  double v = (q & 1) ? c : s;
+  return (q & 2) ? -v : v;
 }
 
 SFP_HD double sfp_cos(double x) {
   if (x != x || x == __builtin_inf() || x == -__builtin_inf()) return __builtin_nan("");
   int q;
   double r = sfp_reduce_pio2(x, &q);
-  switch (q & 3) {
-    case 0: return sfp_cos_poly(r);
-    case 1: return -sfp_sin_poly(r);
-    case 2: return -sfp_cos_poly(r);
-    default: return sfp_sin_poly(r);
-  }
+  double s = sfp_sin_poly(r), c = sfp_cos_poly(r);
+  double v = (q & 1) ? s : c;
+  return ((q + 1) & 2) ? -v : v;
 }
 
 /* --------------------------------------------------------------- atan -- */

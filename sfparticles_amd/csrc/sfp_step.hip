@@ -1,0 +1,1057 @@
+// sfp_step.hip -- libsfp_hip.so, part 2: the fused simulation step
+// (sfparticles/simulation.py:102-144).
+//
+// Every species advances with two persistent kernels per step:
+//
+//  sweep  -- one pass over the species' store.  For each live particle an
+//            upper bound of chi from the field envelopes (no carrier phase,
+//            no Gouy trigonometry) decides whether a QED event is possible at
+//            all this step.  Particles that provably cannot emit (or
+//            convert) are pushed right away; the others are appended, in
+//            ascending index order, to the species' QED list.
+//  hot    -- over the QED list only: field at t, chi, Philox draw, table
+//            rate, hard-photon cut; the stable rank of every event comes from
+//            a wave64 ballot + workgroup scan + decoupled look-back across
+//            tiles; new particles are appended at (count + rank), already
+//            pushed (the reference pushes particles in the step that creates
+//            them); then recoil and push.
+//
+// The culling is exact: a culled particle's exact QED evaluation could only
+// have produced "no event", so results are bit-identical to evaluating every
+// particle (the CPU oracle does exactly that).
+//
+// Capacity: counts stay on the device.  Before appending, the hot kernel
+// checks that every destination has room for one new particle per listed
+// source particle; if not, it raises a device-side halt flag and every later
+// kernel returns at entry without touching state.  The host notices the flag
+// at its next synchronisation, grows the destination store and replays the
+// step from the halted kernel.  No worst-case reservation and no per-step
+// host synchronisation are needed.
+
+#include "sfp_internal.h"
+
+static constexpr int TILE = 256;
+static constexpr int WAVES = TILE / 64;
+#define SFP_MAX_SPECIES 8
+
+enum { KIND_RS = 0, KIND_OD = 1 };
+
+struct SpeciesDev {
+  double *x, *y, *z, *ux, *uy, *uz;
+  uint64_t* id;
+  uint8_t* pruned;
+  double* tau;
+  double *sx, *sy, *sz;
+  uint32_t* hot;  // this step's QED list (ascending particle indices)
+  Counts* cnt;
+  int64_t cap;
+  double q, ae;
+  int32_t rr, push, spin, photon, tag, qed;
+  int32_t photon_dst, pair_e_dst, pair_p_dst, pad;
+};
+
+struct ScanState {
+  uint32_t tile_ctr;
+  uint32_t done_ctr;
+  uint32_t epoch;
+  uint32_t pad;
+};
+
+struct Halt {
+  int32_t flag;
+  int32_t species;
+  int64_t step;
+};
+
+struct Plan {
+  SpeciesDev sp[SFP_MAX_SPECIES];
+  ScanState* scan[SFP_MAX_SPECIES][2];  // [species][0 sweep list, 1 event ranks]
+  uint64_t* flags[SFP_MAX_SPECIES][2];
+  sfp_fieldset fields;
+  const sfp_rs_node* rs_nodes;
+  const double2* rs_env;  // rate envelopes (Tables::rs_env)
+  double rate_unit;       // alpha m c^2 / hbar
+  int64_t z_N;
+  double z_min, z_max, z_delta;
+  const double2* od_photon_nodes;
+  const double2* od_pair_nodes;
+  sfp_od_dev od;
+  double threshold;
+  double chi_coef;  // 2 e hbar / (m^2 c^3): the bound's constant with a factor 2 margin
+  double zmax15;    // z_max^1.5: rate is 0 unless delta/(1-delta) <= zmax15 chi (photon)
+  uint64_t seed;
+  Halt* halt;
+  int32_t nspecies;
+  int32_t qed;
+};
+
+struct StepArgs {
+  double t, t_half, dt, cdt_half, ll_factor;
+  uint64_t step;
+};
+
+// --------------------------------------------------------- look-back scan --
+// Flag words carry their own payload and no other data is exchanged between
+// workgroups inside a launch, so relaxed agent-scope atomics suffice: they
+// are coherent across the XCDs' L2s without the cache write-back/invalidate
+// that acquire/release ordering costs on every publish.
+__device__ inline uint64_t ld_flag(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_flag(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// flag word: [epoch:24][status:2][value:38]; status 1 = aggregate, 2 = inclusive prefix
+static constexpr int FLAG_VAL_BITS = 38;
+static constexpr uint64_t FLAG_VAL_MASK = (1ull << FLAG_VAL_BITS) - 1ull;
+
+__device__ inline uint64_t flag_word(uint32_t epoch, uint32_t status, uint64_t v) {
+  return ((uint64_t)epoch << 40) | ((uint64_t)status << FLAG_VAL_BITS) | (v & FLAG_VAL_MASK);
+}
+
+// Decoupled look-back (one full wave): publish the tile aggregate, accumulate
+// predecessors until an inclusive prefix is found, publish the inclusive
+// prefix and return the exclusive prefix (uniform across the wave).
+__device__ uint64_t lookback(uint64_t* flags, int64_t tile, uint64_t agg, uint32_t epoch) {
+  int lane = wave_lane();
+  if (tile == 0) {
+    if (lane == 0) st_flag(&flags[0], flag_word(epoch, 2, agg));
+    return 0;
+  }
+  if (lane == 0) st_flag(&flags[tile], flag_word(epoch, 1, agg));
+  uint64_t prefix = 0;
+  int64_t j = tile - 1;
+  while (true) {
+    int64_t idx = j - lane;
+    uint64_t w;
+    uint32_t st;
+    while (true) {
+      w = (idx < 0) ? flag_word(epoch, 2, 0) : ld_flag(&flags[idx]);
+      st = ((uint32_t)(w >> 40) == epoch) ? (uint32_t)((w >> FLAG_VAL_BITS) & 3u) : 0u;
+      if (__all(st != 0)) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    uint64_t pm = __ballot(st == 2);
+    uint64_t v = w & FLAG_VAL_MASK;
+    int stop = pm ? (int)__builtin_ctzll(pm) : 63;
+    uint64_t contrib = (lane <= stop) ? v : 0;
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+    prefix += contrib;
+    if (pm) break;
+    j -= 64;
+  }
+  if (lane == 0) st_flag(&flags[tile], flag_word(epoch, 2, agg + prefix));
+  return prefix;
+}
+
+// Stable rank of `pred` among all lanes of all tiles processed so far:
+// exclusive prefix over (tile, wave, lane).  Called by every thread.
+
+__device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, uint32_t epoch,
+                                    uint32_t* sh_wcnt, uint64_t* sh_pref) {
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  uint64_t m = __ballot(pred);
+  uint32_t lrank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) sh_wcnt[wv] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t woff = 0, agg = 0;
+  for (int k = 0; k < WAVES; ++k) {
+    uint32_t c = sh_wcnt[k];
+    if (k < wv) woff += c;
+    agg += c;
+  }
+  if (wv == 0) {
+    uint64_t pre = lookback(flags, tile, agg, epoch);
+    if (lane == 0) *sh_pref = pre;
+  }
+  __syncthreads();
+  return (int64_t)(*sh_pref) + woff + lrank;
+}
+
+// -------------------------------------------------------- particle push --
+// Push of a lepton from the first position half-step to the second
+// (simulation.py:128-137 with particles.py:244-296); ig is 1/gamma on entry.
+template <int FM>
+__device__ inline void lepton_push(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                   double& x, double& y, double& z, double& ux, double& uy,
+                                   double& uz, double& sx, double& sy, double& sz, double ig) {
+  sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+  double F[6];
+  sfp_field_eval<FM>(&P.fields, x, y, z, a.t_half, F);
+  if (sp.spin) {
+    sfp_boris_tbmt(&ux, &uy, &uz, &ig, &sx, &sy, &sz, F[0], F[1], F[2], F[3], F[4], F[5], sp.q,
+                   sp.ae, a.dt);
+  } else {
+    sfp_boris(&ux, &uy, &uz, &ig, F[0], F[1], F[2], F[3], F[4], F[5], sp.q, a.dt);
+  }
+  if (sp.rr == SFP_RR_LL || (sp.rr == SFP_RR_CLL && !sp.spin)) {
+    double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
+    double factor = (sp.rr == SFP_RR_CLL) ? a.ll_factor * sfp_cll_g(chi) : a.ll_factor;
+    sfp_ll_apply(&ux, &uy, &uz, &ig, chi, factor);
+  }
+  sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+}
+
+// Append one new lepton (pair member) at slot d of species `sp`, pushed.
+template <int FM>
+__device__ inline void spawn_lepton(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                    int64_t d, uint64_t id, double x, double y, double z,
+                                    double ux, double uy, double uz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  if (sp.push) {
+    double ig = 1.0 / sfp_sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+  }
+  sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
+  sp.ux[d] = ux; sp.uy[d] = uy; sp.uz[d] = uz;
+  sp.id[d] = id;
+  sp.pruned[d] = 0;
+  if (sp.tau) sp.tau[d] = 0.0;
+  if (sp.spin) { sp.sx[d] = 0.0; sp.sy[d] = 0.0; sp.sz[d] = 0.0; }
+}
+
+// Append one new photon at slot d of species `sp`, pushed.
+__device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int64_t d, uint64_t id,
+                                    double x, double y, double z, double ux, double uy,
+                                    double uz) {
+  if (sp.push) {
+    double ig = sfp_inv_gamma_photon(ux, uy, uz);
+    sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+    sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+  }
+  sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
+  sp.ux[d] = ux; sp.uy[d] = uy; sp.uz[d] = uz;
+  sp.id[d] = id;
+  sp.pruned[d] = 0;
+  if (sp.tau) sp.tau[d] = 0.0;
+}
+
+// Can a QED event happen this step?  chi_b >= 2 chi (bound with a factor-2
+// margin).  Rejection sampling: an event needs r2 < p with r2 >= 2^-53 (the
+// smallest uniform), p = 3 r1^2 w dtau (photon) or w dtau (pair), where the
+// table rate w = (alpha m c^2/hbar) |int Ai(z) + (2/z +- chi' sqrt z) Ai'(z)|
+// vanishes for z > z_max and otherwise is bounded by the envelopes at the
+// smallest reachable z:
+//   pair:   z = (1/(d(1-d) chi))^(2/3) >= (4/chi)^(2/3),      chi' = chi
+//   photon: z = (d/((1-d) chi))^(2/3) >= (d_min/chi)^(2/3),   chi' = d chi <= chi
+// with d_min = threshold / gamma (softer photons are dropped by the hard-photon
+// cut, cpu.py:227-232, and leave no trace).  If the bound on p, doubled, is
+// below 2^-53 no event is possible.  Optical depth: skipped below chi_min
+// (optical_depth.py:43), so only that test is exact.
+__device__ inline bool qed_possible(const Plan& P, bool photon, double chi_b, double ig,
+                                    double dt) {
+  if (P.qed == SFP_QED_OD) return !(chi_b < P.od.chi_min);
+  double num;
+  if (photon) {
+    num = 4.0;
+  } else {
+    num = P.threshold * ig;
+    if (!(num > 0.0)) return true;
+  }
+  if (!(chi_b >= 0.0)) return true;            // NaN bound: evaluate exactly
+  if (P.zmax15 * chi_b < num) return false;    // z > z_max for every reachable d: w = 0
+  // lower bound of the smallest reachable z, single precision, rounded down
+  float q = (float)(num / chi_b);
+  float zlo = exp2f(log2f(q) * (2.0f / 3.0f)) * 0.9999f;
+  double fj = ((double)zlo - P.z_min) / P.z_delta;
+  if (!(fj >= 1.0)) return true;               // first bracket: 2/z unbounded
+  int64_t j = (int64_t)fj;
+  if (j > P.z_N - 2) return false;
+  double2 e = P.rs_env[j];
+  double pbound = P.rate_unit * (e.x + chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
+  return !(2.0 * pbound < 0x1p-53);
+}
+
+// ------------------------------------------------------------ exit protocol --
+// The last workgroup to finish resets the tile counter for the next launch
+// and runs `last` (counts, epoch).
+// Everything the last workgroup reads was produced by atomics (counters,
+// look-back flags) and what it writes is consumed by later launches, so no
+// memory fence is needed.
+template <typename F>
+__device__ inline void kernel_exit(ScanState* scan, F&& last) {
+  if (threadIdx.x == 0) {
+    uint32_t prev = atomicAdd(&scan->done_ctr, 1u);
+    if (prev == gridDim.x - 1) {
+      last();
+      scan->tile_ctr = 0;
+      scan->done_ctr = 0;
+    }
+  }
+}
+
+__device__ inline void block_sum2(uint32_t a, uint32_t b, uint32_t* sh, uint64_t* ta, uint64_t* tb) {
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    sh[wv] = a;
+    sh[WAVES + wv] = b;
+  }
+  __syncthreads();
+  uint64_t sa = 0, sb = 0;
+  for (int k = 0; k < WAVES; ++k) {
+    sa += sh[k];
+    sb += sh[WAVES + k];
+  }
+  *ta = sa;
+  *tb = sb;
+}
+
+__device__ inline uint32_t next_epoch(const ScanState* scan) {
+  uint32_t e = (scan->epoch + 1u) & 0xffffffu;
+  return e == 0 ? 1u : e;
+}
+
+// ------------------------------------------------------------ sweep kernel --
+// A tile is ROWS rows of 256 consecutive particles (thread t handles particle
+// t of every row, so loads stay coalesced); the QED list keeps ascending
+// index order, i.e. (row, wave, lane) order, with one look-back per tile.
+static constexpr int ROWS = 8;
+
+template <bool PHOTON, bool QED, int FM>
+__global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, StepArgs a) {
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+  __shared__ int64_t sh_tile;
+  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint64_t sh_mask[ROWS][WAVES];
+  __shared__ uint64_t sh_pref;
+  constexpr int64_t TSPAN = (int64_t)TILE * ROWS;
+
+  const int64_t n_old = sp.cnt->n_old;
+  const int64_t ntiles = (n_old + TSPAN - 1) / TSPAN;
+  ScanState* scan = P.scan[s][0];
+  const uint32_t epoch = QED ? next_epoch(scan) : 0u;
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  uint32_t alive = 0, proc = 0;
+
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    const int64_t base = tile * TSPAN;
+    uint32_t hotbits = 0;
+
+#pragma unroll 1
+    for (int row = 0; row < ROWS; ++row) {
+      const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
+      bool active = false;
+      double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, sx = 0, sy = 0, sz = 0;
+      if (i < n_old) {
+        active = sp.pruned[i] == 0;
+        if (active) {
+          x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+          ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+        }
+      }
+      double ig = 1.0;
+      if (active) {
+        ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
+        proc += 1;
+      }
+      bool hot = false;
+      if (QED && active && sp.push) {
+        double S = sfp_field_bound<FM>(&P.fields, x, y, z, a.t);
+        double chi_b = P.chi_coef * S / ig;  // >= 2 chi (gamma = 1/ig >= |u|)
+        hot = qed_possible(P, PHOTON, chi_b, ig, a.dt);
+      }
+      if (QED) {
+        uint64_t m = __ballot(hot);
+        if (lane == 0) sh_mask[row][wv] = m;
+        if (hot) hotbits |= 1u << row;
+      }
+      if (active && !hot) {
+        if (PHOTON) {
+          if (sp.push) {
+            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+            sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          }
+        } else if (sp.push) {
+          if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+          lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
+          if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
+        }
+        alive += 1;
+      }
+    }
+
+    if (QED) {
+      __syncthreads();
+      if (wv == 0) {
+        uint64_t c = (lane < ROWS * WAVES) ? (uint64_t)__popcll((&sh_mask[0][0])[lane]) : 0;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        uint64_t pre = lookback(P.flags[s][0], tile, c, epoch);
+        if (lane == 0) sh_pref = pre;
+      }
+      __syncthreads();
+      uint64_t off = sh_pref;
+      const uint64_t lt = (1ull << lane) - 1ull;
+      for (int row = 0; row < ROWS; ++row) {
+        uint32_t before = 0, total = 0;
+        for (int k = 0; k < WAVES; ++k) {
+          uint32_t c = (uint32_t)__popcll(sh_mask[row][k]);
+          if (k < wv) before += c;
+          total += c;
+        }
+        if (hotbits & (1u << row)) {
+          uint64_t r = off + before + (uint32_t)__popcll(sh_mask[row][wv] & lt);
+          sp.hot[r] = (uint32_t)(base + (int64_t)row * TILE + threadIdx.x);
+        }
+        off += total;
+      }
+      __syncthreads();  // sh_mask is rewritten by the next tile
+    }
+  }
+
+  uint64_t talive, tproc;
+  block_sum2(alive, proc, sh_w, &talive, &tproc);
+  if (threadIdx.x == 0) {
+    if (talive) atomicAdd((unsigned long long*)&sp.cnt->psteps, (unsigned long long)talive);
+    if (tproc) atomicAdd((unsigned long long*)&sp.cnt->processed, (unsigned long long)tproc);
+  }
+  kernel_exit(scan, [&]() {
+    if (QED) {
+      uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+      sp.cnt->n_hot = (int64_t)total;
+      sp.cnt->hot_total += (int64_t)total;
+      scan->epoch = epoch;
+    } else {
+      sp.cnt->n_hot = 0;
+    }
+  });
+}
+
+// -------------------------------------------------------------- hot kernel --
+template <bool PHOTON, int KIND, int FM>
+__global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, StepArgs a) {
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+
+  __shared__ sfp_rs_node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
+  __shared__ int64_t sh_tile;
+  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint64_t sh_pref;
+
+  const int64_t n_hot = sp.cnt->n_hot;
+  const SpeciesDev* dA = nullptr;  // photon dst, or electron dst
+  const SpeciesDev* dB = nullptr;  // positron dst
+  if (!PHOTON) {
+    if (sp.photon_dst >= 0) dA = &P.sp[sp.photon_dst];
+  } else {
+    if (sp.pair_e_dst >= 0) dA = &P.sp[sp.pair_e_dst];
+    if (sp.pair_p_dst >= 0) dB = &P.sp[sp.pair_p_dst];
+  }
+  const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
+  const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
+  // room for one append per listed particle in every destination, or halt
+  // (every workgroup reads the same counts and takes the same decision)
+  if ((dA && baseA + n_hot > dA->cap) || (dB && baseB + n_hot > dB->cap)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      P.halt->species = s;
+      P.halt->step = (int64_t)a.step;
+      P.halt->flag = 1;
+    }
+    return;
+  }
+  const int64_t ntiles = (n_hot + TILE - 1) / TILE;
+  ScanState* scan = P.scan[s][1];
+  const uint32_t epoch = next_epoch(scan);
+
+  if (KIND == KIND_RS) {
+    int64_t nn = P.z_N - 1;
+    for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) lds_rs[k] = P.rs_nodes[k];
+  } else {
+    const double2* src = PHOTON ? P.od_pair_nodes : P.od_photon_nodes;
+    int64_t nn = P.od.chi_N - 1;
+    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = src[k];
+  }
+  __syncthreads();
+
+  uint32_t alive = 0, hot_seen = 0;
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+
+    const int64_t k = tile * TILE + threadIdx.x;
+    const bool valid = k < n_hot;
+    int64_t i = 0;
+    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
+    uint64_t pid = 0;
+    if (valid) {
+      i = (int64_t)sp.hot[k];
+      x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+      ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+      if (KIND == KIND_OD) tau = sp.tau[i];
+      if (!PHOTON && sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+      pid = sp.id[i];
+      hot_seen += 1;
+    }
+    double ig = 1.0;
+    if (valid) ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
+
+    // ---- QED front at time t (simulation.py:103-114) ------------------
+    bool event = false;
+    double delta = 0.0;
+    if (valid) {
+      double F[6];
+      sfp_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+      double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
+      if (KIND == KIND_RS) {
+        // qed/rejection_sampling.py:68-101
+        if (chi != 0.0) {
+          sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+          double r1 = sfp_u01(d.v[0], d.v[1]);
+          double r2 = sfp_u01(d.v[2], d.v[3]);
+          double dtau = a.dt * ig;
+          if (!PHOTON) {
+            double r1_3 = sfp_powi(r1, 3);
+            double p = 3 * sfp_powi(r1, 2) *
+                       sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1) * dtau;
+            if (r2 < p) { event = true; delta = r1_3; }
+          } else {
+            double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
+            if (r2 < p) { event = true; delta = r1; }
+          }
+        }
+      } else {
+        // qed/optical_depth.py:15-110 (below chi_min or NaN: skipped)
+        if (chi >= P.od.chi_min) {
+          event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
+                            PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
+        }
+      }
+      // cpu.py:227-232 (radiating leptons only)
+      if (!PHOTON && event) {
+        if (delta / ig < P.threshold) event = false;
+      }
+    }
+    if (!dA && !dB) event = false;
+
+    // ---- stable rank and append (cpu.py:90-191) ------------------------
+    int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
+    if (event) {
+      if (!PHOTON) {
+        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+                     delta * uy, delta * uz);
+      } else {
+        double dp = 1.0 - delta;
+        if (dA)
+          spawn_lepton<FM>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+                           delta * uy, delta * uz);
+        if (dB)
+          spawn_lepton<FM>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
+                           dp * uy, dp * uz);
+      }
+    }
+
+    // ---- recoil (cpu.py:78-88) and push (simulation.py:128-137) -------
+    if (valid) {
+      if (PHOTON) {
+        if (event) {
+          sp.pruned[i] = 1;  // the converted photon becomes a hole (cpu.py:189-190)
+        } else {
+          sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+          sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          if (KIND == KIND_OD) sp.tau[i] = tau;
+          alive += 1;
+        }
+      } else {
+        if (event && sp.rr == SFP_RR_PHOTON) {
+          ux *= 1 - delta;
+          uy *= 1 - delta;
+          uz *= 1 - delta;
+          ig = sfp_inv_gamma_lepton(ux, uy, uz);
+        }
+        lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+        sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+        sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
+        if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
+        if (KIND == KIND_OD) sp.tau[i] = tau;
+        alive += 1;
+      }
+    }
+  }
+
+  uint64_t talive, tseen;
+  block_sum2(alive, hot_seen, sh_w, &talive, &tseen);
+  if (threadIdx.x == 0 && talive)
+    atomicAdd((unsigned long long*)&sp.cnt->psteps, (unsigned long long)talive);
+  kernel_exit(scan, [&]() {
+    uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+    if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
+    if (dA) {
+      dA->cnt->n = baseA + (int64_t)total;
+      dA->cnt->next_id = idA + (int64_t)total;
+      atomicAdd((unsigned long long*)&dA->cnt->psteps, (unsigned long long)total);
+    }
+    if (dB) {
+      dB->cnt->n = baseB + (int64_t)total;
+      dB->cnt->next_id = idB + (int64_t)total;
+      atomicAdd((unsigned long long*)&dB->cnt->psteps, (unsigned long long)total);
+    }
+    scan->epoch = epoch;
+  });
+}
+
+// prologue of a step: snapshot the counts the species kernels iterate over
+__global__ void k_step_prologue(const Plan* plan) {
+  if (plan->halt->flag) return;
+  int s = threadIdx.x;
+  if (s < plan->nspecies) {
+    Counts* c = plan->sp[s].cnt;
+    c->n_old = c->n;
+  }
+}
+
+// ------------------------------------------------------------- dispatch ----
+typedef void (*kfn)(const Plan*, int, StepArgs);
+
+template <int FM>
+static kfn sweep_fn(bool photon, bool qed) {
+  if (photon) return qed ? k_sweep<true, true, FM> : k_sweep<true, false, FM>;
+  return qed ? k_sweep<false, true, FM> : k_sweep<false, false, FM>;
+}
+
+template <int FM>
+static kfn hot_fn(bool photon, int kind) {
+  if (photon) return kind == KIND_OD ? k_hot<true, KIND_OD, FM> : k_hot<true, KIND_RS, FM>;
+  return kind == KIND_OD ? k_hot<false, KIND_OD, FM> : k_hot<false, KIND_RS, FM>;
+}
+
+static kfn pick_sweep(int fm, bool photon, bool qed) {
+  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN>(photon, qed);
+  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE>(photon, qed);
+  return sweep_fn<SFP_FM_ANY>(photon, qed);
+}
+
+static kfn pick_hot(int fm, bool photon, int kind) {
+  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN>(photon, kind);
+  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE>(photon, kind);
+  return hot_fn<SFP_FM_ANY>(photon, kind);
+}
+
+// ------------------------------------------------------------- host side ---
+struct Sim {
+  int nsp = 0;
+  std::vector<sfp_species_desc> desc;
+  std::vector<Store*> stores;
+  std::vector<int> qed;            // species has a QED process
+  kfn sweep[SFP_MAX_SPECIES] = {};
+  kfn hot[SFP_MAX_SPECIES] = {};
+  int grid_sweep[SFP_MAX_SPECIES] = {};
+  int grid_hot[SFP_MAX_SPECIES] = {};
+  std::vector<int64_t> flags_cap[2];
+  std::vector<int64_t> hot_cap;
+  std::vector<uint64_t*> flags[2];
+  std::vector<uint32_t*> hot_mem;
+  ScanState* scan_mem = nullptr;
+  Halt* halt = nullptr;            // device
+  Halt* halt_host = nullptr;       // pinned
+  Counts* counts_host = nullptr;   // pinned [nsp]
+  Plan plan{};
+  Plan* plan_dev = nullptr;
+  Tables* tables = nullptr;
+  sfp_sim_params params{};
+  int fm = SFP_FM_ANY;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending[SFP_MAX_SPECIES][2];
+  double ms_acc[SFP_MAX_SPECIES][2] = {};
+  int64_t launches[SFP_MAX_SPECIES][2] = {};
+  std::vector<hipEvent_t> ev_pool;
+  int64_t syncs = 0, replays = 0;
+};
+
+static int sim_read_counts(Sim* sim) {
+  HIPCHK(hipDeviceSynchronize());
+  for (int s = 0; s < sim->nsp; ++s)
+    HIPCHK(hipMemcpy(&sim->counts_host[s], sim->stores[s]->cnt, sizeof(Counts),
+                     hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sim->halt_host, sim->halt, sizeof(Halt), hipMemcpyDeviceToHost));
+  sim->syncs++;
+  return SFP_OK;
+}
+
+static int sim_refresh_plan(Sim* sim) {
+  Plan& P = sim->plan;
+  for (int s = 0; s < sim->nsp; ++s) {
+    Store* st = sim->stores[s];
+    SpeciesDev& d = P.sp[s];
+    d.x = (double*)st->p[SFP_X];
+    d.y = (double*)st->p[SFP_Y];
+    d.z = (double*)st->p[SFP_Z];
+    d.ux = (double*)st->p[SFP_UX];
+    d.uy = (double*)st->p[SFP_UY];
+    d.uz = (double*)st->p[SFP_UZ];
+    d.id = (uint64_t*)st->p[SFP_ID];
+    d.pruned = (uint8_t*)st->p[SFP_PRUNED];
+    d.tau = (double*)st->p[SFP_TAU];
+    d.sx = (double*)st->p[SFP_SX];
+    d.sy = (double*)st->p[SFP_SY];
+    d.sz = (double*)st->p[SFP_SZ];
+    d.cnt = st->cnt;
+    d.cap = st->cap;
+    if (st->cap > (int64_t)UINT32_MAX)
+      return set_err(SFP_E_ARG, "species %d: capacity above 2^32 particles", s);
+    for (int ph = 0; ph < 2; ++ph) {
+      int64_t need = st->cap / TILE + 2;
+      if (need > sim->flags_cap[ph][s]) {
+        if (sim->flags[ph][s]) HIPCHK(hipFree(sim->flags[ph][s]));
+        int64_t want = need + need / 2;
+        HIPCHK(hipMalloc(&sim->flags[ph][s], sizeof(uint64_t) * want));
+        HIPCHK(hipMemset(sim->flags[ph][s], 0, sizeof(uint64_t) * want));
+        sim->flags_cap[ph][s] = want;
+      }
+      P.flags[s][ph] = sim->flags[ph][s];
+      P.scan[s][ph] = sim->scan_mem + 2 * s + ph;
+    }
+    if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
+      if (sim->hot_mem[s]) HIPCHK(hipFree(sim->hot_mem[s]));
+      int64_t want = std::max<int64_t>(st->cap, 1);
+      HIPCHK(hipMalloc(&sim->hot_mem[s], sizeof(uint32_t) * want));
+      sim->hot_cap[s] = want;
+    }
+    d.hot = sim->hot_mem[s];
+  }
+  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
+                              const sfp_field* terms, int32_t nterms, void* tables,
+                              const sfp_sim_params* params, void** out) {
+  if (nspecies < 1 || nspecies > SFP_MAX_SPECIES)
+    return set_err(SFP_E_ARG, "between 1 and %d species are supported", SFP_MAX_SPECIES);
+  if (params->qed_model != SFP_QED_RS && params->qed_model != SFP_QED_OD)
+    return set_err(SFP_E_ARG, "unknown QED model %d", params->qed_model);
+  Sim* sim = new Sim();
+  sim->nsp = nspecies;
+  sim->params = *params;
+  sim->tables = (Tables*)tables;
+  sim->desc.assign(species, species + nspecies);
+  Plan& P = sim->plan;
+  memset(&P, 0, sizeof(P));
+  CHK(sfp_compile_fields(terms, nterms, &P.fields));
+  sim->fm = sfp_field_mode(&P.fields);
+  P.nspecies = nspecies;
+  P.seed = params->seed;
+  P.threshold = params->photon_threshold;
+  P.qed = params->qed_model;
+  P.chi_coef = 2.0 * SFP_E * SFP_HBAR / ((SFP_ME * SFP_ME) * (SFP_C * SFP_C * SFP_C));
+  bool any_qed = false;
+  for (int s = 0; s < nspecies; ++s) {
+    const sfp_species_desc& d = species[s];
+    if (!d.store) return set_err(SFP_E_ARG, "species %d has no store", s);
+    Store* st = (Store*)d.store;
+    sim->stores.push_back(st);
+    bool photon = d.m == 0.0;
+    for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst})
+      if (dst >= nspecies || dst == s)
+        return set_err(SFP_E_ARG, "species %d: destination %d invalid", s, dst);
+    if (!photon && (d.pair_e_dst >= 0 || d.pair_p_dst >= 0))
+      return set_err(SFP_E_ARG, "species %d: only photons create pairs", s);
+    if (photon && d.photon_dst >= 0)
+      return set_err(SFP_E_ARG, "species %d: photons do not radiate photons", s);
+    bool qed = photon ? (d.pair_e_dst >= 0 || d.pair_p_dst >= 0) : (d.photon_dst >= 0);
+    if (qed && params->qed_model == SFP_QED_OD && !st->p[SFP_TAU])
+      return set_err(SFP_E_ARG, "species %d uses the optical-depth model but has no tau", s);
+    if (!photon && d.spin && !st->p[SFP_SX])
+      return set_err(SFP_E_ARG, "species %d is a spin species without spin storage", s);
+    sim->qed.push_back(qed ? 1 : 0);
+    any_qed |= qed;
+    SpeciesDev& sd = P.sp[s];
+    sd.q = d.q;
+    sd.ae = d.ae;
+    sd.rr = d.rr;
+    sd.push = d.push;
+    sd.spin = photon ? 0 : d.spin;
+    sd.photon = photon;
+    sd.tag = s;
+    sd.qed = qed;
+    sd.photon_dst = d.photon_dst;
+    sd.pair_e_dst = d.pair_e_dst;
+    sd.pair_p_dst = d.pair_p_dst;
+  }
+  if (any_qed && params->qed_model == SFP_QED_RS) {
+    if (!sim->tables || !sim->tables->has_rs)
+      return set_err(SFP_E_ARG, "rejection-sampling tables required");
+    if (sim->tables->z_N - 1 > 1023)
+      return set_err(SFP_E_ARG, "rejection-sampling table larger than 1024 nodes");
+    P.rs_nodes = sim->tables->rs_nodes;
+    P.rs_env = sim->tables->rs_env;
+    P.rate_unit = SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR;
+    P.z_N = sim->tables->z_N;
+    P.z_min = sim->tables->z_min;
+    P.z_max = sim->tables->z_max;
+    P.z_delta = sim->tables->z_delta;
+    P.zmax15 = sim->tables->z_max * sqrt(sim->tables->z_max);
+  }
+  if (any_qed && params->qed_model == SFP_QED_OD) {
+    if (!sim->tables || !sim->tables->has_od)
+      return set_err(SFP_E_ARG, "optical-depth tables required");
+    if (sim->tables->od.chi_N - 1 > 255)
+      return set_err(SFP_E_ARG, "optical-depth table larger than 256 chi nodes");
+    P.od = sim->tables->od;
+    P.od_photon_nodes = sim->tables->od_photon_nodes;
+    P.od_pair_nodes = sim->tables->od_pair_nodes;
+  }
+  HIPCHK(hipMalloc(&sim->plan_dev, sizeof(Plan)));
+  HIPCHK(hipMalloc(&sim->scan_mem, sizeof(ScanState) * 2 * nspecies));
+  HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * 2 * nspecies));
+  HIPCHK(hipMalloc(&sim->halt, sizeof(Halt)));
+  HIPCHK(hipMemset(sim->halt, 0, sizeof(Halt)));
+  HIPCHK(hipHostMalloc((void**)&sim->halt_host, sizeof(Halt), 0));
+  HIPCHK(hipHostMalloc((void**)&sim->counts_host, sizeof(Counts) * nspecies, 0));
+  P.halt = sim->halt;
+  for (int ph = 0; ph < 2; ++ph) {
+    sim->flags[ph].assign(nspecies, nullptr);
+    sim->flags_cap[ph].assign(nspecies, 0);
+  }
+  sim->hot_mem.assign(nspecies, nullptr);
+  sim->hot_cap.assign(nspecies, 0);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  int cus = prop.multiProcessorCount;
+  for (int s = 0; s < nspecies; ++s) {
+    bool photon = P.sp[s].photon;
+    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s]);
+    sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, params->qed_model == SFP_QED_OD
+                                                                ? KIND_OD : KIND_RS)
+                              : nullptr;
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->sweep[s], TILE, 0));
+    sim->grid_sweep[s] = std::max(per_cu, 1) * cus;
+    if (sim->hot[s]) {
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->hot[s], TILE, 0));
+      sim->grid_hot[s] = std::max(per_cu, 1) * cus;
+    }
+  }
+  CHK(sim_refresh_plan(sim));
+  *out = sim;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_destroy(void* p) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return SFP_OK;
+  (void)hipDeviceSynchronize();
+  for (int ph = 0; ph < 2; ++ph)
+    for (auto f : sim->flags[ph])
+      if (f) (void)hipFree(f);
+  for (auto h : sim->hot_mem)
+    if (h) (void)hipFree(h);
+  if (sim->scan_mem) (void)hipFree(sim->scan_mem);
+  if (sim->plan_dev) (void)hipFree(sim->plan_dev);
+  if (sim->halt) (void)hipFree(sim->halt);
+  if (sim->halt_host) (void)hipHostFree(sim->halt_host);
+  if (sim->counts_host) (void)hipHostFree(sim->counts_host);
+  for (int s = 0; s < SFP_MAX_SPECIES; ++s)
+    for (int ph = 0; ph < 2; ++ph)
+      for (auto& e : sim->ev_pending[s][ph]) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+  for (auto e : sim->ev_pool) (void)hipEventDestroy(e);
+  delete sim;
+  return SFP_OK;
+}
+
+static hipEvent_t sim_event(Sim* sim) {
+  if (!sim->ev_pool.empty()) {
+    hipEvent_t e = sim->ev_pool.back();
+    sim->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static int launch(Sim* sim, kfn fn, int grid, int s, int phase, const StepArgs& a) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (sim->timing) {
+    e0 = sim_event(sim);
+    e1 = sim_event(sim);
+    if (e0) HIPCHK(hipEventRecord(e0, 0));
+  }
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(TILE), 0, 0, (const Plan*)sim->plan_dev, s, a);
+  HIPCHK(hipGetLastError());
+  if (sim->timing && e0 && e1) {
+    HIPCHK(hipEventRecord(e1, 0));
+    sim->ev_pending[s][phase].push_back({e0, e1});
+  }
+  return SFP_OK;
+}
+
+// Launch the kernels of one step from (species s0, phase ph0) on.
+static int launch_step(Sim* sim, const StepArgs& a, int s0, int ph0) {
+  if (s0 == 0 && ph0 == 0)
+    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, 0, (const Plan*)sim->plan_dev);
+  for (int s = s0; s < sim->nsp; ++s) {
+    int first = (s == s0) ? ph0 : 0;
+    if (first == 0) CHK(launch(sim, sim->sweep[s], sim->grid_sweep[s], s, 0, a));
+    if (sim->hot[s]) CHK(launch(sim, sim->hot[s], sim->grid_hot[s], s, 1, a));
+  }
+  return SFP_OK;
+}
+
+// After a halt: grow the destinations of the halted species so that every
+// listed particle can append, clear the flag; the caller replays.
+static int sim_recover(Sim* sim, int s) {
+  const sfp_species_desc& d = sim->desc[s];
+  int64_t n_hot = sim->counts_host[s].n_hot;
+  for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst}) {
+    if (dst < 0) continue;
+    Store* st = sim->stores[dst];
+    int64_t need = sim->counts_host[dst].n + n_hot;
+    if (need > st->cap) {
+      int64_t want = need + need / 2 + 4096;
+      CHK(sfp_store_reserve(st, want));
+    }
+  }
+  Halt clear{};
+  HIPCHK(hipMemcpy(sim->halt, &clear, sizeof(Halt), hipMemcpyHostToDevice));
+  CHK(sim_refresh_plan(sim));
+  sim->replays++;
+  return SFP_OK;
+}
+
+static constexpr int64_t SYNC_EVERY = 64;
+
+extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  sim->timing = (flags & SFP_RUN_TIMING) != 0;
+  // stores may have been resized or refilled by the caller since the last run
+  bool dirty = false;
+  for (int s = 0; s < sim->nsp; ++s)
+    if (sim->plan.sp[s].cap != sim->stores[s]->cap || sim->plan.sp[s].x != sim->stores[s]->p[SFP_X])
+      dirty = true;
+  if (dirty) CHK(sim_refresh_plan(sim));
+  const double dt = sim->params.dt;
+  std::vector<StepArgs> args;
+  args.reserve((size_t)std::min<int64_t>(nsteps, SYNC_EVERY));
+  int64_t k = 0;
+  double tt = *t;
+  while (k < nsteps) {
+    int64_t batch = std::min<int64_t>(SYNC_EVERY, nsteps - k);
+    args.clear();
+    for (int64_t j = 0; j < batch; ++j) {
+      StepArgs a;
+      a.t = tt;
+      a.t_half = tt + 0.5 * dt;
+      a.dt = dt;
+      a.cdt_half = SFP_C * (0.5 * dt);
+      a.ll_factor = sfp_ll_factor(dt);
+      a.step = step0 + (uint64_t)(k + j);
+      args.push_back(a);
+      tt = tt + dt;
+      CHK(launch_step(sim, a, 0, 0));
+    }
+    CHK(sim_read_counts(sim));
+    // replay from a halted kernel until the batch completes
+    while (sim->halt_host->flag) {
+      int s = sim->halt_host->species;
+      int64_t h = sim->halt_host->step - (int64_t)(step0 + k);
+      if (h < 0 || h >= batch) return set_err(SFP_E_STATE, "halt outside the launched steps");
+      CHK(sim_recover(sim, s));
+      CHK(launch_step(sim, args[h], s, 1));
+      for (int64_t j = h + 1; j < batch; ++j) CHK(launch_step(sim, args[j], 0, 0));
+      CHK(sim_read_counts(sim));
+    }
+    k += batch;
+  }
+  *t = tt;
+  return SFP_OK;
+}
+
+static int sim_collect_timing(Sim* sim) {
+  HIPCHK(hipDeviceSynchronize());
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < 2; ++ph) {
+      for (auto& e : sim->ev_pending[s][ph]) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        sim->ms_acc[s][ph] += ms;
+        sim->launches[s][ph] += 1;
+        sim->ev_pool.push_back(e.first);
+        sim->ev_pool.push_back(e.second);
+      }
+      sim->ev_pending[s][ph].clear();
+    }
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_kernel_time(void* p, int32_t species, int32_t phase, double* ms,
+                                   int64_t* launches) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1)
+    return set_err(SFP_E_ARG, "bad species/phase");
+  CHK(sim_collect_timing(sim));
+  *ms = sim->ms_acc[species][phase];
+  *launches = sim->launches[species][phase];
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_reset_timing(void* p) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  CHK(sim_collect_timing(sim));
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < 2; ++ph) {
+      sim->ms_acc[s][ph] = 0.0;
+      sim->launches[s][ph] = 0;
+    }
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
+  CHK(sim_read_counts(sim));
+  const Counts& c = sim->counts_host[species];
+  stats[0] = c.psteps;
+  stats[1] = c.processed;
+  stats[2] = c.spawned;
+  stats[3] = c.n;
+  stats[4] = c.hot_total;
+  stats[5] = sim->syncs;
+  stats[6] = sim->replays;
+  stats[7] = sim->stores[species]->cap;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_reset_stats(void* p) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  CHK(sim_read_counts(sim));
+  for (int s = 0; s < sim->nsp; ++s) {
+    Counts c = sim->counts_host[s];
+    c.psteps = c.processed = c.spawned = c.hot_total = 0;
+    HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
+  }
+  sim->syncs = 0;
+  sim->replays = 0;
+  return SFP_OK;
+}

@@ -118,10 +118,12 @@ class Simulation(object):
             p._sync_count_from_device()
             object.__setattr__(p, "_ig_stale", True)
 
-    def kernel_time(self, species: int):
+    def kernel_time(self, species: int, phase: int):
+        """(ms, launches) of a species' sweep (phase 0) or QED (phase 1) kernel."""
         ms = C.c_double(0)
         n = C.c_int64(0)
-        L.call("sfp_sim_kernel_time", self._sim, int(species), C.byref(ms), C.byref(n))
+        L.call("sfp_sim_kernel_time", self._sim, int(species), int(phase), C.byref(ms),
+               C.byref(n))
         return ms.value, int(n.value)
 
     def reset_timing(self):
@@ -131,9 +133,10 @@ class Simulation(object):
         """Counters of one species since the last reset: particle-steps (live
         particles after each step), particles read by its step kernel,
         particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 4)()
+        v = (C.c_int64 * 8)()
         L.call("sfp_sim_stats", self._sim, int(species), v)
-        return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]))
+        return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
+                    hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)

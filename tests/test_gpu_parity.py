@@ -59,11 +59,17 @@ def test_device_math_matches_host():
 
 def test_head_on_rs_bit_exact():
     case = head_on_case(n=3000, nsteps=240, seed=11)
-    gpu, _ = run_gpu_case(case)
+    gpu, sim = run_gpu_case(case)
     ref = run_oracle_case(case)
-    print("counts", _names(gpu))
+    print("counts", _names(gpu), "stats", [sim.stats(i) for i in range(3)])
     assert ref["photon"]["n"] > 1000  # the collision happened
     compare_final(gpu, ref)
+    st = [sim.stats(i) for i in range(3)]
+    # stores started with capacity 1: appends halted and replayed on the device
+    assert st[0]["replays"] > 0
+    # the envelope bound culled part of the photons' QED work (photons
+    # leaving the pulse) and everything still matches
+    assert st[2]["hot"] < st[2]["processed"]
 
 
 def test_head_on_rs_chunked_equals_single():
