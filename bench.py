@@ -153,11 +153,15 @@ def calibrate_copy(n_doubles, reps):
     from sfparticles_amd import _lib as L
     src = L.DeviceArray(8 * n_doubles)
     dst = L.DeviceArray(8 * n_doubles)
+    L.call("sfp_copy_f64", src.ptr, dst.ptr, n_doubles, None)
+    L.call("sfp_synchronize")
+    t0 = time.perf_counter()
     for _ in range(reps):
         L.call("sfp_copy_f64", src.ptr, dst.ptr, n_doubles, None)
     L.call("sfp_synchronize")
+    dt = (time.perf_counter() - t0) / reps
     print(json.dumps({"copy_bytes_read": 8 * n_doubles, "copy_bytes_written": 8 * n_doubles,
-                      "reps": reps}), flush=True)
+                      "reps": reps, "copy_GBps": round(16 * n_doubles / dt / 1e9, 1)}), flush=True)
 
 
 def main():

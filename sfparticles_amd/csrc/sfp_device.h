@@ -42,12 +42,18 @@ typedef struct {
 
 /* term constant slots */
 enum {
-  /* simple (fields.py:69-118) */
+  /* simple (fields.py:69-118); R* are reciprocals used by the envelope bound */
   FS_E0 = 0, FS_K0, FS_W0SQ, FS_KCTN, FS_DIR, FS_DX0, FS_CEP, FS_COSP, FS_SINP,
+  FS_RW0SQ = 20, FS_RKCTN,
 };
 enum {
   /* gaussian (fields.py:120-170) */
   FG_E0 = 0, FG_K0, FG_ZR, FG_W0, FG_KCT2, FG_X0, FG_DC, FG_CEP, FG_COSP, FG_SINP, FG_DIR,
+  FG_RKCT2 = 20, FG_R4ZR, FG_RW0SQ, FG_RZR,
+};
+enum {
+  /* static: |E| + c|B| for the bound */
+  FST_S = 20,
 };
 enum {
   /* focused (fields.py:173-265) */
@@ -64,6 +70,8 @@ static inline void sfp_compile_term(const sfp_field* u, sfp_term_dev* d) {
   switch (u->kind) {
     case SFP_FIELD_STATIC:
       for (int i = 0; i < 6; ++i) d->c[i] = p[i];
+      d->c[FST_S] = sfp_sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) +
+                    SFP_C * sfp_sqrt(p[3] * p[3] + p[4] * p[4] + p[5] * p[5]);
       break;
     case SFP_FIELD_SIMPLE: {
       double a0 = p[0], w0 = p[1], ctau = p[2];
@@ -82,6 +90,8 @@ static inline void sfp_compile_term(const sfp_field* u, sfp_term_dev* d) {
       d->c[FS_CEP] = cep;
       d->c[FS_COSP] = sfp_cos(pol_angle);
       d->c[FS_SINP] = sfp_sin(pol_angle);
+      d->c[FS_RW0SQ] = 1.0 / d->c[FS_W0SQ];
+      d->c[FS_RKCTN] = 1.0 / d->c[FS_KCTN];
       break;
     }
     case SFP_FIELD_GAUSSIAN: {
@@ -103,6 +113,10 @@ static inline void sfp_compile_term(const sfp_field* u, sfp_term_dev* d) {
       d->c[FG_COSP] = sfp_cos(pol_angle);
       d->c[FG_SINP] = sfp_sin(pol_angle);
       d->c[FG_DIR] = direction;
+      d->c[FG_RKCT2] = 1.0 / d->c[FG_KCT2];
+      d->c[FG_R4ZR] = 1.0 / (4.0 * zR);
+      d->c[FG_RW0SQ] = 1.0 / (w0 * w0);
+      d->c[FG_RZR] = 1.0 / zR;
       break;
     }
     default: { /* focused */
@@ -300,6 +314,50 @@ SFP_HD void sfp_field_eval(const sfp_fieldset* fs, double x, double y, double z,
   }
   if (fs->nterms == 0)
     for (int i = 0; i < 6; ++i) F[i] = 0.0;
+}
+
+/* Single-precision version of sfp_field_bound for the culling test: the
+ * phase is formed in FP64 and rounded once, the rest is FP32 with hardware
+ * exp2; the phase margin (1e-3 rad + 1e-6 |phi| beyond the analytic pi/2)
+ * and a factor 1.01 on the result cover the rounding, so the value still
+ * bounds |E| + c|B| from above. */
+template <int FM = SFP_FM_ANY>
+SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, double z, double t) {
+  float S = 0.0f;
+  double ct = SFP_C * t;
+  float r2 = (float)(y * y + z * z);
+  const float LOG2E = 1.4426950408889634f;
+  for (int j = 0; j < fs->nterms; ++j) {
+    const sfp_term_dev* d = &fs->t[j];
+    const double* c = d->c;
+    int kind = (FM == SFP_FM_GAUSSIAN) ? SFP_FIELD_GAUSSIAN : d->kind;
+    if (FM == SFP_FM_PLANE && kind != SFP_FIELD_SIMPLE) kind = SFP_FIELD_STATIC;
+    if (kind == SFP_FIELD_SIMPLE) {
+      if (d->n & 1) return __builtin_inff();  // odd envelope orders grow for phi < 0: no bound
+      float phi = (float)(c[FS_K0] * (c[FS_DIR] * x - c[FS_DX0] - ct));
+      float ph = phi < 0.0f ? -phi : phi;
+      ph = ph - 1e-3f - ph * 1e-6f;
+      if (ph < 0.0f) ph = 0.0f;
+      float pn = 1.0f;
+      for (int k = 0; k < d->n; ++k) pn *= ph;
+      float a = r2 * (float)c[FS_RW0SQ] + pn * (float)c[FS_RKCTN];
+      S += 2.02f * (float)c[FS_E0] * exp2f(-a * LOG2E);
+    } else if (kind == SFP_FIELD_GAUSSIAN) {
+      float k0 = (float)c[FG_K0];
+      float phl = (float)(k0 * (x - c[FG_X0] - c[FG_DC] * t));
+      phl = (phl < 0.0f ? -phl : phl);
+      phl = phl - 1.5717963f - k0 * r2 * (float)c[FG_R4ZR] - phl * 1e-6f;
+      if (phl < 0.0f) phl = 0.0f;
+      float xz = (float)x * (float)c[FG_RZR];
+      float a = phl * phl * (float)c[FG_RKCT2] + r2 * (float)c[FG_RW0SQ] / (1.0f + xz * xz);
+      S += 2.02f * (float)c[FG_E0] * exp2f(-a * LOG2E);
+    } else if (kind == SFP_FIELD_STATIC) {
+      S += 1.01f * (float)c[FST_S];
+    } else {
+      return __builtin_inff();
+    }
+  }
+  return S;
 }
 
 /* Upper bound of |E| + c|B| at (x, y, z, t), from the envelopes only (no

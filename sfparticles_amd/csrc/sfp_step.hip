@@ -77,7 +77,7 @@ struct Plan {
   const double2* od_pair_nodes;
   sfp_od_dev od;
   double threshold;
-  double chi_coef;  // 2 e hbar / (m^2 c^3): the bound's constant with a factor 2 margin
+  double chi_coef;  // e hbar / (m^2 c^3)
   double zmax15;    // z_max^1.5: rate is 0 unless delta/(1-delta) <= zmax15 chi (photon)
   uint64_t seed;
   Halt* halt;
@@ -228,8 +228,7 @@ __device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int
   if (sp.tau) sp.tau[d] = 0.0;
 }
 
-// Can a QED event happen this step?  chi_b >= 2 chi (bound with a factor-2
-// margin).  Rejection sampling: an event needs r2 < p with r2 >= 2^-53 (the
+// Can a QED event happen this step?  chi_b >= chi (bound with a small margin).  Rejection sampling: an event needs r2 < p with r2 >= 2^-53 (the
 // smallest uniform), p = 3 r1^2 w dtau (photon) or w dtau (pair), where the
 // table rate w = (alpha m c^2/hbar) |int Ai(z) + (2/z +- chi' sqrt z) Ai'(z)|
 // vanishes for z > z_max and otherwise is bounded by the envelopes at the
@@ -240,27 +239,27 @@ __device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int
 // cut, cpu.py:227-232, and leave no trace).  If the bound on p, doubled, is
 // below 2^-53 no event is possible.  Optical depth: skipped below chi_min
 // (optical_depth.py:43), so only that test is exact.
-__device__ inline bool qed_possible(const Plan& P, bool photon, double chi_b, double ig,
+__device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
                                     double dt) {
-  if (P.qed == SFP_QED_OD) return !(chi_b < P.od.chi_min);
-  double num;
+  if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
+  float num;
   if (photon) {
-    num = 4.0;
+    num = 4.0f;
   } else {
-    num = P.threshold * ig;
-    if (!(num > 0.0)) return true;
+    num = (float)(P.threshold * ig) * 0.999f;
+    if (!(num > 0.0f)) return true;
   }
-  if (!(chi_b >= 0.0)) return true;            // NaN bound: evaluate exactly
-  if (P.zmax15 * chi_b < num) return false;    // z > z_max for every reachable d: w = 0
-  // lower bound of the smallest reachable z, single precision, rounded down
-  float q = (float)(num / chi_b);
-  float zlo = exp2f(log2f(q) * (2.0f / 3.0f)) * 0.9999f;
-  double fj = ((double)zlo - P.z_min) / P.z_delta;
-  if (!(fj >= 1.0)) return true;               // first bracket: 2/z unbounded
+  if (!(chi_b >= 0.0f)) return true;                    // NaN bound: evaluate exactly
+  if ((float)P.zmax15 * chi_b < num) return false;      // z > z_max for every reachable d: w = 0
+  // lower bound of the smallest reachable z, rounded down
+  float q = num / chi_b;
+  float zlo = exp2f(log2f(q) * (2.0f / 3.0f)) * 0.999f;
+  float fj = (zlo - (float)P.z_min) * (float)(1.0 / P.z_delta);
+  if (!(fj >= 1.0f)) return true;                       // first bracket: 2/z unbounded
   int64_t j = (int64_t)fj;
   if (j > P.z_N - 2) return false;
   double2 e = P.rs_env[j];
-  double pbound = P.rate_unit * (e.x + chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
+  double pbound = P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
   return !(2.0 * pbound < 0x1p-53);
 }
 
@@ -313,7 +312,7 @@ __device__ inline uint32_t next_epoch(const ScanState* scan) {
 // A tile is ROWS rows of 256 consecutive particles (thread t handles particle
 // t of every row, so loads stay coalesced); the QED list keeps ascending
 // index order, i.e. (row, wave, lane) order, with one look-back per tile.
-static constexpr int ROWS = 8;
+static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM>
 __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, StepArgs a) {
@@ -362,8 +361,10 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       }
       bool hot = false;
       if (QED && active && sp.push) {
-        double S = sfp_field_bound<FM>(&P.fields, x, y, z, a.t);
-        double chi_b = P.chi_coef * S / ig;  // >= 2 chi (gamma = 1/ig >= |u|)
+        // chi <= (e hbar/(m^2 c^3)) gamma (|E| + c|B|) (gamma >= |u|); the FP32
+        // bound and this product carry a small margin for their own rounding
+        float S = sfp_field_bound_f32<FM>(&P.fields, x, y, z, a.t);
+        float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
         hot = qed_possible(P, PHOTON, chi_b, ig, a.dt);
       }
       if (QED) {
@@ -392,7 +393,8 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     if (QED) {
       __syncthreads();
       if (wv == 0) {
-        uint64_t c = (lane < ROWS * WAVES) ? (uint64_t)__popcll((&sh_mask[0][0])[lane]) : 0;
+        uint64_t c = 0;
+        for (int k = lane; k < ROWS * WAVES; k += 64) c += (uint64_t)__popcll((&sh_mask[0][0])[k]);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
         uint64_t pre = lookback(P.flags[s][0], tile, c, epoch);
         if (lane == 0) sh_pref = pre;
@@ -756,7 +758,7 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   P.seed = params->seed;
   P.threshold = params->photon_threshold;
   P.qed = params->qed_model;
-  P.chi_coef = 2.0 * SFP_E * SFP_HBAR / ((SFP_ME * SFP_ME) * (SFP_C * SFP_C * SFP_C));
+  P.chi_coef = SFP_E * SFP_HBAR / ((SFP_ME * SFP_ME) * (SFP_C * SFP_C * SFP_C));
   bool any_qed = false;
   for (int s = 0; s < nspecies; ++s) {
     const sfp_species_desc& d = species[s];
