@@ -204,6 +204,17 @@ int sfp_update_tau_gamma(double* tau_dev, const double* inv_gamma_dev, const dou
                          double* delta_dev, const uint64_t* id_dev, uint64_t seed, uint64_t step,
                          uint32_t tag, uint32_t slot, void* tables, sfp_stream_t stream);
 
+/* The FP32 envelope bounds the fused step culls with, for n points: without
+ * velocities (vx_dev == NULL) out[i] bounds |E| + c|B| at (x, y, z)[i] and
+ * time t; with velocities it bounds |E| + c|B| at (x, y, z)[i] + v[i] s and
+ * time t + s for every s in [0, T] (a photon's straight path).  +inf where no
+ * bound is available.  No counterpart in the reference; exposed so that the
+ * bounds can be checked against the exact fields. */
+int sfp_field_bound_eval(const sfp_field* terms, int32_t nterms, const double* x_dev,
+                         const double* y_dev, const double* z_dev, double t, const double* vx_dev,
+                         const double* vy_dev, const double* vz_dev, double T, int64_t n,
+                         float* out_dev, sfp_stream_t stream);
+
 /* Elementary functions of the device math library on n device values, for
  * checking that the device and the host compute them identically (no
  * counterpart in the reference).  fn: 0 exp, 1 log, 2 sin, 3 cos, 4 atan,
@@ -295,13 +306,15 @@ int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t fl
 int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
-/* Counters of one species since the last reset (synchronous), stats[8]:
+/* Counters of one species since the last reset (synchronous), stats[10]:
  * [0] particle-steps (live particles after each step, the reference's
  *     ns/particle denominator, simulation.py:150-151)
  * [1] particles read by the sweep kernel   [2] particles appended by the species
  * [3] current buffered count               [4] particles on QED lists (sum over steps)
  * [5] host synchronisations of the run     [6] replays after a capacity halt
- * [7] current store capacity */
+ * [7] current store capacity
+ * [8] photon-steps inside multi-step pushes (the sweep reads only a 4-byte
+ *     wake step for these)                 [9] multi-step pushes started */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 

@@ -137,6 +137,19 @@ def field_at(terms, x, y, z, t, libm=False):
     return out
 
 
+def eval_field(terms, x, y, z, t, libm=False):
+    """Fields at arrays of points (the per-particle field pass, fields.py:26-46);
+    returns (Ex, Ey, Ez, Bx, By, Bz) arrays."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    n = x.shape[0]
+    F = [np.zeros(n) for _ in range(6)]
+    pr = np.zeros(n, np.uint8)
+    fa = field_array(terms)
+    lib(libm).orc_eval_field(C.byref(fa), len(terms), ptr(x), ptr(y), ptr(z), float(t), n, ptr(pr),
+                             *[ptr(a) for a in F])
+    return F
+
+
 def math(fn, x, y=None):
     """Own elementary functions: fn in exp, log, sin, cos, atan, pow, exp10, log10."""
     names = ["exp", "log", "sin", "cos", "atan", "pow", "exp10", "log10"]

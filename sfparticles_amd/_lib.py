@@ -106,6 +106,7 @@ SIGNATURES = {
     "sfp_update_tau_gamma": [P, P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],
     "sfp_update_inv_gamma": [P, P, P, P, I64, P, I32, P],
     "sfp_math_eval": [I32, P, P, P, I64, P],
+    "sfp_field_bound_eval": [P, I32, P, P, P, D, P, P, P, D, I64, P, P],
     "sfp_store_create": [I32, I64, C.POINTER(P)],
     "sfp_store_destroy": [P],
     "sfp_store_reserve": [P, I64],

@@ -320,12 +320,39 @@ SFP_HD void sfp_field_eval(const sfp_fieldset* fs, double x, double y, double z,
  * phase is formed in FP64 and rounded once, the rest is FP32 with hardware
  * exp2; the phase margin (1e-3 rad + 1e-6 |phi| beyond the analytic pi/2)
  * and a factor 1.01 on the result cover the rounding, so the value still
- * bounds |E| + c|B| from above. */
-template <int FM = SFP_FM_ANY>
-SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, double z, double t) {
+ * bounds |E| + c|B| from above.
+ *
+ * PATH: the bound holds at every point (x, y, z) + v s at time t + s for s in
+ * [0, T] -- a photon's straight path.  Along it the pulse phases are linear
+ * in s, so their smallest magnitude is taken at an end or is 0 where the sign
+ * changes; the radius is smallest at the closest approach to the axis (used
+ * in the transverse factor) and largest at an end (used in the curvature
+ * allowance); |x| is largest at an end (used in the beam width). */
+template <int FM, bool PATH>
+SFP_HD float sfp_field_bound_f32_impl(const sfp_fieldset* fs, double x, double y, double z,
+                                      double t, double vx, double vy, double vz, double T) {
   float S = 0.0f;
   double ct = SFP_C * t;
-  float r2 = (float)(y * y + z * z);
+  float r2, r2hi, ax;
+  double x1 = x, t1 = t;
+  if (PATH) {
+    double ww = vy * vy + vz * vz, pw = y * vy + z * vz;
+    double s = (ww > 0.0) ? -pw / ww : 0.0;
+    s = s < 0.0 ? 0.0 : (s > T ? T : s);
+    double ys = y + vy * s, zs = z + vz * s;
+    double y1 = y + vy * T, z1 = z + vz * T;
+    x1 = x + vx * T;
+    t1 = t + T;
+    double r0 = y * y + z * z, rT = y1 * y1 + z1 * z1;
+    r2 = (float)(ys * ys + zs * zs) * 0.9999f;
+    r2hi = (float)(r0 > rT ? r0 : rT) * 1.0001f;
+    double a0 = x < 0.0 ? -x : x, a1 = x1 < 0.0 ? -x1 : x1;
+    ax = (float)(a0 > a1 ? a0 : a1) * 1.0001f;
+  } else {
+    r2 = (float)(y * y + z * z);
+    r2hi = r2;
+    ax = (float)(x < 0.0 ? -x : x);
+  }
   const float LOG2E = 1.4426950408889634f;
   for (int j = 0; j < fs->nterms; ++j) {
     const sfp_term_dev* d = &fs->t[j];
@@ -334,8 +361,14 @@ SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, dou
     if (FM == SFP_FM_PLANE && kind != SFP_FIELD_SIMPLE) kind = SFP_FIELD_STATIC;
     if (kind == SFP_FIELD_SIMPLE) {
       if (d->n & 1) return __builtin_inff();  // odd envelope orders grow for phi < 0: no bound
-      float phi = (float)(c[FS_K0] * (c[FS_DIR] * x - c[FS_DX0] - ct));
-      float ph = phi < 0.0f ? -phi : phi;
+      double p0 = c[FS_K0] * (c[FS_DIR] * x - c[FS_DX0] - ct);
+      double pm = p0 < 0.0 ? -p0 : p0;
+      if (PATH) {
+        double p1 = c[FS_K0] * (c[FS_DIR] * x1 - c[FS_DX0] - SFP_C * t1);
+        double q1 = p1 < 0.0 ? -p1 : p1;
+        pm = ((p0 <= 0.0) != (p1 < 0.0)) ? 0.0 : (q1 < pm ? q1 : pm);
+      }
+      float ph = (float)pm;
       ph = ph - 1e-3f - ph * 1e-6f;
       if (ph < 0.0f) ph = 0.0f;
       float pn = 1.0f;
@@ -344,11 +377,17 @@ SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, dou
       S += 2.02f * (float)c[FS_E0] * exp2f(-a * LOG2E);
     } else if (kind == SFP_FIELD_GAUSSIAN) {
       float k0 = (float)c[FG_K0];
-      float phl = (float)(k0 * (x - c[FG_X0] - c[FG_DC] * t));
-      phl = (phl < 0.0f ? -phl : phl);
-      phl = phl - 1.5717963f - k0 * r2 * (float)c[FG_R4ZR] - phl * 1e-6f;
+      double p0 = x - c[FG_X0] - c[FG_DC] * t;
+      double pm = p0 < 0.0 ? -p0 : p0;
+      if (PATH) {
+        double p1 = x1 - c[FG_X0] - c[FG_DC] * t1;
+        double q1 = p1 < 0.0 ? -p1 : p1;
+        pm = ((p0 <= 0.0) != (p1 < 0.0)) ? 0.0 : (q1 < pm ? q1 : pm);
+      }
+      float phl = (float)(c[FG_K0] * pm);
+      phl = phl - 1.5717963f - k0 * r2hi * (float)c[FG_R4ZR] - phl * 1e-6f;
       if (phl < 0.0f) phl = 0.0f;
-      float xz = (float)x * (float)c[FG_RZR];
+      float xz = ax * (float)c[FG_RZR];
       float a = phl * phl * (float)c[FG_RKCT2] + r2 * (float)c[FG_RW0SQ] / (1.0f + xz * xz);
       S += 2.02f * (float)c[FG_E0] * exp2f(-a * LOG2E);
     } else if (kind == SFP_FIELD_STATIC) {
@@ -358,6 +397,17 @@ SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, dou
     }
   }
   return S;
+}
+
+template <int FM = SFP_FM_ANY>
+SFP_HD float sfp_field_bound_f32(const sfp_fieldset* fs, double x, double y, double z, double t) {
+  return sfp_field_bound_f32_impl<FM, false>(fs, x, y, z, t, 0.0, 0.0, 0.0, 0.0);
+}
+
+template <int FM = SFP_FM_ANY>
+SFP_HD float sfp_field_bound_f32_path(const sfp_fieldset* fs, double x, double y, double z,
+                                      double t, double vx, double vy, double vz, double T) {
+  return sfp_field_bound_f32_impl<FM, true>(fs, x, y, z, t, vx, vy, vz, T);
 }
 
 /* Upper bound of |E| + c|B| at (x, y, z, t), from the envelopes only (no

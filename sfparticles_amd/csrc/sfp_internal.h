@@ -72,7 +72,9 @@ struct Counts {
   int64_t spawned;    // particles this species appended (per destination)
   int64_t n_hot;      // particles on this step's QED list
   int64_t hot_total;  // sum of n_hot over steps
-  int64_t pad[7];
+  int64_t slept;      // particle-steps skipped inside a multi-step photon push
+  int64_t sleeps;     // multi-step photon pushes started
+  int64_t pad[5];
 };
 
 extern const size_t kAttrSize[SFP_NATTR];

@@ -93,6 +93,35 @@ extern "C" int sfp_eval_field(const sfp_field* terms, int32_t nterms, const doub
   return SFP_OK;
 }
 
+__global__ void k_field_bound(const sfp_fieldset* fs, const double* x, const double* y,
+                              const double* z, double t, const double* vx, const double* vy,
+                              const double* vz, double T, int64_t n, float* out) {
+  KLOOP(n) {
+    out[ip] = vx ? sfp_field_bound_f32_path<SFP_FM_ANY>(fs, x[ip], y[ip], z[ip], t, vx[ip], vy[ip],
+                                                       vz[ip], T)
+                 : sfp_field_bound_f32<SFP_FM_ANY>(fs, x[ip], y[ip], z[ip], t);
+  }
+}
+
+extern "C" int sfp_field_bound_eval(const sfp_field* terms, int32_t nterms, const double* x,
+                                    const double* y, const double* z, double t, const double* vx,
+                                    const double* vy, const double* vz, double T, int64_t n,
+                                    float* out, sfp_stream_t stream) {
+  if (n <= 0) return SFP_OK;
+  if (!(T >= 0.0)) return set_err(SFP_E_ARG, "path duration must be >= 0");
+  if ((vx == nullptr) != (vy == nullptr) || (vx == nullptr) != (vz == nullptr))
+    return set_err(SFP_E_ARG, "give all three velocity components or none");
+  sfp_fieldset fs;
+  CHK(sfp_compile_fields(terms, nterms, &fs));
+  CHK(g_fieldbuf.ensure(sizeof(fs)));
+  HIPCHK(hipMemcpyAsync(g_fieldbuf.p, &fs, sizeof(fs), hipMemcpyHostToDevice, S(stream)));
+  hipLaunchKernelGGL(k_field_bound, dim3(op_grid(n)), dim3(OPB), 0, S(stream),
+                     (const sfp_fieldset*)g_fieldbuf.p, x, y, z, t, vx, vy, vz, T, n, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return SFP_OK;
+}
+
 __global__ void k_push_position(double* x, double* y, double* z, const double* ux,
                                 const double* uy, const double* uz, const double* ig, int64_t n,
                                 const uint8_t* pruned, double cdt) {

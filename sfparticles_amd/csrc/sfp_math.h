@@ -307,4 +307,92 @@ SFP_HD sfp_u32x4 sfp_draw4(uint64_t seed, uint64_t id, uint64_t step, uint32_t s
                            (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
+/* ------------------------------------------------------- repeated add -- */
+/* Exactly the result of n sequential additions x <- x + h (IEEE double,
+ * round to nearest even), in a number of operations proportional to the
+ * binades x crosses rather than to n.
+ *
+ * Inside a binade [2^e, 2^(e+1)) the doubles are the integer multiples k u
+ * of u = 2^(e-52), k in [2^52, 2^53).  While the exact sum (k + h/u) u stays
+ * inside the binade, x + h rounds to (k + q) u with q the integer nearest
+ * r = h/u, the same every step.  A tie (r - floor(r) = 1/2) rounds to the
+ * even neighbour: once k is even the increment is the even one of floor(r),
+ * floor(r) + 1 and k stays even, so the increment is again constant after at
+ * most one real addition.  s steps stay inside the binade when
+ * 2^52 + 1 <= k + s q <= 2^53 - 1 (this covers |r - q| <= 1/2).  Binade
+ * crossings, zeros, subnormal or huge values and non-finite operands take
+ * single real additions.  Negative x uses the mirror image (rounding to
+ * nearest is symmetric). */
+SFP_HD double sfp_repeat_add(double x, double h, uint32_t n) {
+  if (n == 0) return x;
+  if (h == 0.0 || !(h - h == 0.0) || !(x - x == 0.0)) {
+    // the first addition fixes the sign of a zero / produces inf or NaN;
+    // every later one returns its input
+    return x + h;
+  }
+  const int64_t LO = (int64_t)1 << 52, HI = (int64_t)1 << 53;
+  while (n > 0) {
+    const int neg = x < 0.0;
+    const double ax = neg ? -x : x;
+    const uint64_t bits = sfp_d2u(ax);
+    const int be = (int)(bits >> 52);
+    if (be < 54 || be > 2045) {  // zero, tiny or huge: one real addition
+      x = x + h;
+      --n;
+      continue;
+    }
+    const double u = sfp_u2d((uint64_t)(be - 52) << 52);
+    const double H = neg ? -h : h;
+    const double r = H / u;  // exact: division by a power of two
+    if (!(r < 4503599627370496.0 && r > -4503599627370496.0)) {
+      x = x + h;
+      --n;
+      continue;
+    }
+    int64_t k = (int64_t)((bits & (((uint64_t)1 << 52) - 1)) | ((uint64_t)1 << 52));
+    const double fr = sfp_floor(r);
+    const double f = r - fr;  // exact
+    int64_t q;
+    if (f == 0.5) {
+      if (k & 1) {  // the parity of k decides this one rounding
+        x = x + h;
+        --n;
+        continue;
+      }
+      const int64_t fi = (int64_t)fr;
+      q = (fi & 1) ? fi + 1 : fi;
+    } else {
+      q = (int64_t)(f < 0.5 ? fr : fr + 1.0);
+    }
+    int64_t m;  // steps that stay inside the binade
+    if (q != 0) {
+      // floor(room / |q|) with room, |q| < 2^53 exact in double; the rounded
+      // quotient is off by at most one, which the integer checks correct
+      const int64_t room = (q > 0) ? HI - 1 - k : k - LO - 1;
+      const int64_t aq = (q > 0) ? q : -q;
+      if (room < 0) {
+        m = 0;
+      } else {
+        m = (int64_t)sfp_floor((double)room / (double)aq);
+        if (m * aq > room) --m;
+        else if ((m + 1) * aq <= room) ++m;
+      }
+    } else {
+      m = (r >= 0.0 || k >= LO + 1) ? (int64_t)n : 0;  // x does not move
+    }
+    int64_t s = (m < (int64_t)n) ? m : (int64_t)n;
+    if (s > 0) {
+      k += s * q;
+      const double X = (double)k * u;  // exact: integer below 2^53 times 2^j
+      x = neg ? -X : X;
+      n -= (uint32_t)s;
+    }
+    if (n > 0) {  // the next addition leaves the binade: a real one
+      x = x + h;
+      --n;
+    }
+  }
+  return x;
+}
+
 #endif /* SFP_MATH_H */

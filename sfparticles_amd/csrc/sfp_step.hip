@@ -42,7 +42,8 @@ struct SpeciesDev {
   uint8_t* pruned;
   double* tau;
   double *sx, *sy, *sz;
-  uint32_t* hot;  // this step's QED list (ascending particle indices)
+  uint32_t* hot;   // this step's QED list (ascending particle indices)
+  uint32_t* wake;  // photons: run-relative step of the next visit (multi-step push), or null
   Counts* cnt;
   int64_t cap;
   double q, ae;
@@ -86,8 +87,9 @@ struct Plan {
 };
 
 struct StepArgs {
-  double t, t_half, dt, cdt_half, ll_factor;
+  double t, t_half, dt, cdt_half, cdt, ll_factor;
   uint64_t step;
+  uint32_t rel, rel_end;  // step index within this run call, and the run's length
 };
 
 // --------------------------------------------------------- look-back scan --
@@ -281,26 +283,27 @@ __device__ inline void kernel_exit(ScanState* scan, F&& last) {
   }
 }
 
-__device__ inline void block_sum2(uint32_t a, uint32_t b, uint32_t* sh, uint64_t* ta, uint64_t* tb) {
+// Workgroup sums of N per-thread counters; every thread gets the totals.
+// sh holds N * WAVES entries.
+template <int N>
+__device__ inline void block_sum(uint32_t (&v)[N], uint32_t* sh, uint64_t (&tot)[N]) {
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_xor(a, off);
-    b += __shfl_xor(b, off);
-  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], off);
   __syncthreads();
-  if (lane == 0) {
-    sh[wv] = a;
-    sh[WAVES + wv] = b;
-  }
+  if (lane == 0)
+    for (int j = 0; j < N; ++j) sh[j * WAVES + wv] = v[j];
   __syncthreads();
-  uint64_t sa = 0, sb = 0;
-  for (int k = 0; k < WAVES; ++k) {
-    sa += sh[k];
-    sb += sh[WAVES + k];
+  for (int j = 0; j < N; ++j) {
+    uint64_t s = 0;
+    for (int k = 0; k < WAVES; ++k) s += sh[j * WAVES + k];
+    tot[j] = s;
   }
-  *ta = sa;
-  *tb = sb;
+}
+
+__device__ inline void count_add(int64_t* dst, uint64_t v) {
+  if (v) atomicAdd((unsigned long long*)dst, (unsigned long long)v);
 }
 
 __device__ inline uint32_t next_epoch(const ScanState* scan) {
@@ -308,10 +311,47 @@ __device__ inline uint32_t next_epoch(const ScanState* scan) {
   return e == 0 ? 1u : e;
 }
 
+// --------------------------------------------------- multi-step photon push --
+// A photon moves on a straight line at c and its momentum never changes, so
+// while no QED event is possible its push is the same two position
+// increments every step.  Number of steps, from this one on, for which the
+// envelope bound along the photon's own path proves that no event is possible
+// (1 if only this step is proven).  Windows end at the end of the run or at
+// the next multiple of 512, 64 or 8 steps, so that neighbouring photons --
+// one wave's lanes -- wake on the same steps.  Photons of a species without
+// QED sleep to the end of the run.
+template <int FM>
+__device__ inline uint32_t sleep_steps(const Plan& P, const StepArgs& a, bool qed, double x,
+                                       double y, double z, double ux, double uy, double uz,
+                                       double ig) {
+  const uint32_t rem = a.rel_end - a.rel;
+  if (rem <= 1) return 1;
+  if (!qed) return rem;
+  const double vx = SFP_C * ig * ux, vy = SFP_C * ig * uy, vz = SFP_C * ig * uz;
+  const uint32_t q[3] = {512u, 64u, 8u};
+  for (int k = 0; k < 4; ++k) {
+    uint32_t end = (k == 0) ? a.rel_end : (a.rel / q[k - 1] + 1u) * q[k - 1];
+    if (end > a.rel_end || end < a.rel + 2u || (k > 0 && end == a.rel_end)) continue;
+    uint32_t d = end - a.rel;
+    float S = sfp_field_bound_f32_path<FM>(&P.fields, x, y, z, a.t, vx, vy, vz,
+                                           a.dt * (double)d * 1.000001);
+    float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+    if (!qed_possible(P, true, chi_b, ig, a.dt)) return d;
+  }
+  return 1;
+}
+
 // ------------------------------------------------------------ sweep kernel --
 // A tile is ROWS rows of 256 consecutive particles (thread t handles particle
 // t of every row, so loads stay coalesced); the QED list keeps ascending
 // index order, i.e. (row, wave, lane) order, with one look-back per tile.
+//
+// Photons with a wake array: a photon whose wake step lies ahead is skipped
+// (it was pushed through that step already); a photon proven cold for n > 1
+// steps is pushed n steps at once -- the exact result of the 2n sequential
+// FP64 additions of the same increment that n single steps make
+// (sfp_repeat_add), so positions are bit-identical -- and sleeps until then.  Windows end at the end of the run, so every
+// position is current when the run returns.
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM>
@@ -320,7 +360,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
   __shared__ int64_t sh_tile;
-  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_mask[ROWS][WAVES];
   __shared__ uint64_t sh_pref;
   constexpr int64_t TSPAN = (int64_t)TILE * ROWS;
@@ -331,7 +371,11 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
   const uint32_t epoch = QED ? next_epoch(scan) : 0u;
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
-  uint32_t alive = 0, proc = 0;
+  // alive, processed, slept, sleeps
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  uint32_t& alive = cnt[0];
+  uint32_t& proc = cnt[1];
+  uint32_t* const wake = PHOTON ? sp.wake : nullptr;
 
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
@@ -341,18 +385,40 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     if (tile >= ntiles) break;
     const int64_t base = tile * TSPAN;
     uint32_t hotbits = 0;
+    // photons: the wake steps of all rows of the tile, loaded together (one
+    // dependent load per row would leave the sweep latency-bound)
+    uint32_t asleep = 0;
+    if (PHOTON && wake) {
+      uint32_t w[ROWS];
+#pragma unroll
+      for (int row = 0; row < ROWS; ++row) {
+        const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
+        w[row] = (i < n_old) ? wake[i] : 0u;
+      }
+#pragma unroll
+      for (int row = 0; row < ROWS; ++row) asleep |= (uint32_t)(w[row] > a.rel) << row;
+      // a sleeping photon is live (only live photons sleep) and already pushed
+      const uint32_t ns = (uint32_t)__popc(asleep);
+      alive += ns;
+      cnt[2] += ns;
+    }
 
 #pragma unroll 1
     for (int row = 0; row < ROWS; ++row) {
       const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
+      if (PHOTON && wake && !__any(i < n_old && !((asleep >> row) & 1u))) {
+        // the whole wave sleeps (or lies past the end) in this row
+        if (QED && lane == 0) sh_mask[row][wv] = 0;
+        continue;
+      }
       bool active = false;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, sx = 0, sy = 0, sz = 0;
-      if (i < n_old) {
-        active = sp.pruned[i] == 0;
-        if (active) {
-          x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-          ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-        }
+      if (i < n_old && !((asleep >> row) & 1u)) {
+        // the pruned flag and the state are loaded together
+        const uint8_t pr = sp.pruned[i];
+        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+        active = pr == 0;
       }
       double ig = 1.0;
       if (active) {
@@ -374,10 +440,27 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       }
       if (active && !hot) {
         if (PHOTON) {
+          uint32_t nst = 1;
+          if (wake) nst = sp.push ? sleep_steps<FM>(P, a, QED, x, y, z, ux, uy, uz, ig) : a.rel_end - a.rel;
           if (sp.push) {
-            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-            sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
+            // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
+            // increment each time
+            const double hx = a.cdt_half * ig * ux, hy = a.cdt_half * ig * uy,
+                         hz = a.cdt_half * ig * uz;
+            if (nst == 1) {
+              x += hx; x += hx;
+              y += hy; y += hy;
+              z += hz; z += hz;
+            } else {  // the same 2 nst additions, in closed form
+              x = sfp_repeat_add(x, hx, 2 * nst);
+              y = sfp_repeat_add(y, hy, 2 * nst);
+              z = sfp_repeat_add(z, hz, 2 * nst);
+            }
             sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          }
+          if (nst > 1) {
+            wake[i] = a.rel + nst;
+            cnt[3] += 1;
           }
         } else if (sp.push) {
           if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
@@ -419,11 +502,13 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     }
   }
 
-  uint64_t talive, tproc;
-  block_sum2(alive, proc, sh_w, &talive, &tproc);
+  uint64_t tot[4];
+  block_sum(cnt, sh_w, tot);
   if (threadIdx.x == 0) {
-    if (talive) atomicAdd((unsigned long long*)&sp.cnt->psteps, (unsigned long long)talive);
-    if (tproc) atomicAdd((unsigned long long*)&sp.cnt->processed, (unsigned long long)tproc);
+    count_add(&sp.cnt->psteps, tot[0]);
+    count_add(&sp.cnt->processed, tot[1]);
+    count_add(&sp.cnt->slept, tot[2]);
+    count_add(&sp.cnt->sleeps, tot[3]);
   }
   kernel_exit(scan, [&]() {
     if (QED) {
@@ -485,7 +570,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   }
   __syncthreads();
 
-  uint32_t alive = 0, hot_seen = 0;
+  uint32_t alive = 0;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
@@ -505,7 +590,6 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       if (KIND == KIND_OD) tau = sp.tau[i];
       if (!PHOTON && sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       pid = sp.id[i];
-      hot_seen += 1;
     }
     double ig = 1.0;
     if (valid) ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
@@ -594,10 +678,10 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
     }
   }
 
-  uint64_t talive, tseen;
-  block_sum2(alive, hot_seen, sh_w, &talive, &tseen);
-  if (threadIdx.x == 0 && talive)
-    atomicAdd((unsigned long long*)&sp.cnt->psteps, (unsigned long long)talive);
+  uint32_t cnt[1] = {alive};
+  uint64_t tot[1];
+  block_sum(cnt, sh_w, tot);
+  if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
   kernel_exit(scan, [&]() {
     uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
     if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
@@ -666,6 +750,8 @@ struct Sim {
   std::vector<int64_t> hot_cap;
   std::vector<uint64_t*> flags[2];
   std::vector<uint32_t*> hot_mem;
+  std::vector<uint32_t*> wake_mem;  // photon species: multi-step push bookkeeping
+  std::vector<int64_t> wake_cap;
   ScanState* scan_mem = nullptr;
   Halt* halt = nullptr;            // device
   Halt* halt_host = nullptr;       // pinned
@@ -733,6 +819,21 @@ static int sim_refresh_plan(Sim* sim) {
       sim->hot_cap[s] = want;
     }
     d.hot = sim->hot_mem[s];
+    if (d.photon && st->cap > sim->wake_cap[s]) {
+      // grown inside a run: sleeping photons keep their wake steps
+      int64_t want = std::max<int64_t>(st->cap, 1);
+      uint32_t* w = nullptr;
+      HIPCHK(hipMalloc(&w, sizeof(uint32_t) * want));
+      HIPCHK(hipMemset(w, 0, sizeof(uint32_t) * want));
+      if (sim->wake_mem[s]) {
+        HIPCHK(hipMemcpy(w, sim->wake_mem[s], sizeof(uint32_t) * sim->wake_cap[s],
+                         hipMemcpyDeviceToDevice));
+        HIPCHK(hipFree(sim->wake_mem[s]));
+      }
+      sim->wake_mem[s] = w;
+      sim->wake_cap[s] = want;
+    }
+    d.wake = sim->wake_mem[s];
   }
   HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
   return SFP_OK;
@@ -830,6 +931,8 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   }
   sim->hot_mem.assign(nspecies, nullptr);
   sim->hot_cap.assign(nspecies, 0);
+  sim->wake_mem.assign(nspecies, nullptr);
+  sim->wake_cap.assign(nspecies, 0);
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -863,6 +966,8 @@ extern "C" int sfp_sim_destroy(void* p) {
       if (f) (void)hipFree(f);
   for (auto h : sim->hot_mem)
     if (h) (void)hipFree(h);
+  for (auto w : sim->wake_mem)
+    if (w) (void)hipFree(w);
   if (sim->scan_mem) (void)hipFree(sim->scan_mem);
   if (sim->plan_dev) (void)hipFree(sim->plan_dev);
   if (sim->halt) (void)hipFree(sim->halt);
@@ -951,6 +1056,13 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
     if (sim->plan.sp[s].cap != sim->stores[s]->cap || sim->plan.sp[s].x != sim->stores[s]->p[SFP_X])
       dirty = true;
   if (dirty) CHK(sim_refresh_plan(sim));
+  if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
+    return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
+  // every photon is visited at the first step of a run (positions are current
+  // between runs; the store may have changed)
+  for (int s = 0; s < sim->nsp; ++s)
+    if (sim->wake_mem[s])
+      HIPCHK(hipMemsetAsync(sim->wake_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], 0));
   const double dt = sim->params.dt;
   std::vector<StepArgs> args;
   args.reserve((size_t)std::min<int64_t>(nsteps, SYNC_EVERY));
@@ -965,8 +1077,11 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       a.t_half = tt + 0.5 * dt;
       a.dt = dt;
       a.cdt_half = SFP_C * (0.5 * dt);
+      a.cdt = SFP_C * dt;
       a.ll_factor = sfp_ll_factor(dt);
       a.step = step0 + (uint64_t)(k + j);
+      a.rel = (uint32_t)(k + j);
+      a.rel_end = (uint32_t)nsteps;
       args.push_back(a);
       tt = tt + dt;
       CHK(launch_step(sim, a, 0, 0));
@@ -1041,6 +1156,8 @@ extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
   stats[5] = sim->syncs;
   stats[6] = sim->replays;
   stats[7] = sim->stores[species]->cap;
+  stats[8] = c.slept;
+  stats[9] = c.sleeps;
   return SFP_OK;
 }
 
@@ -1050,7 +1167,7 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   CHK(sim_read_counts(sim));
   for (int s = 0; s < sim->nsp; ++s) {
     Counts c = sim->counts_host[s];
-    c.psteps = c.processed = c.spawned = c.hot_total = 0;
+    c.psteps = c.processed = c.spawned = c.hot_total = c.slept = c.sleeps = 0;
     HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   }
   sim->syncs = 0;

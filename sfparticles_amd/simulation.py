@@ -133,10 +133,11 @@ class Simulation(object):
         """Counters of one species since the last reset: particle-steps (live
         particles after each step), particles read by its step kernel,
         particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 8)()
+        v = (C.c_int64 * 10)()
         L.call("sfp_sim_stats", self._sim, int(species), v)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
-                    hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]))
+                    hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
+                    slept=int(v[8]), sleeps=int(v[9]))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)

@@ -70,6 +70,8 @@ def test_head_on_rs_bit_exact():
     # the envelope bound culled part of the photons' QED work (photons
     # leaving the pulse) and everything still matches
     assert st[2]["hot"] < st[2]["processed"]
+    # photons behind the pulse were pushed several steps at once, bit-exactly
+    assert st[2]["slept"] > 0 and st[2]["sleeps"] > 0
 
 
 def test_head_on_rs_chunked_equals_single():
@@ -272,3 +274,64 @@ def test_pair_bookkeeping_and_momentum():
         pho._create_pair(ele, pos)
         assert pos.N_buffered == pos.Npart
         assert N - pho.Npart == pos.Npart
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["gaussian", "standing_wave"])
+def test_envelope_bounds_are_upper_bounds(which):
+    """The FP32 envelope bounds the step culls with exceed |E| + c|B| of the
+    exact FP64 field (oracle): the point bound at the point, the path bound at
+    every sampled point (x + v s, t + s), s in [0, T], of a photon's straight
+    path -- and neither is vacuous."""
+    import oracle
+    from sfparticles_amd import _lib as L
+    from sfparticles_amd.fields import terms_to_c
+    C_LIGHT = 299792458.0
+    rng = np.random.Generator(np.random.PCG64(7))
+    if which == "gaussian":
+        case = head_on_case(n=1, nsteps=3000)
+    else:
+        case = cascade_case(n=1, nsteps=100, a0=500.0, x0_ctau=1.0)
+    terms = case["field"]
+    tc = terms_to_c(terms)
+    m = 4000
+    t0 = 1500 * case["dt"] if which == "gaussian" else 0.0
+    x = rng.uniform(-20e-6, 20e-6, m)
+    y = rng.normal(0, 2e-6, m)
+    z = rng.normal(0, 2e-6, m)
+    d = rng.normal(size=(3, m))
+    d[0] += np.where(rng.random(m) < 0.5, -3.0, 3.0)  # mostly along the axis, both ways
+    d /= np.linalg.norm(d, axis=0)
+    v = C_LIGHT * d
+
+    def bound(T, path):
+        dev = [L.DeviceArray.of(np.ascontiguousarray(a)) for a in (x, y, z, v[0], v[1], v[2])]
+        out = L.DeviceArray(4 * m)
+        vel = [a.ptr for a in dev[3:]] if path else [None, None, None]
+        L.call("sfp_field_bound_eval", tc, len(terms), dev[0].ptr, dev[1].ptr, dev[2].ptr,
+               float(t0), *vel, float(T), m, out.ptr, None)
+        b = np.empty(m, np.float32)
+        out.download(b)
+        return b.astype(np.float64)
+
+    def mag(s):
+        F = oracle.eval_field(terms, x + v[0] * s, y + v[1] * s, z + v[2] * s, t0 + s)
+        return (np.sqrt(F[0] ** 2 + F[1] ** 2 + F[2] ** 2)
+                + C_LIGHT * np.sqrt(F[3] ** 2 + F[4] ** 2 + F[5] ** 2))
+
+    b0 = bound(0.0, False)
+    m0 = mag(0.0)
+    assert np.all(b0 >= m0), np.max(m0 - b0)
+    print(which, "point: max field/bound", np.max(m0 / b0))
+    assert np.max(m0 / b0) > 0.05
+    for nsteps in (8, 64, 512, 3000):
+        T = nsteps * case["dt"]
+        b = bound(T, True)
+        assert np.all(b >= b0 * 0.999999)
+        worst = 0.0
+        for s in np.concatenate([[0.0, T], rng.random(10) * T]):
+            mm = mag(s)
+            assert np.all(b >= mm), (nsteps, s, np.max(mm - b))
+            worst = max(worst, float(np.max(mm / b)))
+        print(which, nsteps, "path: max field/bound", worst)
+        assert worst > 0.05
