@@ -37,23 +37,31 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes of the two step kernels of a species (the persistent
 # state read once and written once; transients stay in registers):
-#   sweep: photons: reads the 4-byte wake step of every buffered photon; a
-#          photon inside a multi-step push costs nothing more.  Every particle
-#          it visits: reads x, y, z, u and the pruned flag (49 B); writes
-#          x, y, z (photon, 24 B) or x, y, z, u (lepton, 48 B) of every
-#          particle it pushes, a 4-byte wake step for every multi-step push it
-#          starts and a 4-byte index for every particle it puts on the QED list
-#   hot:   per listed particle reads the index, x, y, z, u and the id (60 B,
-#          + 8 B tau read and written under the optical-depth model), writes
-#          x, y, z (+ u for leptons); per appended particle writes x, y, z, u,
-#          id and flag (57 B)
+#   leptons  sweep: reads x, y, z, u and the pruned flag of every buffered
+#            particle (49 B); writes x, y, z, u (48 B) of every particle it
+#            pushes and a 4-byte index for every particle it lists.
+#            QED kernel: per listed particle reads the index, x, y, z, u and
+#            the id (60 B) and writes x, y, z, u (48 B).
+#   photons  list kernel: reads the 4-byte wake step of every buffered photon
+#            and writes a 4-byte index for every photon due this step.
+#            photon kernel: per listed photon reads the index, the pruned flag,
+#            x, y, z and u (53 B), the id where the exact QED front runs
+#            (8 B); writes x, y, z (24 B) of every photon it pushes, a 4-byte
+#            wake step per multi-step push, and per conversion the pruned flag
+#            and wake step (5 B).
+#   Optical depth: + tau read and written (16 B) per exact QED evaluation.
+#   Per appended particle: x, y, z, u, id and flag (57 B) in each destination.
 def kernel_bytes(photon, phase, st, od=False):
-    pushed_w = 24 if photon else 48
+    hot = st["hot"]
+    tau = 16 * hot if od else 0
+    if photon:
+        if phase == 0:
+            return 4 * (st["processed"] + st["slept"]) + 4 * st["processed"]
+        return (53 * st["processed"] + 8 * hot + tau + 24 * (st["processed"] - st["spawned"])
+                + 4 * st["sleeps"] + 5 * st["spawned"] + 57 * 2 * st["spawned"])
     if phase == 0:
-        wake = 4 * (st["processed"] + st["slept"]) + 4 * st["sleeps"] if photon else 0
-        return wake + 49 * st["processed"] + pushed_w * (st["processed"] - st["hot"]) + 4 * st["hot"]
-    return ((60 + (16 if od else 0)) * st["hot"] + pushed_w * st["hot"]
-            + 57 * st["spawned"] * (2 if photon else 1))
+        return 49 * st["processed"] + 48 * (st["processed"] - hot) + 4 * hot
+    return 60 * hot + 48 * hot + tau + 57 * st["spawned"]
 
 
 def make_case(n, nsteps, seed=0):
@@ -203,8 +211,9 @@ def main():
     per_launch = alg_bytes / args.steps
     avg_s = ms / 1e3 / args.steps
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    kname = (f"k_{'sweep' if ph == 0 else 'hot'}<{'photon' if dom == 2 else 'lepton'}, RS, "
-             f"gaussian field> ({names[dom]})")
+    kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>", (1, False): "k_hot<RS, gaussian field>",
+              (0, True): "k_awake", (1, True): "k_photon<RS, gaussian field>"}[(ph, dom == 2)]
+             + f" ({names[dom]})")
     traffic = load_traffic(f"{'sweep' if ph == 0 else 'hot'}_{names[dom]}_rs")
     total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -74,7 +74,9 @@ struct Counts {
   int64_t hot_total;  // sum of n_hot over steps
   int64_t slept;      // particle-steps skipped inside a multi-step photon push
   int64_t sleeps;     // multi-step photon pushes started
-  int64_t pad[5];
+  int64_t list_end;   // photons: n_old when the visit list was last built or extended
+  int64_t asleep_n;   // photons: live photons inside a multi-step push
+  int64_t pad[3];
 };
 
 extern const size_t kAttrSize[SFP_NATTR];

@@ -441,7 +441,12 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       if (active && !hot) {
         if (PHOTON) {
           uint32_t nst = 1;
-          if (wake) nst = sp.push ? sleep_steps<FM>(P, a, QED, x, y, z, ux, uy, uz, ig) : a.rel_end - a.rel;
+          // multi-step decisions are taken every 8th step, so that the lanes of
+          // a wave take them together
+          if (wake)
+            nst = !sp.push ? a.rel_end - a.rel
+                           : ((a.rel & 7u) == 0 ? sleep_steps<FM>(P, a, QED, x, y, z, ux, uy, uz, ig)
+                                                : 1u);
           if (sp.push) {
             // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
             // increment each time
@@ -493,8 +498,9 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
           total += c;
         }
         if (hotbits & (1u << row)) {
-          uint64_t r = off + before + (uint32_t)__popcll(sh_mask[row][wv] & lt);
-          sp.hot[r] = (uint32_t)(base + (int64_t)row * TILE + threadIdx.x);
+          const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
+          const uint64_t r = off + before + (uint32_t)__popcll(sh_mask[row][wv] & lt);
+          sp.hot[r] = (uint32_t)i;
         }
         off += total;
       }
@@ -522,8 +528,105 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
   });
 }
 
-// -------------------------------------------------------------- hot kernel --
+// ------------------------------------------------------------- QED front --
+// Room for one append per listed particle in every destination, or raise the
+// halt flag (every workgroup reads the same counts and takes the same
+// decision).  Returns false when the kernel must stop.
+__device__ inline bool room_or_halt(const Plan& P, int s, const StepArgs& a, const SpeciesDev* dA,
+                                    const SpeciesDev* dB, int64_t n_list) {
+  const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
+  if ((dA && baseA + n_list > dA->cap) || (dB && baseB + n_list > dB->cap)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      P.halt->species = s;
+      P.halt->step = (int64_t)a.step;
+      P.halt->flag = 1;
+    }
+    return false;
+  }
+  return true;
+}
+
+// Exact QED decision of one particle at time t (simulation.py:103-114):
+// field, chi, then the rejection-sampling draw (qed/rejection_sampling.py:68-101)
+// or the optical-depth update (qed/optical_depth.py:15-110), then the
+// hard-photon cut for radiating leptons (cpu.py:227-232).
 template <bool PHOTON, int KIND, int FM>
+__device__ inline bool qed_event(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                 const sfp_rs_node* lds_rs, const double2* lds_od, double x,
+                                 double y, double z, double ux, double uy, double uz, double ig,
+                                 uint64_t pid, double& tau, double& delta) {
+  bool event = false;
+  delta = 0.0;
+  double F[6];
+  sfp_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
+  if (KIND == KIND_RS) {
+    if (chi != 0.0) {
+      sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+      double r1 = sfp_u01(d.v[0], d.v[1]);
+      double r2 = sfp_u01(d.v[2], d.v[3]);
+      double dtau = a.dt * ig;
+      if (!PHOTON) {
+        double r1_3 = sfp_powi(r1, 3);
+        double p = 3 * sfp_powi(r1, 2) *
+                   sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1) * dtau;
+        if (r2 < p) { event = true; delta = r1_3; }
+      } else {
+        double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
+        if (r2 < p) { event = true; delta = r1; }
+      }
+    }
+  } else {
+    // below chi_min (or NaN): skipped, tau unchanged
+    if (chi >= P.od.chi_min) {
+      event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
+                        PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
+    }
+  }
+  if (!PHOTON && event) {
+    if (delta / ig < P.threshold) event = false;
+  }
+  return event;
+}
+
+template <int KIND, bool PHOTON>
+__device__ inline void stage_tables(const Plan& P, sfp_rs_node* lds_rs, double2* lds_od) {
+  if (KIND == KIND_RS) {
+    int64_t nn = P.z_N - 1;
+    for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) lds_rs[k] = P.rs_nodes[k];
+  } else {
+    const double2* src = PHOTON ? P.od_pair_nodes : P.od_photon_nodes;
+    int64_t nn = P.od.chi_N - 1;
+    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = src[k];
+  }
+  __syncthreads();
+}
+
+// Counts of the destinations after the launch (run by its last workgroup).
+__device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntiles,
+                                      const SpeciesDev& sp, const SpeciesDev* dA,
+                                      const SpeciesDev* dB, int64_t baseA, int64_t baseB,
+                                      int64_t idA, int64_t idB) {
+  uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][ph][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+  if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
+  if (dA) {
+    dA->cnt->n = baseA + (int64_t)total;
+    dA->cnt->next_id = idA + (int64_t)total;
+    atomicAdd((unsigned long long*)&dA->cnt->psteps, (unsigned long long)total);
+  }
+  if (dB) {
+    dB->cnt->n = baseB + (int64_t)total;
+    dB->cnt->next_id = idB + (int64_t)total;
+    atomicAdd((unsigned long long*)&dB->cnt->psteps, (unsigned long long)total);
+  }
+}
+
+// ------------------------------------------------------- lepton QED kernel --
+// Over the sweep's QED list: exact QED front, stable rank of every emission
+// (wave64 ballot + workgroup scan + decoupled look-back), the new photon
+// appended already pushed (the reference pushes particles in the step that
+// creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
+template <int KIND, int FM>
 __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, StepArgs a) {
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -536,39 +639,14 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   __shared__ uint64_t sh_pref;
 
   const int64_t n_hot = sp.cnt->n_hot;
-  const SpeciesDev* dA = nullptr;  // photon dst, or electron dst
-  const SpeciesDev* dB = nullptr;  // positron dst
-  if (!PHOTON) {
-    if (sp.photon_dst >= 0) dA = &P.sp[sp.photon_dst];
-  } else {
-    if (sp.pair_e_dst >= 0) dA = &P.sp[sp.pair_e_dst];
-    if (sp.pair_p_dst >= 0) dB = &P.sp[sp.pair_p_dst];
-  }
-  const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
-  const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
-  // room for one append per listed particle in every destination, or halt
-  // (every workgroup reads the same counts and takes the same decision)
-  if ((dA && baseA + n_hot > dA->cap) || (dB && baseB + n_hot > dB->cap)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      P.halt->species = s;
-      P.halt->step = (int64_t)a.step;
-      P.halt->flag = 1;
-    }
-    return;
-  }
+  const SpeciesDev* dA = sp.photon_dst >= 0 ? &P.sp[sp.photon_dst] : nullptr;
+  if (!room_or_halt(P, s, a, dA, nullptr, n_hot)) return;
+  const int64_t baseA = dA ? dA->cnt->n : 0;
+  const int64_t idA = dA ? dA->cnt->next_id : 0;
   const int64_t ntiles = (n_hot + TILE - 1) / TILE;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
-
-  if (KIND == KIND_RS) {
-    int64_t nn = P.z_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) lds_rs[k] = P.rs_nodes[k];
-  } else {
-    const double2* src = PHOTON ? P.od_pair_nodes : P.od_photon_nodes;
-    int64_t nn = P.od.chi_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = src[k];
-  }
-  __syncthreads();
+  stage_tables<KIND, false>(P, lds_rs, lds_od);
 
   uint32_t alive = 0;
   while (true) {
@@ -588,93 +666,35 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
       ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
       if (KIND == KIND_OD) tau = sp.tau[i];
-      if (!PHOTON && sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+      if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       pid = sp.id[i];
     }
-    double ig = 1.0;
-    if (valid) ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
-
-    // ---- QED front at time t (simulation.py:103-114) ------------------
+    double ig = valid ? sfp_inv_gamma_lepton(ux, uy, uz) : 1.0;
     bool event = false;
     double delta = 0.0;
-    if (valid) {
-      double F[6];
-      sfp_field_eval<FM>(&P.fields, x, y, z, a.t, F);
-      double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
-      if (KIND == KIND_RS) {
-        // qed/rejection_sampling.py:68-101
-        if (chi != 0.0) {
-          sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
-          double r1 = sfp_u01(d.v[0], d.v[1]);
-          double r2 = sfp_u01(d.v[2], d.v[3]);
-          double dtau = a.dt * ig;
-          if (!PHOTON) {
-            double r1_3 = sfp_powi(r1, 3);
-            double p = 3 * sfp_powi(r1, 2) *
-                       sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1) * dtau;
-            if (r2 < p) { event = true; delta = r1_3; }
-          } else {
-            double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
-            if (r2 < p) { event = true; delta = r1; }
-          }
-        }
-      } else {
-        // qed/optical_depth.py:15-110 (below chi_min or NaN: skipped)
-        if (chi >= P.od.chi_min) {
-          event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
-                            PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
-        }
-      }
-      // cpu.py:227-232 (radiating leptons only)
-      if (!PHOTON && event) {
-        if (delta / ig < P.threshold) event = false;
-      }
-    }
-    if (!dA && !dB) event = false;
+    if (valid)
+      event = qed_event<false, KIND, FM>(P, sp, a, lds_rs, lds_od, x, y, z, ux, uy, uz, ig, pid,
+                                         tau, delta);
+    if (!dA) event = false;
 
-    // ---- stable rank and append (cpu.py:90-191) ------------------------
     int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
-    if (event) {
-      if (!PHOTON) {
-        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
-                     delta * uy, delta * uz);
-      } else {
-        double dp = 1.0 - delta;
-        if (dA)
-          spawn_lepton<FM>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
-                           delta * uy, delta * uz);
-        if (dB)
-          spawn_lepton<FM>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
-                           dp * uy, dp * uz);
-      }
-    }
+    if (event)
+      spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux, delta * uy,
+                   delta * uz);
 
-    // ---- recoil (cpu.py:78-88) and push (simulation.py:128-137) -------
     if (valid) {
-      if (PHOTON) {
-        if (event) {
-          sp.pruned[i] = 1;  // the converted photon becomes a hole (cpu.py:189-190)
-        } else {
-          sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-          sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
-          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-          if (KIND == KIND_OD) sp.tau[i] = tau;
-          alive += 1;
-        }
-      } else {
-        if (event && sp.rr == SFP_RR_PHOTON) {
-          ux *= 1 - delta;
-          uy *= 1 - delta;
-          uz *= 1 - delta;
-          ig = sfp_inv_gamma_lepton(ux, uy, uz);
-        }
-        lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
-        sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-        sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
-        if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
-        if (KIND == KIND_OD) sp.tau[i] = tau;
-        alive += 1;
+      if (event && sp.rr == SFP_RR_PHOTON) {
+        ux *= 1 - delta;
+        uy *= 1 - delta;
+        uz *= 1 - delta;
+        ig = sfp_inv_gamma_lepton(ux, uy, uz);
       }
+      lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+      sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+      sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
+      if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
+      if (KIND == KIND_OD) sp.tau[i] = tau;
+      alive += 1;
     }
   }
 
@@ -683,29 +703,268 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   block_sum(cnt, sh_w, tot);
   if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
   kernel_exit(scan, [&]() {
-    uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
-    if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
-    if (dA) {
-      dA->cnt->n = baseA + (int64_t)total;
-      dA->cnt->next_id = idA + (int64_t)total;
-      atomicAdd((unsigned long long*)&dA->cnt->psteps, (unsigned long long)total);
+    finish_appends(P, s, 1, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
+    scan->epoch = epoch;
+  });
+}
+
+// ---------------------------------------------------------- photon kernels --
+// Photons with QED advance in two kernels per step:
+//  k_awake  -- streams the 4-byte wake steps of the whole store and lists, in
+//              ascending index order, the photons to visit this step (wake
+//              step reached); photons inside a multi-step push are counted
+//              as live and skipped.
+//  k_photon -- one listed photon per thread: the envelope bound decides
+//              whether a pair is possible; if not, the photon is pushed one
+//              step, or -- every 8th step -- as many steps as the bound along
+//              its straight path allows (closed form, sfp_repeat_add) and put
+//              to sleep; if so, the exact Breit-Wheeler front runs, events
+//              are ranked in list order (= index order) and the pairs are
+//              appended already pushed; the converted photon becomes a hole
+//              and is never visited again.
+static constexpr uint32_t WAKE_NEVER = 0xffffffffu;
+
+// Wake steps per thread in k_awake: four 16-byte loads.  The wake array is
+// allocated in whole tiles (WAKE_PAD words), so full-tile loads stay inside it.
+static constexpr int AW = 16;
+static constexpr int64_t WAKE_PAD = (int64_t)TILE * AW;
+
+__global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, StepArgs a) {
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+  __shared__ int64_t sh_tile;
+  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint64_t sh_pref;
+  const int64_t n_old = sp.cnt->n_old;
+  ScanState* scan = P.scan[s][0];
+  if ((a.rel & 7u) != 0) {
+    // Photons fall asleep and wake only on every 8th step: in between, the
+    // list of the previous step stays valid (holes it holds are skipped) and
+    // is extended by the photons appended since, whose indices are larger.
+    const int64_t from = sp.cnt->list_end, n_list = sp.cnt->n_hot;
+    for (int64_t j = from + (int64_t)blockIdx.x * TILE + threadIdx.x; j < n_old;
+         j += (int64_t)gridDim.x * TILE)
+      sp.hot[n_list + (j - from)] = (uint32_t)j;
+    kernel_exit(scan, [&]() {
+      Counts* c = sp.cnt;
+      c->n_hot = n_list + (n_old - from);
+      c->list_end = n_old;
+      c->psteps += c->asleep_n;
+      c->slept += c->asleep_n;
+    });
+    return;
+  }
+  const int64_t ntiles = (n_old + WAKE_PAD - 1) / WAKE_PAD;
+  const uint32_t epoch = next_epoch(scan);
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  uint32_t cnt[2] = {0, 0};  // asleep (live), listed
+
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    // thread t owns words [i0, i0 + AW): list order = (thread, word) = index order
+    const int64_t i0 = tile * WAKE_PAD + (int64_t)threadIdx.x * AW;
+    const uint4* src = (const uint4*)(sp.wake + i0);
+    uint32_t w[AW];
+#pragma unroll
+    for (int q = 0; q < AW / 4; ++q) {
+      const uint4 v = src[q];
+      w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
     }
-    if (dB) {
-      dB->cnt->n = baseB + (int64_t)total;
-      dB->cnt->next_id = idB + (int64_t)total;
-      atomicAdd((unsigned long long*)&dB->cnt->psteps, (unsigned long long)total);
+    uint32_t up = 0;
+#pragma unroll
+    for (int j = 0; j < AW; ++j) {
+      const bool in = i0 + j < n_old;
+      up |= (uint32_t)(in && w[j] <= a.rel) << j;
+      cnt[0] += (uint32_t)(in && w[j] > a.rel && w[j] != WAKE_NEVER);
     }
+    const uint32_t c = (uint32_t)__popc(up);
+    cnt[1] += c;
+    // exclusive prefix of c over the workgroup, then over the tiles
+    uint32_t incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    if (lane == 63) sh_w[wv] = incl;
+    __syncthreads();
+    uint32_t woff = 0, agg = 0;
+    for (int k = 0; k < WAVES; ++k) {
+      const uint32_t t = sh_w[k];
+      if (k < wv) woff += t;
+      agg += t;
+    }
+    if (wv == 0) {
+      uint64_t pre = lookback(P.flags[s][0], tile, agg, epoch);
+      if (lane == 0) sh_pref = pre;
+    }
+    __syncthreads();
+    uint64_t r = sh_pref + woff + incl - c;
+#pragma unroll
+    for (int j = 0; j < AW; ++j)
+      if ((up >> j) & 1u) sp.hot[r++] = (uint32_t)(i0 + j);
+    __syncthreads();  // sh_w and sh_pref are rewritten by the next tile
+  }
+
+  uint64_t tot[2];
+  block_sum(cnt, sh_w, tot);
+  if (threadIdx.x == 0) {
+    count_add(&sp.cnt->psteps, tot[0]);
+    count_add(&sp.cnt->slept, tot[0]);
+    count_add(&sp.cnt->asleep_n, tot[0]);  // zeroed by this step's prologue
+  }
+  kernel_exit(scan, [&]() {
+    uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+    sp.cnt->n_hot = (int64_t)total;
+    sp.cnt->list_end = n_old;
+    scan->epoch = epoch;
+  });
+}
+
+template <int KIND, int FM>
+__global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, StepArgs a) {
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+
+  __shared__ sfp_rs_node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
+  __shared__ int64_t sh_tile;
+  __shared__ uint32_t sh_w[4 * WAVES];
+  __shared__ uint64_t sh_pref;
+
+  const int64_t n_list = sp.cnt->n_hot;
+  const SpeciesDev* dA = sp.pair_e_dst >= 0 ? &P.sp[sp.pair_e_dst] : nullptr;
+  const SpeciesDev* dB = sp.pair_p_dst >= 0 ? &P.sp[sp.pair_p_dst] : nullptr;
+  if (!room_or_halt(P, s, a, dA, dB, n_list)) return;
+  const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
+  const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
+  const int64_t ntiles = (n_list + TILE - 1) / TILE;
+  ScanState* scan = P.scan[s][1];
+  const uint32_t epoch = next_epoch(scan);
+  stage_tables<KIND, true>(P, lds_rs, lds_od);
+
+  uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+
+    const int64_t k = tile * TILE + threadIdx.x;
+    bool live = false;
+    int64_t i = 0;
+    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
+    if (k < n_list) {
+      i = (int64_t)sp.hot[k];
+      // a list kept from an earlier step holds photons that have fallen
+      // asleep since (counted by the list kernel) and holes: skipped
+      if (sp.wake[i] <= a.rel) {
+        const uint8_t pr = sp.pruned[i];
+        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+        live = pr == 0;
+        if (!live) sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
+      }
+    }
+    const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
+    bool hot = false;
+    if (live && sp.push) {
+      float S = sfp_field_bound_f32<FM>(&P.fields, x, y, z, a.t);
+      float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+      hot = qed_possible(P, true, chi_b, ig, a.dt);
+    }
+    bool event = false;
+    double delta = 0.0;
+    if (hot) {
+      if (KIND == KIND_OD) tau = sp.tau[i];
+      event = qed_event<true, KIND, FM>(P, sp, a, lds_rs, lds_od, x, y, z, ux, uy, uz, ig,
+                                        sp.id[i], tau, delta);
+    }
+    if (!dA && !dB) event = false;
+
+    int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
+    if (event) {
+      double dp = 1.0 - delta;
+      if (dA)
+        spawn_lepton<FM>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+                         delta * uy, delta * uz);
+      if (dB)
+        spawn_lepton<FM>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
+                         dp * uy, dp * uz);
+    }
+
+    if (live) {
+      cnt[1] += 1;
+      cnt[2] += hot ? 1u : 0u;
+      if (event) {
+        // the converted photon becomes a hole (cpu.py:189-190), not pushed
+        sp.pruned[i] = 1;
+        sp.wake[i] = WAKE_NEVER;
+      } else {
+        uint32_t nst = 1;
+        if (!sp.push) {
+          nst = a.rel_end - a.rel;
+        } else if (!hot && (a.rel & 7u) == 0) {
+          nst = sleep_steps<FM>(P, a, true, x, y, z, ux, uy, uz, ig);
+        }
+        if (sp.push) {
+          // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
+          // increment each time
+          const double hx = a.cdt_half * ig * ux, hy = a.cdt_half * ig * uy,
+                       hz = a.cdt_half * ig * uz;
+          if (nst == 1) {
+            x += hx; x += hx;
+            y += hy; y += hy;
+            z += hz; z += hz;
+          } else {  // the same 2 nst additions, in closed form
+            x = sfp_repeat_add(x, hx, 2 * nst);
+            y = sfp_repeat_add(y, hy, 2 * nst);
+            z = sfp_repeat_add(z, hz, 2 * nst);
+          }
+          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+        }
+        if (hot && KIND == KIND_OD) sp.tau[i] = tau;
+        if (nst > 1) {
+          sp.wake[i] = a.rel + nst;
+          cnt[3] += 1;
+        }
+        cnt[0] += 1;
+      }
+    }
+  }
+
+  uint64_t tot[4];
+  block_sum(cnt, sh_w, tot);
+  if (threadIdx.x == 0) {
+    count_add(&sp.cnt->psteps, tot[0]);
+    count_add(&sp.cnt->processed, tot[1]);
+    count_add(&sp.cnt->hot_total, tot[2]);
+    count_add(&sp.cnt->sleeps, tot[3]);
+    count_add(&sp.cnt->asleep_n, tot[3]);
+  }
+  kernel_exit(scan, [&]() {
+    finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
     scan->epoch = epoch;
   });
 }
 
 // prologue of a step: snapshot the counts the species kernels iterate over
-__global__ void k_step_prologue(const Plan* plan) {
+// (and, on the steps where photons fall asleep and wake, restart the count of
+// sleeping photons that the list kernel rebuilds)
+__global__ void k_step_prologue(const Plan* plan, StepArgs a) {
   if (plan->halt->flag) return;
   int s = threadIdx.x;
   if (s < plan->nspecies) {
     Counts* c = plan->sp[s].cnt;
     c->n_old = c->n;
+    if ((a.rel & 7u) == 0) c->asleep_n = 0;
   }
 }
 
@@ -714,14 +973,14 @@ typedef void (*kfn)(const Plan*, int, StepArgs);
 
 template <int FM>
 static kfn sweep_fn(bool photon, bool qed) {
-  if (photon) return qed ? k_sweep<true, true, FM> : k_sweep<true, false, FM>;
+  if (photon) return qed ? k_awake : k_sweep<true, false, FM>;
   return qed ? k_sweep<false, true, FM> : k_sweep<false, false, FM>;
 }
 
 template <int FM>
 static kfn hot_fn(bool photon, int kind) {
-  if (photon) return kind == KIND_OD ? k_hot<true, KIND_OD, FM> : k_hot<true, KIND_RS, FM>;
-  return kind == KIND_OD ? k_hot<false, KIND_OD, FM> : k_hot<false, KIND_RS, FM>;
+  if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM> : k_photon<KIND_RS, FM>;
+  return kind == KIND_OD ? k_hot<KIND_OD, FM> : k_hot<KIND_RS, FM>;
 }
 
 static kfn pick_sweep(int fm, bool photon, bool qed) {
@@ -813,15 +1072,24 @@ static int sim_refresh_plan(Sim* sim) {
       P.scan[s][ph] = sim->scan_mem + 2 * s + ph;
     }
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
-      if (sim->hot_mem[s]) HIPCHK(hipFree(sim->hot_mem[s]));
+      // photons keep their visit list from step to step: grown inside a run,
+      // the list keeps its entries
       int64_t want = std::max<int64_t>(st->cap, 1);
-      HIPCHK(hipMalloc(&sim->hot_mem[s], sizeof(uint32_t) * want));
+      uint32_t* h = nullptr;
+      HIPCHK(hipMalloc(&h, sizeof(uint32_t) * want));
+      if (sim->hot_mem[s]) {
+        HIPCHK(hipMemcpy(h, sim->hot_mem[s], sizeof(uint32_t) * sim->hot_cap[s],
+                         hipMemcpyDeviceToDevice));
+        HIPCHK(hipFree(sim->hot_mem[s]));
+      }
+      sim->hot_mem[s] = h;
       sim->hot_cap[s] = want;
     }
     d.hot = sim->hot_mem[s];
     if (d.photon && st->cap > sim->wake_cap[s]) {
-      // grown inside a run: sleeping photons keep their wake steps
-      int64_t want = std::max<int64_t>(st->cap, 1);
+      // grown inside a run: sleeping photons keep their wake steps; whole
+      // tiles of k_awake, so that its vector loads stay inside the array
+      int64_t want = (st->cap + WAKE_PAD - 1) / WAKE_PAD * WAKE_PAD;
       uint32_t* w = nullptr;
       HIPCHK(hipMalloc(&w, sizeof(uint32_t) * want));
       HIPCHK(hipMemset(w, 0, sizeof(uint32_t) * want));
@@ -1014,7 +1282,7 @@ static int launch(Sim* sim, kfn fn, int grid, int s, int phase, const StepArgs& 
 // Launch the kernels of one step from (species s0, phase ph0) on.
 static int launch_step(Sim* sim, const StepArgs& a, int s0, int ph0) {
   if (s0 == 0 && ph0 == 0)
-    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, 0, (const Plan*)sim->plan_dev);
+    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, 0, (const Plan*)sim->plan_dev, a);
   for (int s = s0; s < sim->nsp; ++s) {
     int first = (s == s0) ? ph0 : 0;
     if (first == 0) CHK(launch(sim, sim->sweep[s], sim->grid_sweep[s], s, 0, a));
