@@ -75,7 +75,7 @@ def make_case(n, nsteps, seed=0):
     return (x, y, z, ux, np.zeros(n), np.zeros(n)), [(2, (100.0, 2e-6, CTAU, 1, -d, LAMBDA, 0.0, 0.0))]
 
 
-def build_gpu(n, nsteps, reserve):
+def build_gpu(n, nsteps, reserve, precision="fp64"):
     import sfparticles_amd as sfa
     from sfparticles_amd import qed
     from sfparticles_amd.fields import Fields
@@ -88,7 +88,7 @@ def build_gpu(n, nsteps, reserve):
     p.set_photon(g)
     g.set_pair(e, p)
     sim = sfa.Simulation(e, p, g, dt=DT, fields=Fields(terms=field), print_every=None,
-                         photon_threshold=2.0, seed=0)
+                         photon_threshold=2.0, seed=0, precision=precision)
     for s in (e, p, g):
         s._ensure_device()
     # preallocated SoA capacity (the reference's _extend), so the timed steps
@@ -100,14 +100,36 @@ def build_gpu(n, nsteps, reserve):
     return sim, (e, p, g)
 
 
-def gpu_run(n, steps, warmup, nsteps_config):
+def photon_energy_sample(g, chunks=100, size=2000, seed=1):
+    """|u| of live photons in `chunks` slices of `size` at the same pseudo-random
+    offsets of the photon store (a sample of the spectrum without copying the
+    whole store)."""
+    import ctypes as C
+    from sfparticles_amd import _lib as L
+    n = g.N_buffered
+    rng = np.random.Generator(np.random.PCG64(seed))
+    offs = np.sort(rng.integers(0, max(n - size, 1), chunks))
+    out = []
+    for o in offs:
+        m = int(min(size, n - o))
+        u = []
+        for attr in (L.UX, L.UY, L.UZ, L.PRUNED):
+            a = np.empty(m, np.uint8 if attr == L.PRUNED else np.float64)
+            L.call("sfp_store_download", g._store, attr, a.ctypes.data_as(C.c_void_p), int(o), m)
+            u.append(a)
+        live = u[3] == 0
+        out.append(np.sqrt(u[0] ** 2 + u[1] ** 2 + u[2] ** 2)[live])
+    return np.concatenate(out)
+
+
+def gpu_run(n, steps, warmup, nsteps_config, precision="fp64"):
     from sfparticles_amd import _lib as L
     if warmup > 0:
-        wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0))
+        wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision)
         wsim.run_steps(warmup)
         del wsim
     reserve = (2 * n + 4096, n + 4096, 26 * n + 4096)
-    sim, species = build_gpu(n, nsteps_config, reserve)
+    sim, species = build_gpu(n, nsteps_config, reserve, precision)
     L.call("sfp_synchronize")
     t0 = time.perf_counter()
     done = 0
@@ -121,7 +143,10 @@ def gpu_run(n, steps, warmup, nsteps_config):
     t1 = time.perf_counter()
     stats = [sim.stats(i) for i in range(3)]
     ktime = [[sim.kernel_time(i, ph) for ph in (0, 1)] for i in range(3)]
-    return t1 - t0, stats, ktime, [s.N_buffered for s in species]
+    spectrum = photon_energy_sample(species[2])
+    counts = [s.N_buffered for s in species]
+    del sim, species
+    return t1 - t0, stats, ktime, counts, spectrum
 
 
 def cpu_baseline(n, nsteps):
@@ -185,6 +210,10 @@ def main():
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--path", choices=["fp64", "mixed", "both"], default="both",
+                    help="fp64: the reference's arithmetic; mixed: FP32 fields, push and QED "
+                         "with FP64 positions, time and chi; both: measure both, report mixed "
+                         "with its throughput ratio and yield/spectrum differences to fp64")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -197,31 +226,67 @@ def main():
 
     n = int(args.n)
     nsteps_config = 3000
-    elapsed, stats, ktime, counts = gpu_run(n, args.steps, args.warmup, nsteps_config)
-    psteps = sum(s["psteps"] for s in stats)
-    value = psteps / elapsed
-
-    # dominant kernel: largest accumulated time over (species, phase)
     names = ["electron", "positron", "photon"]
-    dom, ph = max(((i, p) for i in range(3) for p in (0, 1)), key=lambda ip: ktime[ip[0]][ip[1]][0])
-    # one launch of each kernel per timed step does the work; launches replayed
-    # after a capacity halt return at entry, so averages are taken per step
-    ms, _ = ktime[dom][ph]
-    alg_bytes = kernel_bytes(dom == 2, ph, stats[dom])
-    per_launch = alg_bytes / args.steps
-    avg_s = ms / 1e3 / args.steps
-    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>", (1, False): "k_hot<RS, gaussian field>",
-              (0, True): "k_awake", (1, True): "k_photon<RS, gaussian field>"}[(ph, dom == 2)]
-             + f" ({names[dom]})")
-    traffic = load_traffic(f"{'sweep' if ph == 0 else 'hot'}_{names[dom]}_rs")
-    total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "kernel": kname, "avg_launch_ms": round(ms / args.steps, 4),
-                "alg_bytes_per_launch": int(per_launch),
-                "kernel_time_share": round(ms / max(1e-9, total_ms), 3)}
+
+    def measure(precision):
+        elapsed, stats, ktime, counts, spectrum = gpu_run(n, args.steps, args.warmup,
+                                                          nsteps_config, precision)
+        psteps = sum(s["psteps"] for s in stats)
+        # dominant kernel: largest accumulated time over (species, phase); one
+        # launch of each kernel per timed step does the work (launches replayed
+        # after a capacity halt return at entry), so averages are per step
+        dom, ph = max(((i, p) for i in range(3) for p in (0, 1)),
+                      key=lambda ip: ktime[ip[0]][ip[1]][0])
+        ms, _ = ktime[dom][ph]
+        per_launch = kernel_bytes(dom == 2, ph, stats[dom]) / args.steps
+        avg_s = ms / 1e3 / args.steps
+        achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+        kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>",
+                  (1, False): "k_hot<RS, gaussian field>", (0, True): "k_awake",
+                  (1, True): "k_photon<RS, gaussian field>"}[(ph, dom == 2)]
+                 + f"{', mixed' if precision == 'mixed' else ''} ({names[dom]})")
+        kfile = ({(0, False): "k_sweep", (1, False): "k_hot", (0, True): "k_awake",
+                  (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_" + precision)
+        traffic = load_traffic(kfile)
+        total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                    "kernel": kname, "avg_launch_ms": round(ms / args.steps, 4),
+                    "alg_bytes_per_launch": int(per_launch),
+                    "kernel_time_share": round(ms / max(1e-9, total_ms), 3)}
+        detail = {"value": round(psteps / elapsed, 1),
+                  "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+                  "final_counts": dict(zip(names, counts)),
+                  "particle_steps": {names[i]: stats[i]["psteps"] for i in range(3)},
+                  "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
+                  "photon_steps_in_multistep_push": stats[2]["slept"],
+                  "photon_multistep_pushes": stats[2]["sleeps"],
+                  "kernel_ms": {f"{names[i]}_{'sweep' if p == 0 else 'hot'}": round(ktime[i][p][0], 2)
+                                for i in range(3) for p in (0, 1)},
+                  "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
+        return detail, roofline, counts, spectrum
+
+    paths = ["fp64", "mixed"] if args.path == "both" else [args.path]
+    res = {p: measure(p) for p in paths}
+    head = paths[-1]
+    detail, roofline, counts, spectrum = res[head]
+    config = {"workload": f"head-on collision, {n} e-, a0 100, gaussian pulse, RS tables, "
+                          f"{nsteps_config}-step geometry",
+              "path": {"fp64": "fp64-tables", "mixed": "mixed-fp32/fp64-tables"}[head],
+              "electrons": n, "steps_timed": args.steps}
+    config.update(detail)
+    if len(paths) == 2:
+        d64, _, c64, s64 = res["fp64"]
+        from scipy import stats as st
+        config["fp64"] = d64
+        config["speedup_vs_fp64"] = round(detail["value"] / d64["value"], 3)
+        config["error_vs_fp64"] = {
+            "photon_yield": round((counts[2] - c64[2]) / c64[2], 6),
+            "pair_yield": round((counts[1] - c64[1]) / max(c64[1], 1), 6),
+            "pair_yield_3sigma": round(3 * np.sqrt(max(c64[1], 1)) / max(c64[1], 1), 6),
+            "photon_spectrum_ks_p": float(st.ks_2samp(s64, spectrum).pvalue),
+            "photon_spectrum_sample": int(len(spectrum))}
 
     cpu = None
     if not args.no_cpu:
@@ -235,28 +300,18 @@ def main():
 
     line = {
         "metric": "particle-steps/s (1e7 e-, QED on): FP64 tables vs mixed+MLP at <=1% yield err",
-        "value": round(value, 1),
+        "value": detail["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": detail["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"head-on collision, {n} e-, a0 100, gaussian pulse, RS tables, "
-                               f"FP64, {nsteps_config}-step geometry",
-                   "path": "fp64-tables", "electrons": n, "steps_timed": args.steps,
-                   "final_counts": dict(zip(names, counts)),
-                   "particle_steps": {names[i]: stats[i]["psteps"] for i in range(3)},
-                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
-                   "photon_steps_in_multistep_push": stats[2]["slept"],
-                   "photon_multistep_pushes": stats[2]["sleeps"],
-                   "kernel_ms": {f"{names[i]}_{'sweep' if p == 0 else 'hot'}": round(ktime[i][p][0], 2)
-                                 for i in range(3) for p in (0, 1)},
-                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]},
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -281,12 +281,19 @@ typedef struct {
   int32_t pad[2];
 } sfp_species_desc;
 
+/* Arithmetic of the fused step.  FP64: the reference's precision, results
+ * bit-identical to the CPU restatement.  MIXED: fields, pushers and QED
+ * sampling in FP32, positions, time and chi in FP64 (no counterpart in the
+ * reference; its error against FP64 is measured per observable). */
+#define SFP_PRECISION_FP64 0
+#define SFP_PRECISION_MIXED 1
+
 typedef struct {
   double dt;
   double photon_threshold; /* simulation.py:19 */
   uint64_t seed;
   int32_t qed_model;       /* SFP_QED_* */
-  int32_t pad;
+  int32_t precision;       /* SFP_PRECISION_* */
 } sfp_sim_params;
 
 int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies, const sfp_field* terms,

@@ -65,7 +65,7 @@ class SpeciesDesc(C.Structure):
 
 class SimParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("photon_threshold", C.c_double), ("seed", C.c_uint64),
-                ("qed_model", C.c_int32), ("pad", C.c_int32)]
+                ("qed_model", C.c_int32), ("precision", C.c_int32)]
 
 
 P = C.c_void_p

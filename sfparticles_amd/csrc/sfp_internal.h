@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "sfp_device.h"
+#include "sfp_device32.h"
 
 // errors: set the thread's last-error message and return the code
 int sfp_set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -49,6 +50,7 @@ __device__ inline int wave_lane() { return (int)__lane_id(); }
 // QED tables in device memory
 struct Tables {
   sfp_rs_node* rs_nodes = nullptr;  // [z_N-1]
+  float4* rs_nodes32 = nullptr;     // [z_N-1] the same in FP32 (sfp_rs_node32)
   // decreasing envelopes of the table rates over brackets j.. (see sfp_tables_create)
   double2* rs_env = nullptr;        // [z_N-1] {E1, E2}
   int64_t z_N = 0;

@@ -630,6 +630,14 @@ extern "C" int sfp_tables_create(const sfp_rs_tables* rs, const sfp_od_tables* o
     HIPCHK(hipMalloc(&T->rs_nodes, sizeof(sfp_rs_node) * nodes.size()));
     HIPCHK(hipMemcpy(T->rs_nodes, nodes.data(), sizeof(sfp_rs_node) * nodes.size(),
                      hipMemcpyHostToDevice));
+    // the same nodes rounded to FP32 for the mixed-precision step
+    std::vector<float4> n32(nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i)
+      n32[i] = make_float4((float)nodes[i].ia, (float)nodes[i].ka, (float)nodes[i].ap,
+                           (float)nodes[i].kp);
+    HIPCHK(hipMalloc(&T->rs_nodes32, sizeof(float4) * n32.size()));
+    HIPCHK(hipMemcpy(T->rs_nodes32, n32.data(), sizeof(float4) * n32.size(),
+                     hipMemcpyHostToDevice));
     // Envelopes used to prove that an emission/pair rate is negligible.  On
     // bracket j (z in [z_j, z_j+1], linear interpolation of the tables) the
     // interpolated int Ai and |Ai'| are bounded by their node maxima, so for
@@ -708,6 +716,7 @@ extern "C" int sfp_tables_destroy(void* tables) {
   Tables* T = (Tables*)tables;
   if (!T) return SFP_OK;
   if (T->rs_nodes) (void)hipFree(T->rs_nodes);
+  if (T->rs_nodes32) (void)hipFree(T->rs_nodes32);
   if (T->rs_env) (void)hipFree(T->rs_env);
   if (T->od_mem) (void)hipFree(T->od_mem);
   if (T->od_photon_nodes) (void)hipFree(T->od_photon_nodes);

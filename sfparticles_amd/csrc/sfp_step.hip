@@ -70,6 +70,7 @@ struct Plan {
   uint64_t* flags[SFP_MAX_SPECIES][2];
   sfp_fieldset fields;
   const sfp_rs_node* rs_nodes;
+  const sfp_rs_node32* rs_nodes32;  // the same nodes in FP32 (mixed precision)
   const double2* rs_env;  // rate envelopes (Tables::rs_env)
   double rate_unit;       // alpha m c^2 / hbar
   int64_t z_N;
@@ -196,15 +197,52 @@ __device__ inline void lepton_push(const Plan& P, const SpeciesDev& sp, const St
   sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
 }
 
-// Append one new lepton (pair member) at slot d of species `sp`, pushed.
+// The same push in mixed precision: positions accumulate in FP64 from the
+// FP32 momentum, field and pusher run in FP32, chi for LL/CLL in FP64.
 template <int FM>
+__device__ inline void lepton_push32(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                     double& x, double& y, double& z, float& ux, float& uy,
+                                     float& uz, float& sx, float& sy, float& sz, float ig) {
+  double h = a.cdt_half * (double)ig;
+  x += h * (double)ux;
+  y += h * (double)uy;
+  z += h * (double)uz;
+  float F[6];
+  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t_half, F);
+  const float q = (float)sp.q, dt = (float)a.dt;
+  if (sp.spin) {
+    ig = sfp32_boris_tbmt(ux, uy, uz, sx, sy, sz, F, q, (float)sp.ae, dt);
+  } else {
+    ig = sfp32_boris(ux, uy, uz, F, q, dt);
+  }
+  if (sp.rr == SFP_RR_LL || (sp.rr == SFP_RR_CLL && !sp.spin)) {
+    const float chi = (float)sfp32_chi(F, ux, uy, uz);
+    float factor = (float)a.ll_factor;
+    if (sp.rr == SFP_RR_CLL) factor *= sfp32_cll_g(chi);
+    ig = sfp32_ll(ux, uy, uz, ig, chi, factor);
+  }
+  h = a.cdt_half * (double)ig;
+  x += h * (double)ux;
+  y += h * (double)uy;
+  z += h * (double)uz;
+}
+
+// Append one new lepton (pair member) at slot d of species `sp`, pushed.
+template <int FM, bool MIX = false>
 __device__ inline void spawn_lepton(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
                                     int64_t d, uint64_t id, double x, double y, double z,
                                     double ux, double uy, double uz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (sp.push) {
-    double ig = 1.0 / sfp_sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-    lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+    if (MIX) {
+      float fx = (float)ux, fy = (float)uy, fz = (float)uz, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+      lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2,
+                        __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz));
+      ux = fx; uy = fy; uz = fz;
+    } else {
+      double ig = 1.0 / sfp_sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+      lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+    }
   }
   sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
   sp.ux[d] = ux; sp.uy[d] = uy; sp.uz[d] = uz;
@@ -354,7 +392,7 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const StepArgs& a, bool qe
 // position is current when the run returns.
 static constexpr int ROWS = 32;
 
-template <bool PHOTON, bool QED, int FM>
+template <bool PHOTON, bool QED, int FM, bool MIX>
 __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, StepArgs a) {
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -422,7 +460,12 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       }
       double ig = 1.0;
       if (active) {
-        ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
+        if (MIX && !PHOTON) {  // mixed precision: the momentum is an FP32 quantity
+          const float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+          ig = (double)__frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+        } else {
+          ig = PHOTON ? sfp_inv_gamma_photon(ux, uy, uz) : sfp_inv_gamma_lepton(ux, uy, uz);
+        }
         proc += 1;
       }
       bool hot = false;
@@ -469,7 +512,15 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
           }
         } else if (sp.push) {
           if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
-          lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+          if (MIX) {
+            float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+            float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+            lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, (float)ig);
+            ux = fx; uy = fy; uz = fz;
+            sx = s0; sy = s1; sz = s2;
+          } else {
+            lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+          }
           sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
           sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
           if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
@@ -589,11 +640,98 @@ __device__ inline bool qed_event(const Plan& P, const SpeciesDev& sp, const Step
   return event;
 }
 
-template <int KIND, bool PHOTON>
-__device__ inline void stage_tables(const Plan& P, sfp_rs_node* lds_rs, double2* lds_od) {
+// One optical-depth update in mixed precision (optical_depth.py:42-62): the
+// total rate and the draws in FP32, tau accumulated in FP64; the photon energy
+// of an event from the FP64 inverse CDF (events are rare).
+__device__ inline bool od_update32(double* tau, float ig, double chi, float dt, uint64_t seed,
+                                   uint64_t id, uint64_t step, uint32_t tag, const double2* nodes,
+                                   const double* cdf, const sfp_od_dev& od, double* delta) {
+  float lc = __log10f((float)chi);
+  const float lmax = (float)od.log_chi_max, lmin = (float)od.log_chi_min;
+  if (lc > lmax) lc = lmax;
+  float rate = 0.0f;
+  if (lc >= lmin) {
+    const float ld = (float)od.log_chi_delta;
+    int idx = (int)((lc - lmin) / ld);
+    if (idx > (int)od.chi_N - 2) idx = (int)od.chi_N - 2;
+    const double2 nd = nodes[idx];
+    rate = (float)nd.x + (float)nd.y * (lc - (lmin + (float)idx * ld));
+  }
+  double t = *tau;
+  if (t == 0.0) {
+    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, 1u);
+    t = (double)(-__logf(1.0f - sfp32_u01(d.v[0])));
+  }
+  t -= (double)(rate * (dt * ig));
+  bool ev = false;
+  if (t < 0) {
+    sfp_u32x4 d = sfp_draw4(seed, id, step, tag, 2u);
+    t = (double)(-__logf(1.0f - sfp32_u01(d.v[0])));
+    ev = true;
+    *delta = sfp_od_inverse_cdf(cdf, &od, chi, sfp_u01(d.v[2], d.v[3]));
+  } else {
+    *delta = 0.0;
+  }
+  *tau = t;
+  return ev;
+}
+
+// The QED decision in mixed precision: field in FP32, chi in FP64, the
+// rejection-sampling rate from FP32 tables with 24-bit uniforms (the leading
+// bits of the same Philox words as the FP64 step), optical depth as above.
+template <bool PHOTON, int KIND, int FM>
+__device__ inline bool qed_event32(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                   const sfp_rs_node32* lds_rs, const double2* lds_od, double x,
+                                   double y, double z, float ux, float uy, float uz, float ig,
+                                   uint64_t pid, double& tau, double& delta) {
+  bool event = false;
+  delta = 0.0;
+  float F[6];
+  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  const double chi = PHOTON ? sfp32_chi_photon(F, ux, uy, uz) : sfp32_chi(F, ux, uy, uz);
+  if (KIND == KIND_RS) {
+    if (chi != 0.0) {
+      sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+      const float r1 = sfp32_u01(d.v[0]), r2 = sfp32_u01(d.v[2]);
+      const float dtau = (float)a.dt * ig;
+      const float chif = (float)chi;
+      const float zmin = (float)P.z_min, zmax = (float)P.z_max, zd = (float)P.z_delta;
+      const float izd = (float)(1.0 / P.z_delta);
+      if (!PHOTON) {
+        const float r1_3 = r1 * r1 * r1;
+        const float p = 3.0f * (r1 * r1) *
+                        sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1_3, true) *
+                        dtau;
+        if (r2 < p) { event = true; delta = r1_3; }
+      } else {
+        const float p =
+            sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1, false) * dtau;
+        if (r2 < p) { event = true; delta = r1; }
+      }
+    }
+  } else {
+    if (chi >= P.od.chi_min) {
+      event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag,
+                          lds_od, PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
+    }
+  }
+  if (!PHOTON && event) {
+    if ((float)delta / ig < (float)P.threshold) event = false;
+  }
+  return event;
+}
+
+template <int KIND, bool PHOTON, bool MIX = false>
+__device__ inline void stage_tables(const Plan& P, void* lds_rs, double2* lds_od) {
   if (KIND == KIND_RS) {
     int64_t nn = P.z_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) lds_rs[k] = P.rs_nodes[k];
+    if (MIX) {
+      sfp_rs_node32* dst = (sfp_rs_node32*)lds_rs;
+      for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) dst[k] = P.rs_nodes32[k];
+    } else {
+      sfp_rs_node* dst = (sfp_rs_node*)lds_rs;
+      for (int64_t k = threadIdx.x; k < nn && k < 1023; k += TILE) dst[k] = P.rs_nodes[k];
+    }
   } else {
     const double2* src = PHOTON ? P.od_pair_nodes : P.od_photon_nodes;
     int64_t nn = P.od.chi_N - 1;
@@ -626,13 +764,14 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // (wave64 ballot + workgroup scan + decoupled look-back), the new photon
 // appended already pushed (the reference pushes particles in the step that
 // creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
-template <int KIND, int FM>
+template <int KIND, int FM, bool MIX>
 __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, StepArgs a) {
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
+  using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
 
-  __shared__ sfp_rs_node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
   __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[2 * WAVES];
@@ -646,7 +785,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   const int64_t ntiles = (n_hot + TILE - 1) / TILE;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
-  stage_tables<KIND, false>(P, lds_rs, lds_od);
+  stage_tables<KIND, false, MIX>(P, lds_rs, lds_od);
 
   uint32_t alive = 0;
   while (true) {
@@ -669,27 +808,63 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       pid = sp.id[i];
     }
-    double ig = valid ? sfp_inv_gamma_lepton(ux, uy, uz) : 1.0;
+    float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+    float igf = 1.0f;
+    double ig = 1.0;
+    if (valid) {
+      if (MIX) {
+        igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+      } else {
+        ig = sfp_inv_gamma_lepton(ux, uy, uz);
+      }
+    }
     bool event = false;
     double delta = 0.0;
-    if (valid)
-      event = qed_event<false, KIND, FM>(P, sp, a, lds_rs, lds_od, x, y, z, ux, uy, uz, ig, pid,
-                                         tau, delta);
+    if (valid) {
+      if (MIX) {
+        event = qed_event32<false, KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+                                             z, fx, fy, fz, igf, pid, tau, delta);
+      } else {
+        event = qed_event<false, KIND, FM>(P, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+                                           ux, uy, uz, ig, pid, tau, delta);
+      }
+    }
     if (!dA) event = false;
 
     int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
-    if (event)
-      spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux, delta * uy,
-                   delta * uz);
+    if (event) {
+      if (MIX) {
+        const float df = (float)delta;
+        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
+                     df * fz);
+      } else {
+        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+                     delta * uy, delta * uz);
+      }
+    }
 
     if (valid) {
-      if (event && sp.rr == SFP_RR_PHOTON) {
-        ux *= 1 - delta;
-        uy *= 1 - delta;
-        uz *= 1 - delta;
-        ig = sfp_inv_gamma_lepton(ux, uy, uz);
+      if (MIX) {
+        if (event && sp.rr == SFP_RR_PHOTON) {
+          const float f = 1.0f - (float)delta;
+          fx *= f;
+          fy *= f;
+          fz *= f;
+          igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+        }
+        float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+        lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+        ux = fx; uy = fy; uz = fz;
+        sx = s0; sy = s1; sz = s2;
+      } else {
+        if (event && sp.rr == SFP_RR_PHOTON) {
+          ux *= 1 - delta;
+          uy *= 1 - delta;
+          uz *= 1 - delta;
+          ig = sfp_inv_gamma_lepton(ux, uy, uz);
+        }
+        lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
       }
-      lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
       sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
       sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
       if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
@@ -826,13 +1001,14 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   });
 }
 
-template <int KIND, int FM>
+template <int KIND, int FM, bool MIX>
 __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, StepArgs a) {
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
+  using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
 
-  __shared__ sfp_rs_node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
   __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
@@ -847,7 +1023,7 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
   const int64_t ntiles = (n_list + TILE - 1) / TILE;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
-  stage_tables<KIND, true>(P, lds_rs, lds_od);
+  stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
   while (true) {
@@ -884,8 +1060,14 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     double delta = 0.0;
     if (hot) {
       if (KIND == KIND_OD) tau = sp.tau[i];
-      event = qed_event<true, KIND, FM>(P, sp, a, lds_rs, lds_od, x, y, z, ux, uy, uz, ig,
-                                        sp.id[i], tau, delta);
+      if (MIX) {
+        event = qed_event32<true, KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+                                            z, (float)ux, (float)uy, (float)uz, (float)ig,
+                                            sp.id[i], tau, delta);
+      } else {
+        event = qed_event<true, KIND, FM>(P, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+                                          ux, uy, uz, ig, sp.id[i], tau, delta);
+      }
     }
     if (!dA && !dB) event = false;
 
@@ -893,11 +1075,11 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     if (event) {
       double dp = 1.0 - delta;
       if (dA)
-        spawn_lepton<FM>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
-                         delta * uy, delta * uz);
+        spawn_lepton<FM, MIX>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
+                              delta * ux, delta * uy, delta * uz);
       if (dB)
-        spawn_lepton<FM>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
-                         dp * uy, dp * uz);
+        spawn_lepton<FM, MIX>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
+                              dp * uy, dp * uz);
     }
 
     if (live) {
@@ -971,28 +1153,38 @@ __global__ void k_step_prologue(const Plan* plan, StepArgs a) {
 // ------------------------------------------------------------- dispatch ----
 typedef void (*kfn)(const Plan*, int, StepArgs);
 
-template <int FM>
+template <int FM, bool MIX>
 static kfn sweep_fn(bool photon, bool qed) {
-  if (photon) return qed ? k_awake : k_sweep<true, false, FM>;
-  return qed ? k_sweep<false, true, FM> : k_sweep<false, false, FM>;
+  if (photon) return qed ? k_awake : k_sweep<true, false, FM, false>;
+  return qed ? k_sweep<false, true, FM, MIX> : k_sweep<false, false, FM, MIX>;
 }
 
-template <int FM>
+template <int FM, bool MIX>
 static kfn hot_fn(bool photon, int kind) {
-  if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM> : k_photon<KIND_RS, FM>;
-  return kind == KIND_OD ? k_hot<KIND_OD, FM> : k_hot<KIND_RS, FM>;
+  if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM, MIX> : k_photon<KIND_RS, FM, MIX>;
+  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX> : k_hot<KIND_RS, FM, MIX>;
 }
 
-static kfn pick_sweep(int fm, bool photon, bool qed) {
-  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN>(photon, qed);
-  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE>(photon, qed);
-  return sweep_fn<SFP_FM_ANY>(photon, qed);
+template <bool MIX>
+static kfn pick_sweep_p(int fm, bool photon, bool qed) {
+  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN, MIX>(photon, qed);
+  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE, MIX>(photon, qed);
+  return sweep_fn<SFP_FM_ANY, MIX>(photon, qed);
 }
 
-static kfn pick_hot(int fm, bool photon, int kind) {
-  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN>(photon, kind);
-  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE>(photon, kind);
-  return hot_fn<SFP_FM_ANY>(photon, kind);
+template <bool MIX>
+static kfn pick_hot_p(int fm, bool photon, int kind) {
+  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind);
+  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind);
+  return hot_fn<SFP_FM_ANY, MIX>(photon, kind);
+}
+
+static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed) {
+  return mixed ? pick_sweep_p<true>(fm, photon, qed) : pick_sweep_p<false>(fm, photon, qed);
+}
+
+static kfn pick_hot(int fm, bool photon, int kind, bool mixed) {
+  return mixed ? pick_hot_p<true>(fm, photon, kind) : pick_hot_p<false>(fm, photon, kind);
 }
 
 // ------------------------------------------------------------- host side ---
@@ -1114,6 +1306,8 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
     return set_err(SFP_E_ARG, "between 1 and %d species are supported", SFP_MAX_SPECIES);
   if (params->qed_model != SFP_QED_RS && params->qed_model != SFP_QED_OD)
     return set_err(SFP_E_ARG, "unknown QED model %d", params->qed_model);
+  if (params->precision != SFP_PRECISION_FP64 && params->precision != SFP_PRECISION_MIXED)
+    return set_err(SFP_E_ARG, "unknown precision %d", params->precision);
   Sim* sim = new Sim();
   sim->nsp = nspecies;
   sim->params = *params;
@@ -1168,6 +1362,7 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
     if (sim->tables->z_N - 1 > 1023)
       return set_err(SFP_E_ARG, "rejection-sampling table larger than 1024 nodes");
     P.rs_nodes = sim->tables->rs_nodes;
+    P.rs_nodes32 = (const sfp_rs_node32*)sim->tables->rs_nodes32;
     P.rs_env = sim->tables->rs_env;
     P.rate_unit = SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR;
     P.z_N = sim->tables->z_N;
@@ -1208,9 +1403,10 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   int cus = prop.multiProcessorCount;
   for (int s = 0; s < nspecies; ++s) {
     bool photon = P.sp[s].photon;
-    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s]);
+    const bool mixed = params->precision == SFP_PRECISION_MIXED;
+    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed);
     sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, params->qed_model == SFP_QED_OD
-                                                                ? KIND_OD : KIND_RS)
+                                                                ? KIND_OD : KIND_RS, mixed)
                               : nullptr;
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->sweep[s], TILE, 0));

@@ -28,7 +28,14 @@ class Simulation(object):
     """simulation.py:12-161"""
 
     def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
-                 t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None) -> None:
+                 t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None,
+                 precision: str = "fp64") -> None:
+        """precision: "fp64" (the reference's arithmetic, bit-identical to the CPU
+        restatement) or "mixed" (fields, pushers and QED sampling in FP32;
+        positions, time and chi in FP64)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        self.precision = precision
         self.all_particles = all_particles
         self.particles_push = []
         self.particles_bw = []
@@ -84,7 +91,8 @@ class Simulation(object):
             d.pair_e_dst = idx[id(p.bw_electron_ref())] if p.bw else -1
             d.pair_p_dst = idx[id(p.bw_positron_ref())] if p.bw else -1
         params = L.SimParams(float(self.dt), float(self.photon_threshold), int(self.seed),
-                             L.QED_OD if self.qed_model == "od" else L.QED_RS, 0)
+                             L.QED_OD if self.qed_model == "od" else L.QED_RS,
+                             1 if self.precision == "mixed" else 0)
         arr = terms_to_c(terms)
         h = C.c_void_p()
         L.call("sfp_sim_create", descs, n, arr, len(terms), qed.device_tables(), C.byref(params),
