@@ -75,7 +75,7 @@ def make_case(n, nsteps, seed=0):
     return (x, y, z, ux, np.zeros(n), np.zeros(n)), [(2, (100.0, 2e-6, CTAU, 1, -d, LAMBDA, 0.0, 0.0))]
 
 
-def build_gpu(n, nsteps, reserve, precision="fp64"):
+def build_gpu(n, nsteps, reserve, path="fp64"):
     import sfparticles_amd as sfa
     from sfparticles_amd import qed
     from sfparticles_amd.fields import Fields
@@ -88,7 +88,9 @@ def build_gpu(n, nsteps, reserve, precision="fp64"):
     p.set_photon(g)
     g.set_pair(e, p)
     sim = sfa.Simulation(e, p, g, dt=DT, fields=Fields(terms=field), print_every=None,
-                         photon_threshold=2.0, seed=0, precision=precision)
+                         photon_threshold=2.0, seed=0,
+                         precision="fp64" if path == "fp64" else "mixed",
+                         sampler="mlp" if path == "mixed+mlp" else None)
     for s in (e, p, g):
         s._ensure_device()
     # preallocated SoA capacity (the reference's _extend), so the timed steps
@@ -128,7 +130,7 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64"):
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision)
         wsim.run_steps(warmup)
         del wsim
-    reserve = (2 * n + 4096, n + 4096, 26 * n + 4096)
+    reserve = (2 * n + 4096, n + 4096, 28 * n + 4096)
     sim, species = build_gpu(n, nsteps_config, reserve, precision)
     L.call("sfp_synchronize")
     t0 = time.perf_counter()
@@ -210,10 +212,11 @@ def main():
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--path", choices=["fp64", "mixed", "both"], default="both",
+    ap.add_argument("--path", choices=["fp64", "mixed", "mixed+mlp", "all"], default="all",
                     help="fp64: the reference's arithmetic; mixed: FP32 fields, push and QED "
-                         "with FP64 positions, time and chi; both: measure both, report mixed "
-                         "with its throughput ratio and yield/spectrum differences to fp64")
+                         "with FP64 positions, time and chi; mixed+mlp: mixed with the learned "
+                         "photon-energy sampler; all: the three, reporting mixed+mlp with its "
+                         "throughput ratio and yield/spectrum differences to fp64")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -244,9 +247,10 @@ def main():
         kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>",
                   (1, False): "k_hot<RS, gaussian field>", (0, True): "k_awake",
                   (1, True): "k_photon<RS, gaussian field>"}[(ph, dom == 2)]
-                 + f"{', mixed' if precision == 'mixed' else ''} ({names[dom]})")
+                 + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
         kfile = ({(0, False): "k_sweep", (1, False): "k_hot", (0, True): "k_awake",
-                  (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_" + precision)
+                  (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_"
+                 + precision.replace("+", "_"))
         traffic = load_traffic(kfile)
         total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -267,24 +271,28 @@ def main():
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
         return detail, roofline, counts, spectrum
 
-    paths = ["fp64", "mixed"] if args.path == "both" else [args.path]
+    paths = ["fp64", "mixed", "mixed+mlp"] if args.path == "all" else [args.path]
     res = {p: measure(p) for p in paths}
     head = paths[-1]
     detail, roofline, counts, spectrum = res[head]
     config = {"workload": f"head-on collision, {n} e-, a0 100, gaussian pulse, RS tables, "
                           f"{nsteps_config}-step geometry",
-              "path": {"fp64": "fp64-tables", "mixed": "mixed-fp32/fp64-tables"}[head],
+              "path": {"fp64": "fp64-tables", "mixed": "mixed-fp32/fp64-tables",
+                       "mixed+mlp": "mixed-fp32/fp64 + bf16-mfma photon-energy network"}[head],
               "electrons": n, "steps_timed": args.steps}
     config.update(detail)
-    if len(paths) == 2:
+    if len(paths) > 1:
         d64, _, c64, s64 = res["fp64"]
         from scipy import stats as st
         config["fp64"] = d64
+        if "mixed" in res and head != "mixed":
+            config["mixed"] = res["mixed"][0]
         config["speedup_vs_fp64"] = round(detail["value"] / d64["value"], 3)
         config["error_vs_fp64"] = {
             "photon_yield": round((counts[2] - c64[2]) / c64[2], 6),
             "pair_yield": round((counts[1] - c64[1]) / max(c64[1], 1), 6),
             "pair_yield_3sigma": round(3 * np.sqrt(max(c64[1], 1)) / max(c64[1], 1), 6),
+            "photon_spectrum_ks_distance": float(st.ks_2samp(s64, spectrum).statistic),
             "photon_spectrum_ks_p": float(st.ks_2samp(s64, spectrum).pvalue),
             "photon_spectrum_sample": int(len(spectrum))}
 
@@ -309,7 +317,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "mixed+mlp": "f32+f64+bf16"}[head],
         "data": "synthetic",
         "config": config,
         "roofline": roofline,

@@ -313,6 +313,12 @@ int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t fl
 int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
+/* Use the learned photon-energy sampler (sfp_mlp_*) for the photon emission
+ * of the leptons, or stop using it (mlp NULL).  Mixed precision only; needs
+ * the optical-depth tables.  The network must outlive its use by the
+ * simulation.  No counterpart in the reference. */
+int sfp_sim_set_sampler(void* sim, void* mlp);
+
 /* Counters of one species since the last reset (synchronous), stats[10]:
  * [0] particle-steps (live particles after each step, the reference's
  *     ns/particle denominator, simulation.py:150-151)
@@ -324,6 +330,47 @@ int sfp_sim_reset_timing(void* sim);
  *     wake step for these)                 [9] multi-step pushes started */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
+
+/* ---------------------------------------------------------------------------
+ * 4. Learned photon-energy sampler (no counterpart in the reference)
+ *
+ * An MLP (log10 chi, r) -> log delta, 2 -> 64 -> 64 -> 64 -> 1 with SiLU, BF16
+ * matrix-core products with FP32 accumulation, trained on the GPU on samples
+ * of the optical-depth photon-emission inverse CDF (the table of
+ * qed/optical_depth_tables.py:59-101, which it replaces in the mixed-precision
+ * step and which stays its fallback per bin of log10 chi).
+ * Parameters: SFP_MLP_NPARAM floats, W1[64][2] b1 W2[64][64] b2 W3[64][64] b3
+ * W4[64] b4; norm[4] = {lc_min, lc_max, mu, sd}: inputs (log10 chi - centre) /
+ * half-width and 2 r - 1, output (log delta - mu) / sd.
+ * ------------------------------------------------------------------------- */
+#define SFP_MLP_NPARAM 8577
+int sfp_mlp_create(void** mlp);
+int sfp_mlp_destroy(void* mlp);
+int sfp_mlp_set_params(void* mlp, const float* params, const float* norm);
+int sfp_mlp_get_params(void* mlp, float* params, float* norm);
+/* bit i set: bin i of 64 equal bins of log10 chi over [lc_min, lc_max] uses the table */
+int sfp_mlp_set_fallback(void* mlp, uint64_t mask);
+/* n samples: x_dev [n][2] = (log10 chi ~ U(lc_min, lc_max), r ~ U(0,1)) from
+ * Philox(seed), y_dev [n] = log delta of the table's inverse CDF (synchronous) */
+int sfp_mlp_make_data(void* mlp, void* tables, int64_t n, uint64_t seed, float* x_dev,
+                      float* y_dev);
+/* `steps` Adam steps on batches of `batch` samples (a multiple of 32) taken in
+ * order from (x_dev, y_dev), cosine-decayed learning rate from lr; the mean
+ * squared error of each step goes to loss_curve_host[steps] (may be NULL) */
+int sfp_mlp_train(void* mlp, const float* x_dev, const float* y_dev, int64_t n, int32_t steps,
+                  int32_t batch, float lr, float* loss_curve_host);
+/* gradient of the mean squared error over n samples (n a multiple of 32)
+ * from the training kernels, to the host (synchronous) */
+int sfp_mlp_grad(void* mlp, const float* x_dev, const float* y_dev, int64_t n, float* grad_host,
+                 float* loss_host);
+/* normalised network output for raw inputs x_dev [n][2] */
+int sfp_mlp_forward(void* mlp, const float* x_dev, float* out_dev, int64_t n);
+/* delta for raw inputs x_dev [n][2] by the network (method 0) or the table
+ * inverse CDF (method 1); *ms = kernel time (synchronous) */
+int sfp_mlp_sample(void* mlp, void* tables, const float* x_dev, int64_t n, int32_t method,
+                   double* delta_dev, float* ms);
+/* device copy of the network and of its fallback bins, for the simulation */
+int sfp_mlp_device(void* mlp, const void** dev, const void** bins);
 
 /* dst[i] = src[i] for n doubles: a plain streaming copy kernel used to
  * calibrate HBM byte counters (no counterpart in the reference) */

@@ -107,6 +107,18 @@ SIGNATURES = {
     "sfp_update_inv_gamma": [P, P, P, P, I64, P, I32, P],
     "sfp_math_eval": [I32, P, P, P, I64, P],
     "sfp_field_bound_eval": [P, I32, P, P, P, D, P, P, P, D, I64, P, P],
+    "sfp_mlp_create": [C.POINTER(P)],
+    "sfp_mlp_destroy": [P],
+    "sfp_mlp_set_params": [P, P, P],
+    "sfp_mlp_get_params": [P, P, P],
+    "sfp_mlp_set_fallback": [P, C.c_uint64],
+    "sfp_mlp_make_data": [P, P, I64, C.c_uint64, P, P],
+    "sfp_mlp_train": [P, P, P, I64, I32, I32, C.c_float, P],
+    "sfp_mlp_grad": [P, P, P, I64, P, P],
+    "sfp_mlp_forward": [P, P, P, I64],
+    "sfp_mlp_sample": [P, P, P, I64, I32, P, P],
+    "sfp_mlp_device": [P, C.POINTER(P), C.POINTER(P)],
+    "sfp_sim_set_sampler": [P, P],
     "sfp_store_create": [I32, I64, C.POINTER(P)],
     "sfp_store_destroy": [P],
     "sfp_store_reserve": [P, I64],

@@ -695,4 +695,84 @@ SFP_HD double sfp_od_inverse_cdf(const double* table2d, const sfp_od_dev* T, dou
   return delta_left + k * (r - y1);
 }
 
+/* Row ends of the cumulative table at chi (the ymin, ymax of
+ * sfp_od_inverse_cdf): cumulative rate up to the first delta node and up to
+ * delta = 1, each linear in chi between grid points. */
+SFP_HD void sfp_od_row_ends(const double* table2d, const sfp_od_dev* T, double chi, double* ymin,
+                            double* ymax) {
+  int64_t DN = T->delta_N;
+  if (chi > T->chi_max) chi = T->chi_max;
+  double log_chi = sfp_log10(chi);
+  if (log_chi > T->log_chi_max) log_chi = T->log_chi_max;
+  double fidx = log_chi < T->log_chi_min ? 0.0 : (log_chi - T->log_chi_min) / T->log_chi_delta;
+  int64_t lo_i = (int64_t)sfp_floor(fidx);
+  if (lo_i > T->chi_N - 2) lo_i = T->chi_N - 2;
+  const double* rl = table2d + lo_i * DN;
+  const double* rh = rl + DN;
+  double dchi = T->chi_grid[lo_i + 1] - T->chi_grid[lo_i];
+  double w = chi - T->chi_grid[lo_i];
+  *ymin = rl[0] + (rh[0] - rl[0]) / dchi * w;
+  *ymax = rl[DN - 1] + (rh[DN - 1] - rl[DN - 1]) / dchi * w;
+}
+
+/* The inverse of the normalised cumulative rate measured from delta = 0:
+ * delta with C(delta) = r C(1), C the table row at chi (qed/optical_depth_
+ * tables.py:126-139 tabulates C from 0).  This is the exact photon-energy
+ * distribution of the emission rate, the one rejection sampling draws; the
+ * optical-depth model's inverse CDF instead maps its uniform onto
+ * [C(delta_0), C(1)] and so never draws below the first node delta_0.  Below
+ * delta_0 the small-delta law C ~ delta^(1/3) gives delta = delta_0
+ * (y / C(delta_0))^3, floored at 1e-3 delta_0; above, the bisection and
+ * interpolation of sfp_od_inverse_cdf.  Samplers floor the delta they return
+ * at SFP_DELTA_FLOOR: photons that soft are below the hard-photon cut
+ * (delta gamma >= threshold) of every lepton with gamma < 2e5 and are dropped
+ * either way. */
+#define SFP_DELTA_FLOOR 1e-5
+SFP_HD double sfp_od_inverse_cdf_from0(const double* table2d, const sfp_od_dev* T, double chi,
+                                       double r) {
+  int64_t DN = T->delta_N;
+  if (chi > T->chi_max) chi = T->chi_max;
+  double log_chi = sfp_log10(chi);
+  if (log_chi > T->log_chi_max) log_chi = T->log_chi_max;
+  double fidx = log_chi < T->log_chi_min ? 0.0 : (log_chi - T->log_chi_min) / T->log_chi_delta;
+  int64_t lo_i = (int64_t)sfp_floor(fidx);
+  if (lo_i > T->chi_N - 2) lo_i = T->chi_N - 2;
+  const double* rl = table2d + lo_i * DN;
+  const double* rh = rl + DN;
+  double dchi = T->chi_grid[lo_i + 1] - T->chi_grid[lo_i];
+  double w = chi - T->chi_grid[lo_i];
+  double ymin = rl[0] + (rh[0] - rl[0]) / dchi * w;
+  double ymax = rl[DN - 1] + (rh[DN - 1] - rl[DN - 1]) / dchi * w;
+  double y = r * ymax;
+  if (y < ymin) {
+    double q = y / ymin;
+    double d = T->delta_grid[0] * q * q * q;
+    double f = 1e-3 * T->delta_grid[0];
+    return d > f ? d : f;
+  }
+  int64_t low = 0, high = DN - 1;
+  while (low <= high) {
+    int64_t mid = (low + high) / 2;
+    double mid_delta = rl[mid] + (rh[mid] - rl[mid]) / dchi * w;
+    if (mid_delta < y)
+      low = mid + 1;
+    else
+      high = mid - 1;
+  }
+  int64_t di = high;
+  if (di < 0) di = 0;
+  if (di > DN - 2) di = DN - 2;
+  double y1 = rl[di] + (rh[di] - rl[di]) / dchi * w;
+  double y2 = rl[di + 1] + (rh[di + 1] - rl[di + 1]) / dchi * w;
+  double dl = T->delta_grid[di], dr = T->delta_grid[di + 1];
+  return dl + (dr - dl) / (y2 - y1) * (y - y1);
+}
+
+/* the from-zero inverse CDF as a sampler returns it (floored) */
+SFP_HD double sfp_od_sample_from0(const double* table2d, const sfp_od_dev* T, double chi,
+                                  double r) {
+  double d = sfp_od_inverse_cdf_from0(table2d, T, chi, r);
+  return d > SFP_DELTA_FLOOR ? d : SFP_DELTA_FLOOR;
+}
+
 #endif /* SFP_DEVICE_H */

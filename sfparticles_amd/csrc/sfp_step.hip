@@ -29,6 +29,7 @@
 // host synchronisation are needed.
 
 #include "sfp_internal.h"
+#include "sfp_mlp.h"
 
 static constexpr int TILE = 256;
 static constexpr int WAVES = TILE / 64;
@@ -83,6 +84,8 @@ struct Plan {
   double zmax15;    // z_max^1.5: rate is 0 unless delta/(1-delta) <= zmax15 chi (photon)
   uint64_t seed;
   Halt* halt;
+  const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
+  const sfp_mlp_bins* net_bins;  // its per-bin table fallback
   int32_t nspecies;
   int32_t qed;
 };
@@ -721,6 +724,120 @@ __device__ inline bool qed_event32(const Plan& P, const SpeciesDev& sp, const St
   return event;
 }
 
+// The lepton QED decision of the mixed-precision step with the learned
+// sampler, which is trained on the inverse of the emission's cumulative rate
+// measured from delta = 0 (sfp_od_inverse_cdf_from0).  Optical depth: as
+// qed_event32, with delta of an emission from the network, its uniform mapped
+// onto that scale.  Rejection sampling: for chi inside the network's range
+// the emission is sampled by composition -- an event with probability
+// W(chi) dtau, W the total rate (qed/optical_depth_tables.py:46-56, the
+// integral of the rejection-sampling rate over delta), then delta from the
+// inverse CDF, here the network -- which draws the same (event, delta)
+// distribution as rejection sampling; below the range the rejection sampling
+// of qed_event32.
+// Where the network is used, `net` is set with its normalised inputs and
+// delta is filled in by the wave afterwards; bins flagged for fallback take
+// the table's inverse CDF.  The hard-photon cut is left to the caller.
+template <int KIND, int FM>
+__device__ inline bool qed_event32_net(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                       const sfp_rs_node32* lds_rs, const double2* lds_od,
+                                       double x, double y, double z, float ux, float uy, float uz,
+                                       float ig, uint64_t pid, double& tau, double& delta,
+                                       bool& net, float& in0, float& in1, double& chi_out,
+                                       double& u_table) {
+  bool event = false;
+  delta = 0.0;
+  net = false;
+  float F[6];
+  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  const double chi = sfp32_chi(F, ux, uy, uz);
+  chi_out = chi;
+  const sfp_mlp_bins& nb = *P.net_bins;
+  float u = 0.0f;     // uniform that selects delta, on the network's from-zero scale
+  double od_u = 0.0;  // optical depth: the uniform of its own inverse CDF
+  if (KIND == KIND_RS) {
+    if (chi == 0.0) return false;
+    sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+    const float r1 = sfp32_u01(d.v[0]), r2 = sfp32_u01(d.v[2]);
+    const float dtau = (float)a.dt * ig;
+    if (!(chi >= P.od.chi_min)) {
+      const float r1_3 = r1 * r1 * r1;
+      const float p = 3.0f * (r1 * r1) *
+                      sfp32_rs_rate(lds_rs, (int)P.z_N, (float)P.z_min, (float)P.z_max,
+                                    (float)P.z_delta, (float)(1.0 / P.z_delta), (float)chi, r1_3,
+                                    true) *
+                      dtau;
+      if (r2 < p) { event = true; delta = r1_3; }
+      return event;
+    }
+    float lc = __log10f((float)chi);
+    const float lmax = (float)P.od.log_chi_max, lmin = (float)P.od.log_chi_min;
+    if (lc > lmax) lc = lmax;
+    const float ld = (float)P.od.log_chi_delta;
+    int idx = (int)((lc - lmin) / ld);
+    if (idx > (int)P.od.chi_N - 2) idx = (int)P.od.chi_N - 2;
+    const double2 nd = lds_od[idx];
+    const float W = (float)nd.x + (float)nd.y * (lc - (lmin + (float)idx * ld));
+    if (!(r2 < W * dtau)) return false;
+    event = true;
+    u = r1;
+    u_table = (double)r1;
+  } else {
+    if (!(chi >= P.od.chi_min)) return false;
+    event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, lds_od,
+                        P.od.photon_cdf, P.od, &delta);
+    if (!event) return false;
+    sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 2u);
+    // the optical-depth inverse CDF maps its uniform onto [ymin, ymax] of the
+    // cumulative rate; on the network's from-zero scale that is
+    // (ymin + u (ymax - ymin)) / ymax
+    double ymin, ymax;
+    sfp_od_row_ends(P.od.photon_cdf, &P.od, chi, &ymin, &ymax);
+    const double ud = sfp_u01(d.v[2], d.v[3]);
+    u = (float)((ymin + ud * (ymax - ymin)) / ymax);
+    od_u = ud;
+    u_table = ud;
+  }
+  // delta: the network, or the table in a fallback bin / outside the range
+  const float lc = __log10f((float)chi);
+  const float fb = (lc - nb.lc_min) / (nb.lc_max - nb.lc_min) * (float)SFP_MLP_BINS;
+  const int bin = fb < 0.0f ? 0 : (fb >= (float)SFP_MLP_BINS ? SFP_MLP_BINS - 1 : (int)fb);
+  if (lc >= nb.lc_min && lc <= nb.lc_max && !((nb.fallback >> bin) & 1ull)) {
+    net = true;
+    in0 = sfp_mlp_x0(P.net, lc);
+    in1 = 2.0f * u - 1.0f;
+  } else if (KIND == KIND_RS) {
+    delta = sfp_od_sample_from0(P.od.photon_cdf, &P.od, chi, (double)u);
+  } else {
+    delta = sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, chi, od_u);
+  }
+  return event;
+}
+
+// delta = exp(network(in0, in1)) for the lanes with `net` set, sixteen at a
+// time on the matrix cores.  Called by all lanes of the wave.
+__device__ inline void net_delta_wave(const Plan& P, bool net, float in0, float in1,
+                                      double& delta) {
+  const sfp_mlp_dev* M = P.net;
+  const int lane = wave_lane(), c = lane & 15;
+  uint64_t m = __ballot(net);
+  const uint64_t below = m & ((1ull << lane) - 1ull);
+  int slot = (int)__popcll(below);
+  while (m) {
+    const int cnt = __popcll(m) < 16 ? (int)__popcll(m) : 16;
+    uint64_t mm = m;
+    const int cc = c < cnt ? c : 0;
+    for (int j = 0; j < cc; ++j) mm &= mm - 1ull;
+    const int src = (int)__builtin_ctzll(mm);
+    const float a0 = __shfl(in0, src), a1 = __shfl(in1, src);
+    const float o = sfp_mlp_eval16(M, M->W2p, M->W3p, a0, a1);
+    const float res = __shfl(o, slot >= 0 && slot < 16 ? slot : 0);
+    if (net && slot >= 0 && slot < 16) delta = sfp_mlp_delta(M, res);
+    for (int j = 0; j < cnt; ++j) m &= m - 1ull;
+    slot -= 16;
+  }
+}
+
 template <int KIND, bool PHOTON, bool MIX = false>
 __device__ inline void stage_tables(const Plan& P, void* lds_rs, double2* lds_od) {
   if (KIND == KIND_RS) {
@@ -764,7 +881,7 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // (wave64 ballot + workgroup scan + decoupled look-back), the new photon
 // appended already pushed (the reference pushes particles in the step that
 // creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
-template <int KIND, int FM, bool MIX>
+template <int KIND, int FM, bool MIX, bool NET>
 __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, StepArgs a) {
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -772,7 +889,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
 
   __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
-  __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
+  __shared__ double2 lds_od[(KIND == KIND_OD || NET) ? 255 : 1];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[2 * WAVES];
   __shared__ uint64_t sh_pref;
@@ -786,6 +903,11 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
   stage_tables<KIND, false, MIX>(P, lds_rs, lds_od);
+  if (NET && KIND == KIND_RS) {  // the total photon rate for composition sampling
+    const int64_t nn = P.od.chi_N - 1;
+    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
+    __syncthreads();
+  }
 
   uint32_t alive = 0;
   while (true) {
@@ -820,7 +942,21 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
     }
     bool event = false;
     double delta = 0.0;
-    if (valid) {
+    if (NET) {
+      bool net = false;
+      float in0 = 0.0f, in1 = 0.0f;
+      double chi_n = 0.0, u_tab = 0.0;
+      if (valid)
+        event = qed_event32_net<KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+                                          fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
+                                          u_tab);
+      net_delta_wave(P, net, in0, in1, delta);
+      if (net && delta > SFP_MLP_DELTA_CUT)  // the hard tail from the table, same uniform
+        delta = (KIND == KIND_RS) ? sfp_od_sample_from0(P.od.photon_cdf, &P.od, chi_n, u_tab)
+                                  : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, chi_n, u_tab);
+      // hard-photon cut (cpu.py:227-232)
+      if (event && (float)delta / igf < (float)P.threshold) event = false;
+    } else if (valid) {
       if (MIX) {
         event = qed_event32<false, KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
                                              z, fx, fy, fz, igf, pid, tau, delta);
@@ -1160,9 +1296,10 @@ static kfn sweep_fn(bool photon, bool qed) {
 }
 
 template <int FM, bool MIX>
-static kfn hot_fn(bool photon, int kind) {
+static kfn hot_fn(bool photon, int kind, bool net) {
   if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM, MIX> : k_photon<KIND_RS, FM, MIX>;
-  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX> : k_hot<KIND_RS, FM, MIX>;
+  if (MIX && net) return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, MIX> : k_hot<KIND_RS, FM, MIX, MIX>;
+  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, false> : k_hot<KIND_RS, FM, MIX, false>;
 }
 
 template <bool MIX>
@@ -1173,18 +1310,19 @@ static kfn pick_sweep_p(int fm, bool photon, bool qed) {
 }
 
 template <bool MIX>
-static kfn pick_hot_p(int fm, bool photon, int kind) {
-  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind);
-  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind);
-  return hot_fn<SFP_FM_ANY, MIX>(photon, kind);
+static kfn pick_hot_p(int fm, bool photon, int kind, bool net) {
+  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind, net);
+  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind, net);
+  return hot_fn<SFP_FM_ANY, MIX>(photon, kind, net);
 }
 
 static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed) {
   return mixed ? pick_sweep_p<true>(fm, photon, qed) : pick_sweep_p<false>(fm, photon, qed);
 }
 
-static kfn pick_hot(int fm, bool photon, int kind, bool mixed) {
-  return mixed ? pick_hot_p<true>(fm, photon, kind) : pick_hot_p<false>(fm, photon, kind);
+static kfn pick_hot(int fm, bool photon, int kind, bool mixed, bool net = false) {
+  return mixed ? pick_hot_p<true>(fm, photon, kind, net)
+               : pick_hot_p<false>(fm, photon, kind, false);
 }
 
 // ------------------------------------------------------------- host side ---
@@ -1376,6 +1514,9 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
       return set_err(SFP_E_ARG, "optical-depth tables required");
     if (sim->tables->od.chi_N - 1 > 255)
       return set_err(SFP_E_ARG, "optical-depth table larger than 256 chi nodes");
+  }
+  if (sim->tables && sim->tables->has_od && sim->tables->od.chi_N - 1 <= 255) {
+    // (also the learned sampler's total rate and fallback under rejection sampling)
     P.od = sim->tables->od;
     P.od_photon_nodes = sim->tables->od_photon_nodes;
     P.od_pair_nodes = sim->tables->od_pair_nodes;
@@ -1418,6 +1559,39 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   }
   CHK(sim_refresh_plan(sim));
   *out = sim;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  if (sim->params.precision != SFP_PRECISION_MIXED)
+    return set_err(SFP_E_ARG, "the learned sampler runs in the mixed-precision step");
+  if (!sim->tables || !sim->tables->has_od || !sim->plan.od_photon_nodes)
+    return set_err(SFP_E_ARG, "the learned sampler needs the optical-depth tables");
+  Plan& P = sim->plan;
+  if (mlp) {
+    const void *dev = nullptr, *bins = nullptr;
+    CHK(sfp_mlp_device(mlp, &dev, &bins));
+    P.net = (const sfp_mlp_dev*)dev;
+    P.net_bins = (const sfp_mlp_bins*)bins;
+  } else {
+    P.net = nullptr;
+    P.net_bins = nullptr;
+  }
+  const int kind = sim->params.qed_model == SFP_QED_OD ? KIND_OD : KIND_RS;
+  for (int s = 0; s < sim->nsp; ++s) {
+    if (!sim->qed[s] || P.sp[s].photon) continue;
+    sim->hot[s] = pick_hot(sim->fm, false, kind, true, mlp != nullptr);
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->hot[s], TILE, 0));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    sim->grid_hot[s] = std::max(per_cu, 1) * prop.multiProcessorCount;
+  }
+  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
   return SFP_OK;
 }
 

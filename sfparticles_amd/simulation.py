@@ -29,13 +29,23 @@ class Simulation(object):
 
     def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
                  t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None,
-                 precision: str = "fp64") -> None:
+                 precision: str = "fp64", sampler=None) -> None:
         """precision: "fp64" (the reference's arithmetic, bit-identical to the CPU
         restatement) or "mixed" (fields, pushers and QED sampling in FP32;
-        positions, time and chi in FP64)."""
+        positions, time and chi in FP64).  sampler (mixed only): a
+        qed.mlp.PhotonSampler, or "mlp" for the trained default, that draws the
+        energy of emitted photons in place of the inverse-CDF table."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        if sampler is not None and precision != "mixed":
+            raise ValueError("the learned sampler runs in the mixed-precision step")
+        if isinstance(sampler, str):
+            if sampler != "mlp":
+                raise ValueError(f"unknown sampler {sampler!r}")
+            from .qed.mlp import default_sampler
+            sampler = default_sampler()
         self.precision = precision
+        self.sampler = sampler
         self.all_particles = all_particles
         self.particles_push = []
         self.particles_bw = []
@@ -99,6 +109,8 @@ class Simulation(object):
                C.byref(h))
         self._sim = h.value
         self._sim_stores = stores
+        if self.sampler is not None:
+            L.call("sfp_sim_set_sampler", self._sim, self.sampler.handle)
 
     def _destroy(self):
         if self._sim is not None and L._lib is not None:

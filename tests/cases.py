@@ -70,7 +70,7 @@ def _species_list(case, make):
     return make(case)
 
 
-def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64"):
+def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None):
     """Run a case through sfparticles_amd.Simulation; returns per-species arrays."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
@@ -99,7 +99,8 @@ def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64"):
         if p is not None:
             g.set_pair(e, p)
     sim = sfa.Simulation(*sp, dt=case["dt"], fields=Fields(terms=case["field"]), print_every=None,
-                         photon_threshold=case["threshold"], seed=case["seed"], precision=precision)
+                         photon_threshold=case["threshold"], seed=case["seed"], precision=precision,
+                         sampler=sampler)
     steps = case["nsteps"] if nsteps is None else nsteps
     per = steps // chunks
     done = 0

@@ -11,8 +11,11 @@ the spin within 1e-3 of the FP64 oracle after 300 steps; QED on, photon and
 pair yields within 1% (or 3 sigma of the FP64 count, whichever is larger),
 the photon energy spectra indistinguishable by a two-sample KS test
 (p > 0.01) and the mean electron energy within 1e-3; cascades, mean counts
-over six seeds within three standard errors.
+over six seeds within three standard errors.  With the learned photon-energy
+sampler the spectrum bar is the spectrum error, the KS distance, within 1%.
 """
+import os
+
 import numpy as np
 import pytest
 from scipy import stats
@@ -71,6 +74,45 @@ def _compare_qed(case, what):
     assert abs(gm - g64) < 1e-3 * g64
     for r in rows:
         print(what, r)
+
+
+def _sampler():
+    """The shipped network if present, else one trained here on the GPU."""
+    from sfparticles_amd.qed import mlp
+    if os.path.exists(mlp.DEFAULT_PATH):
+        return mlp.default_sampler()
+    s = mlp.PhotonSampler(seed=0)
+    s.fit(n_data=2_000_000, steps=3000, batch=4096, lr=3e-3, seed=0)
+    s.calibrate_fallback(per_bin=20000)
+    return s
+
+
+def _compare_net(case, what):
+    """mixed precision with the learned photon-energy sampler against FP64:
+    photon and pair yields within 1% (or 3 sigma), photon spectrum error (KS
+    distance, the largest difference of the cumulative spectra) within 1%."""
+    f64, _ = run_gpu_case(case)
+    net, _ = run_gpu_case(case, precision="mixed", sampler=_sampler())
+    for name in ("photon", "positron"):
+        n64, nm = f64[name]["n"], net[name]["n"]
+        tol = max(0.01 * n64, 3 * np.sqrt(max(n64, 1)))
+        print(what, name, "yield", n64, nm, (nm - n64) / max(n64, 1))
+        assert abs(nm - n64) <= tol, (what, name, n64, nm)
+    e64 = np.sqrt(f64["photon"]["ux"] ** 2 + f64["photon"]["uy"] ** 2 + f64["photon"]["uz"] ** 2)
+    em = np.sqrt(net["photon"]["ux"] ** 2 + net["photon"]["uy"] ** 2 + net["photon"]["uz"] ** 2)
+    ks = stats.ks_2samp(e64, em)
+    print(what, "photon spectrum KS distance", ks.statistic, "p", ks.pvalue)
+    # spectrum error: largest difference of the two cumulative spectra <= 1%
+    assert ks.statistic <= 0.01, (what, ks)
+
+
+def test_mixed_mlp_head_on_rs():
+    _compare_net(head_on_case(n=40000, nsteps=240, seed=11, a0=250.0), "head-on RS + network")
+
+
+def test_mixed_mlp_head_on_od():
+    _compare_net(head_on_case(n=40000, nsteps=240, seed=12, a0=250.0, qed="od"),
+                 "head-on OD + network")
 
 
 def test_mixed_head_on_rs_yields_and_spectrum():

@@ -159,6 +159,35 @@ def test_rs_photon_rate_integrates_to_total():
         assert abs(total - od.photon_total[k]) / od.photon_total[k] < 3e-3, (chi, total, od.photon_total[k])
 
 
+def test_composition_from_zero_cdf_matches_rejection_sampling():
+    """Photon emission by composition -- an event with the total rate, delta
+    from the inverse of the cumulative rate measured from delta = 0 -- has the
+    hard-photon rate of rejection sampling (the learned sampler's scheme):
+    W (1 - C(thr) / C(1)) against the integral of the RS rate over delta >= thr
+    (the Gonoskov substitution delta = r^3)."""
+    import ctypes
+    L = oracle.lib()
+    T = oracle.rs_struct(load_rs_tables())
+    od = load_od_tables()
+    S = oracle.od_struct(od)
+    n = 20000
+    r = (np.arange(n) + 0.5) / n
+    for chi in (0.05, 0.3, 1.0, 3.0):
+        w = np.array([3 * ri * ri * L.orc_photon_prob_rate(chi, ri ** 3, ctypes.byref(T)) for ri in r])
+        W = L.orc_od_total_rate(od.photon_total.ctypes.data, ctypes.byref(S), chi)
+        k = min(int(np.floor((np.log10(chi) - od.log_chi_min) / od.log_chi_delta)), od.chi_N - 2)
+        cl, ch = od.chi_grid[k], od.chi_grid[k + 1]
+        row = od.photon_cdf[k] + (od.photon_cdf[k + 1] - od.photon_cdf[k]) / (ch - cl) * (chi - cl)
+        for thr in (1e-3, 4e-3):
+            rs_hard = np.mean(w * (r ** 3 >= thr))
+            comp = W * (1 - np.interp(thr, od.delta_grid, row) / row[-1])
+            assert abs(comp / rs_hard - 1) < 3e-3, (chi, thr, comp, rs_hard)
+            # the optical-depth model's own inverse CDF (uniform mapped onto
+            # [C(delta_0), C(1)]) draws measurably more hard photons
+            od_hard = W * (1 - (np.interp(thr, od.delta_grid, row) - row[0]) / (row[-1] - row[0]))
+            assert od_hard > comp
+
+
 def test_photon_rate_classical_limits():
     """chi -> 0: photon number rate / chi -> 5/(2 sqrt 3) alpha m c^2/hbar and
     radiated power / chi^2 -> 2/3 alpha m c^2/hbar (the LL drag of inline.py:105)
