@@ -234,6 +234,7 @@ enum sfp_attr {
   SFP_EX, SFP_EY, SFP_EZ, SFP_BX, SFP_BY, SFP_BZ,  /* float64, auxiliary */
   SFP_EVENT,                                       /* uint8, auxiliary */
   SFP_DELTA,                                       /* float64, auxiliary */
+  SFP_SHOT,                                        /* uint32: ensemble shot index, optional */
   SFP_NATTR
 };
 #define SFP_STORE_TAU 1  /* optical-depth attribute */
@@ -255,6 +256,14 @@ int sfp_store_download(void* store, int32_t attr, void* host, int64_t offset, in
 /* particles.py:503-516: stable compaction of every allocated attribute over
  * the non-pruned particles; the new count is returned (synchronous) */
 int sfp_store_prune(void* store, int64_t* n_kept);
+/* Diagnostics (no counterpart in the reference): per-shot counts and energy
+ * spectra of the live particles.  out[nshots][nbins + 1]: column 0 the count
+ * of the shot (particles without a SFP_SHOT attribute count as shot 0; larger
+ * indices as the last shot), columns 1..nbins the counts in equal bins of
+ * log10(energy) over [log10_min, log10_max), energy |u| for photon != 0 and
+ * sqrt(1 + u^2) otherwise.  Synchronous. */
+int sfp_store_histogram(void* store, int32_t photon, int32_t nshots, int32_t nbins,
+                        double log10_min, double log10_max, int64_t* out);
 
 /* ---------------------------------------------------------------------------
  * 3. Fused simulation (simulation.py:12-161)
@@ -305,7 +314,25 @@ int sfp_sim_destroy(void* sim);
  * would overflow a store halts on the device before changing anything; the
  * store grows and the step is replayed from the halted kernel. */
 #define SFP_RUN_TIMING 1 /* record HIP events around each species kernel */
+/* Run each full batch of 64 steps as one replay of a captured HIP graph
+ * (captured at the first such batch, again after the kernels or grids in use
+ * change); ignored with SFP_RUN_TIMING.  No counterpart in the reference. */
+#define SFP_RUN_GRAPH 2
+/* Online launch-shape tuner: a batch whose steps alternate between candidate
+ * persistent grids of each kernel, timed with events, picks the fastest; the
+ * tuner runs such a batch again when the work of a kernel changes by more
+ * than a factor of two.  Results do not depend on the grid.  No counterpart
+ * in the reference. */
+#define SFP_RUN_AUTOTUNE 4
 int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t flags);
+/* Pin the persistent grid (workgroups of 256 threads) of one species' sweep
+ * (phase 0) or QED (phase 1) kernel; 0 returns it to the tuner, whose default
+ * is full occupancy.  No counterpart in the reference. */
+int sfp_sim_set_grid(void* sim, int32_t species, int32_t phase, int32_t grid);
+/* Grid in use, workgroup size and number of candidate grids of a kernel
+ * (all 0 for a species without that kernel). */
+int sfp_sim_launch_shape(void* sim, int32_t species, int32_t phase, int32_t* grid,
+                         int32_t* tile, int32_t* ncand);
 /* Accumulated time (ms) and launch count of one species' step kernel since
  * the last reset (requires SFP_RUN_TIMING; synchronous).  phase 0 is the
  * sweep kernel (envelope bound, push of particles that cannot undergo a QED
@@ -313,13 +340,20 @@ int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t fl
 int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
+/* Ensemble of independent shots (no counterpart in the reference): shot s has
+ * its own field, the nterms[s] terms starting at terms + sum_{j<s} nterms[j].
+ * Every particle of a species with a SFP_SHOT attribute moves in the field of
+ * its shot, and particles it creates inherit the shot.  Draws stay keyed by
+ * particle id, so shots are independent streams. */
+int sfp_sim_set_shots(void* sim, const sfp_field* terms, const int32_t* nterms, int32_t nshots);
+
 /* Use the learned photon-energy sampler (sfp_mlp_*) for the photon emission
  * of the leptons, or stop using it (mlp NULL).  Mixed precision only; needs
  * the optical-depth tables.  The network must outlive its use by the
  * simulation.  No counterpart in the reference. */
 int sfp_sim_set_sampler(void* sim, void* mlp);
 
-/* Counters of one species since the last reset (synchronous), stats[10]:
+/* Counters of one species since the last reset (synchronous), stats[13]:
  * [0] particle-steps (live particles after each step, the reference's
  *     ns/particle denominator, simulation.py:150-151)
  * [1] particles read by the sweep kernel   [2] particles appended by the species
@@ -327,7 +361,9 @@ int sfp_sim_set_sampler(void* sim, void* mlp);
  * [5] host synchronisations of the run     [6] replays after a capacity halt
  * [7] current store capacity
  * [8] photon-steps inside multi-step pushes (the sweep reads only a 4-byte
- *     wake step for these)                 [9] multi-step pushes started */
+ *     wake step for these)                 [9] multi-step pushes started
+ * [10] batches run as a graph replay       [11] graph captures
+ * [12] tuning batches (10-12: of the simulation, not the species) */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 

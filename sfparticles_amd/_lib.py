@@ -19,14 +19,16 @@ SFP_E_CAPACITY = 3
 SFP_E_STATE = 4
 
 # store attributes (enum sfp_attr)
-X, Y, Z, UX, UY, UZ, ID, PRUNED, TAU, SX, SY, SZ, INV_GAMMA, CHI, EX, EY, EZ, BX, BY, BZ, EVENT, DELTA = range(22)
-NATTR = 22
+X, Y, Z, UX, UY, UZ, ID, PRUNED, TAU, SX, SY, SZ, INV_GAMMA, CHI, EX, EY, EZ, BX, BY, BZ, EVENT, DELTA, SHOT = range(23)
+NATTR = 23
 STORE_TAU = 1
 STORE_SPIN = 2
 
 RR_NONE, RR_PHOTON, RR_LL, RR_CLL = 0, 1, 2, 3
 QED_RS, QED_OD = 0, 1
 RUN_TIMING = 1
+RUN_GRAPH = 2
+RUN_AUTOTUNE = 4
 
 
 class SFPError(RuntimeError):
@@ -119,6 +121,7 @@ SIGNATURES = {
     "sfp_mlp_sample": [P, P, P, I64, I32, P, P],
     "sfp_mlp_device": [P, C.POINTER(P), C.POINTER(P)],
     "sfp_sim_set_sampler": [P, P],
+    "sfp_sim_set_shots": [P, P, P, I32],
     "sfp_store_create": [I32, I64, C.POINTER(P)],
     "sfp_store_destroy": [P],
     "sfp_store_reserve": [P, I64],
@@ -134,6 +137,9 @@ SIGNATURES = {
     "sfp_sim_run": [P, C.POINTER(D), U64, I64, I32],
     "sfp_sim_kernel_time": [P, I32, I32, C.POINTER(D), C.POINTER(I64)],
     "sfp_sim_reset_timing": [P],
+    "sfp_store_histogram": [P, I32, I32, I32, D, D, P],
+    "sfp_sim_set_grid": [P, I32, I32, I32],
+    "sfp_sim_launch_shape": [P, I32, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
     "sfp_sim_stats": [P, I32, C.POINTER(I64)],
     "sfp_sim_reset_stats": [P],
     "sfp_copy_f64": [P, P, I64, P],

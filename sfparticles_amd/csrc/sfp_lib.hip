@@ -727,7 +727,7 @@ extern "C" int sfp_tables_destroy(void* tables) {
 
 // --------------------------------------------------------------- stores ---
 const size_t kAttrSize[SFP_NATTR] = {8, 8, 8, 8, 8, 8, 8, 1, 8, 8, 8, 8,
-                                     8, 8, 8, 8, 8, 8, 8, 8, 1, 8};
+                                     8, 8, 8, 8, 8, 8, 8, 8, 1, 8, 4};
 
 static bool attr_core(int a) { return a <= SFP_PRUNED; }
 
@@ -882,6 +882,9 @@ extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
       if (kAttrSize[a] == 8)
         hipLaunchKernelGGL(k_gather<uint64_t>, dim3(op_grid(kept)), dim3(OPB), 0, 0,
                            (const uint64_t*)s->p[a], (uint64_t*)tmp, idx, kept);
+      else if (kAttrSize[a] == 4)
+        hipLaunchKernelGGL(k_gather<uint32_t>, dim3(op_grid(kept)), dim3(OPB), 0, 0,
+                           (const uint32_t*)s->p[a], (uint32_t*)tmp, idx, kept);
       else
         hipLaunchKernelGGL(k_gather<uint8_t>, dim3(op_grid(kept)), dim3(OPB), 0, 0,
                            (const uint8_t*)s->p[a], (uint8_t*)tmp, idx, kept);
@@ -899,6 +902,79 @@ extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
   c.n_old = kept;
   HIPCHK(hipMemcpy(s->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   *n_kept = kept;
+  return SFP_OK;
+}
+
+// --------------------------------------------------------- diagnostics ----
+// Per-shot counts and energy histograms of live particles: row k of out holds
+// the count of shot k, then nbins bins of log10(energy), energy = |u| for
+// photons and sqrt(1 + u^2) otherwise (m c^2 units); values outside
+// [lmin, lmax) are counted in the total only.  Rows are privatised in LDS when
+// they fit.
+static constexpr int HIST_LDS = 4096;
+
+__global__ void __launch_bounds__(OPB)
+k_histogram(const double* __restrict__ ux, const double* __restrict__ uy,
+            const double* __restrict__ uz, const uint8_t* __restrict__ pruned,
+            const uint32_t* __restrict__ shot, int64_t n, int32_t nshots, int32_t nbins,
+            double lmin, double inv_w, int photon, unsigned long long* __restrict__ out) {
+  __shared__ uint32_t sh[HIST_LDS];
+  const int64_t cells = (int64_t)nshots * (nbins + 1);
+  const bool lds = cells <= HIST_LDS;
+  if (lds) {
+    for (int64_t c = threadIdx.x; c < cells; c += blockDim.x) sh[c] = 0;
+    __syncthreads();
+  }
+  KLOOP(n) {
+    if (pruned[ip]) continue;
+    const uint32_t k = shot ? min(shot[ip], (uint32_t)nshots - 1u) : 0u;
+    const int64_t row = (int64_t)k * (nbins + 1);
+    int64_t cell[2] = {row, -1};
+    if (nbins > 0) {
+      const double u2 = ux[ip] * ux[ip] + uy[ip] * uy[ip] + uz[ip] * uz[ip];
+      const double en = photon ? sqrt(u2) : sqrt(1.0 + u2);
+      const double b = (log10(en) - lmin) * inv_w;
+      if (b >= 0.0 && b < (double)nbins) cell[1] = row + 1 + (int64_t)b;
+    }
+    for (int j = 0; j < 2; ++j) {
+      if (cell[j] < 0) continue;
+      if (lds)
+        atomicAdd(&sh[cell[j]], 1u);
+      else
+        atomicAdd(&out[cell[j]], 1ull);
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < cells; c += blockDim.x)
+      if (sh[c]) atomicAdd(&out[c], (unsigned long long)sh[c]);
+  }
+}
+
+extern "C" int sfp_store_histogram(void* store, int32_t photon, int32_t nshots, int32_t nbins,
+                                   double log10_min, double log10_max, int64_t* out) {
+  Store* s = (Store*)store;
+  if (!s || !out) return set_err(SFP_E_ARG, "null store/output");
+  if (nshots < 1 || nbins < 0 || (int64_t)nshots * (nbins + 1) > ((int64_t)1 << 28))
+    return set_err(SFP_E_ARG, "bad histogram shape");
+  if (nbins > 0 && !(log10_max > log10_min)) return set_err(SFP_E_ARG, "empty energy range");
+  int64_t n = 0;
+  uint64_t next_id = 0;
+  CHK(sfp_store_get_count(s, &n, &next_id));
+  const size_t bytes = sizeof(uint64_t) * (size_t)nshots * (nbins + 1);
+  unsigned long long* dev = nullptr;
+  HIPCHK(hipMalloc(&dev, bytes));
+  HIPCHK(hipMemset(dev, 0, bytes));
+  if (n > 0) {
+    const double inv_w = nbins > 0 ? nbins / (log10_max - log10_min) : 0.0;
+    hipLaunchKernelGGL(k_histogram, dim3(op_grid(n)), dim3(OPB), 0, 0, (const double*)s->p[SFP_UX],
+                       (const double*)s->p[SFP_UY], (const double*)s->p[SFP_UZ],
+                       (const uint8_t*)s->p[SFP_PRUNED], (const uint32_t*)s->p[SFP_SHOT], n,
+                       nshots, nbins, log10_min, inv_w, photon, dev);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpy(out, dev, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipFree(dev));
   return SFP_OK;
 }
 

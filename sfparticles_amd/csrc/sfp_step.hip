@@ -45,6 +45,7 @@ struct SpeciesDev {
   double *sx, *sy, *sz;
   uint32_t* hot;   // this step's QED list (ascending particle indices)
   uint32_t* wake;  // photons: run-relative step of the next visit (multi-step push), or null
+  uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
   Counts* cnt;
   int64_t cap;
   double q, ae;
@@ -84,6 +85,8 @@ struct Plan {
   double zmax15;    // z_max^1.5: rate is 0 unless delta/(1-delta) <= zmax15 chi (photon)
   uint64_t seed;
   Halt* halt;
+  const sfp_fieldset* shot_fields;  // ensemble: one field set per shot, or null
+  uint32_t nshots;
   const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
   const sfp_mlp_bins* net_bins;  // its per-bin table fallback
   int32_t nspecies;
@@ -176,16 +179,29 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
   return (int64_t)(*sh_pref) + woff + lrank;
 }
 
+// The field set of particle i: its shot's in an ensemble (indices at or above
+// the number of shots take the last shot), else the common one.
+__device__ inline const sfp_fieldset* fields_of(const Plan& P, const SpeciesDev& sp, int64_t i,
+                                               uint32_t& shot) {
+  if (sp.shot) {
+    shot = min(sp.shot[i], P.nshots - 1u);
+    return &P.shot_fields[shot];
+  }
+  shot = 0;
+  return &P.fields;
+}
+
 // -------------------------------------------------------- particle push --
 // Push of a lepton from the first position half-step to the second
 // (simulation.py:128-137 with particles.py:244-296); ig is 1/gamma on entry.
 template <int FM>
-__device__ inline void lepton_push(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+__device__ inline void lepton_push(const Plan& P, const sfp_fieldset* FS, const SpeciesDev& sp,
+                                   const StepArgs& a,
                                    double& x, double& y, double& z, double& ux, double& uy,
                                    double& uz, double& sx, double& sy, double& sz, double ig) {
   sfp_push_position(&x, &y, &z, ux, uy, uz, ig, a.cdt_half);
   double F[6];
-  sfp_field_eval<FM>(&P.fields, x, y, z, a.t_half, F);
+  sfp_field_eval<FM>(FS, x, y, z, a.t_half, F);
   if (sp.spin) {
     sfp_boris_tbmt(&ux, &uy, &uz, &ig, &sx, &sy, &sz, F[0], F[1], F[2], F[3], F[4], F[5], sp.q,
                    sp.ae, a.dt);
@@ -203,7 +219,8 @@ __device__ inline void lepton_push(const Plan& P, const SpeciesDev& sp, const St
 // The same push in mixed precision: positions accumulate in FP64 from the
 // FP32 momentum, field and pusher run in FP32, chi for LL/CLL in FP64.
 template <int FM>
-__device__ inline void lepton_push32(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+__device__ inline void lepton_push32(const Plan& P, const sfp_fieldset* FS, const SpeciesDev& sp,
+                                     const StepArgs& a,
                                      double& x, double& y, double& z, float& ux, float& uy,
                                      float& uz, float& sx, float& sy, float& sz, float ig) {
   double h = a.cdt_half * (double)ig;
@@ -211,7 +228,7 @@ __device__ inline void lepton_push32(const Plan& P, const SpeciesDev& sp, const 
   y += h * (double)uy;
   z += h * (double)uz;
   float F[6];
-  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t_half, F);
+  sfp32_field_eval<FM>(FS, x, y, z, a.t_half, F);
   const float q = (float)sp.q, dt = (float)a.dt;
   if (sp.spin) {
     ig = sfp32_boris_tbmt(ux, uy, uz, sx, sy, sz, F, q, (float)sp.ae, dt);
@@ -232,19 +249,20 @@ __device__ inline void lepton_push32(const Plan& P, const SpeciesDev& sp, const 
 
 // Append one new lepton (pair member) at slot d of species `sp`, pushed.
 template <int FM, bool MIX = false>
-__device__ inline void spawn_lepton(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
-                                    int64_t d, uint64_t id, double x, double y, double z,
-                                    double ux, double uy, double uz) {
+__device__ inline void spawn_lepton(const Plan& P, const sfp_fieldset* FS, uint32_t shot,
+                                    const SpeciesDev& sp, const StepArgs& a, int64_t d,
+                                    uint64_t id, double x, double y, double z, double ux,
+                                    double uy, double uz) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (sp.push) {
     if (MIX) {
       float fx = (float)ux, fy = (float)uy, fz = (float)uz, s0 = 0.f, s1 = 0.f, s2 = 0.f;
-      lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2,
+      lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2,
                         __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz));
       ux = fx; uy = fy; uz = fz;
     } else {
       double ig = 1.0 / sfp_sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-      lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+      lepton_push<FM>(P, FS, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
     }
   }
   sp.x[d] = x; sp.y[d] = y; sp.z[d] = z;
@@ -253,10 +271,12 @@ __device__ inline void spawn_lepton(const Plan& P, const SpeciesDev& sp, const S
   sp.pruned[d] = 0;
   if (sp.tau) sp.tau[d] = 0.0;
   if (sp.spin) { sp.sx[d] = 0.0; sp.sy[d] = 0.0; sp.sz[d] = 0.0; }
+  if (sp.shot) sp.shot[d] = shot;
 }
 
 // Append one new photon at slot d of species `sp`, pushed.
-__device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int64_t d, uint64_t id,
+__device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const StepArgs& a,
+                                    int64_t d, uint64_t id,
                                     double x, double y, double z, double ux, double uy,
                                     double uz) {
   if (sp.push) {
@@ -269,6 +289,7 @@ __device__ inline void spawn_photon(const SpeciesDev& sp, const StepArgs& a, int
   sp.id[d] = id;
   sp.pruned[d] = 0;
   if (sp.tau) sp.tau[d] = 0.0;
+  if (sp.shot) sp.shot[d] = shot;
 }
 
 // Can a QED event happen this step?  chi_b >= chi (bound with a small margin).  Rejection sampling: an event needs r2 < p with r2 >= 2^-53 (the
@@ -362,7 +383,8 @@ __device__ inline uint32_t next_epoch(const ScanState* scan) {
 // one wave's lanes -- wake on the same steps.  Photons of a species without
 // QED sleep to the end of the run.
 template <int FM>
-__device__ inline uint32_t sleep_steps(const Plan& P, const StepArgs& a, bool qed, double x,
+__device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, const StepArgs& a,
+                                       bool qed, double x,
                                        double y, double z, double ux, double uy, double uz,
                                        double ig) {
   const uint32_t rem = a.rel_end - a.rel;
@@ -374,7 +396,7 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const StepArgs& a, bool qe
     uint32_t end = (k == 0) ? a.rel_end : (a.rel / q[k - 1] + 1u) * q[k - 1];
     if (end > a.rel_end || end < a.rel + 2u || (k > 0 && end == a.rel_end)) continue;
     uint32_t d = end - a.rel;
-    float S = sfp_field_bound_f32_path<FM>(&P.fields, x, y, z, a.t, vx, vy, vz,
+    float S = sfp_field_bound_f32_path<FM>(FS, x, y, z, a.t, vx, vy, vz,
                                            a.dt * (double)d * 1.000001);
     float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
     if (!qed_possible(P, true, chi_b, ig, a.dt)) return d;
@@ -396,7 +418,8 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const StepArgs& a, bool qe
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, StepArgs a) {
+__global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
@@ -471,11 +494,13 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
         }
         proc += 1;
       }
+      uint32_t shot = 0;
+      const sfp_fieldset* FS = active ? fields_of(P, sp, i, shot) : &P.fields;
       bool hot = false;
       if (QED && active && sp.push) {
         // chi <= (e hbar/(m^2 c^3)) gamma (|E| + c|B|) (gamma >= |u|); the FP32
         // bound and this product carry a small margin for their own rounding
-        float S = sfp_field_bound_f32<FM>(&P.fields, x, y, z, a.t);
+        float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
         float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
         hot = qed_possible(P, PHOTON, chi_b, ig, a.dt);
       }
@@ -491,7 +516,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
           // a wave take them together
           if (wake)
             nst = !sp.push ? a.rel_end - a.rel
-                           : ((a.rel & 7u) == 0 ? sleep_steps<FM>(P, a, QED, x, y, z, ux, uy, uz, ig)
+                           : ((a.rel & 7u) == 0 ? sleep_steps<FM>(P, FS, a, QED, x, y, z, ux, uy, uz, ig)
                                                 : 1u);
           if (sp.push) {
             // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
@@ -518,11 +543,11 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
           if (MIX) {
             float fx = (float)ux, fy = (float)uy, fz = (float)uz;
             float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
-            lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, (float)ig);
+            lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, (float)ig);
             ux = fx; uy = fy; uz = fz;
             sx = s0; sy = s1; sz = s2;
           } else {
-            lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+            lepton_push<FM>(P, FS, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
           }
           sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
           sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
@@ -605,14 +630,15 @@ __device__ inline bool room_or_halt(const Plan& P, int s, const StepArgs& a, con
 // or the optical-depth update (qed/optical_depth.py:15-110), then the
 // hard-photon cut for radiating leptons (cpu.py:227-232).
 template <bool PHOTON, int KIND, int FM>
-__device__ inline bool qed_event(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+__device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const SpeciesDev& sp,
+                                 const StepArgs& a,
                                  const sfp_rs_node* lds_rs, const double2* lds_od, double x,
                                  double y, double z, double ux, double uy, double uz, double ig,
                                  uint64_t pid, double& tau, double& delta) {
   bool event = false;
   delta = 0.0;
   double F[6];
-  sfp_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  sfp_field_eval<FM>(FS, x, y, z, a.t, F);
   double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
   if (KIND == KIND_RS) {
     if (chi != 0.0) {
@@ -683,14 +709,15 @@ __device__ inline bool od_update32(double* tau, float ig, double chi, float dt, 
 // rejection-sampling rate from FP32 tables with 24-bit uniforms (the leading
 // bits of the same Philox words as the FP64 step), optical depth as above.
 template <bool PHOTON, int KIND, int FM>
-__device__ inline bool qed_event32(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+__device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const SpeciesDev& sp,
+                                   const StepArgs& a,
                                    const sfp_rs_node32* lds_rs, const double2* lds_od, double x,
                                    double y, double z, float ux, float uy, float uz, float ig,
                                    uint64_t pid, double& tau, double& delta) {
   bool event = false;
   delta = 0.0;
   float F[6];
-  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = PHOTON ? sfp32_chi_photon(F, ux, uy, uz) : sfp32_chi(F, ux, uy, uz);
   if (KIND == KIND_RS) {
     if (chi != 0.0) {
@@ -739,7 +766,8 @@ __device__ inline bool qed_event32(const Plan& P, const SpeciesDev& sp, const St
 // delta is filled in by the wave afterwards; bins flagged for fallback take
 // the table's inverse CDF.  The hard-photon cut is left to the caller.
 template <int KIND, int FM>
-__device__ inline bool qed_event32_net(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+__device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
+                                       const SpeciesDev& sp, const StepArgs& a,
                                        const sfp_rs_node32* lds_rs, const double2* lds_od,
                                        double x, double y, double z, float ux, float uy, float uz,
                                        float ig, uint64_t pid, double& tau, double& delta,
@@ -749,7 +777,7 @@ __device__ inline bool qed_event32_net(const Plan& P, const SpeciesDev& sp, cons
   delta = 0.0;
   net = false;
   float F[6];
-  sfp32_field_eval<FM>(&P.fields, x, y, z, a.t, F);
+  sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = sfp32_chi(F, ux, uy, uz);
   chi_out = chi;
   const sfp_mlp_bins& nb = *P.net_bins;
@@ -882,7 +910,8 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // appended already pushed (the reference pushes particles in the step that
 // creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
 template <int KIND, int FM, bool MIX, bool NET>
-__global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, StepArgs a) {
+__global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
@@ -930,6 +959,8 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       pid = sp.id[i];
     }
+    uint32_t shot = 0;
+    const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
     float fx = (float)ux, fy = (float)uy, fz = (float)uz;
     float igf = 1.0f;
     double ig = 1.0;
@@ -947,7 +978,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       float in0 = 0.0f, in1 = 0.0f;
       double chi_n = 0.0, u_tab = 0.0;
       if (valid)
-        event = qed_event32_net<KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+        event = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
                                           fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
                                           u_tab);
       net_delta_wave(P, net, in0, in1, delta);
@@ -958,10 +989,10 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       if (event && (float)delta / igf < (float)P.threshold) event = false;
     } else if (valid) {
       if (MIX) {
-        event = qed_event32<false, KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+        event = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
                                              z, fx, fy, fz, igf, pid, tau, delta);
       } else {
-        event = qed_event<false, KIND, FM>(P, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+        event = qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
                                            ux, uy, uz, ig, pid, tau, delta);
       }
     }
@@ -971,10 +1002,10 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
     if (event) {
       if (MIX) {
         const float df = (float)delta;
-        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
+        spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
                      df * fz);
       } else {
-        spawn_photon(*dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+        spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
                      delta * uy, delta * uz);
       }
     }
@@ -989,7 +1020,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
           igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
         }
         float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
-        lepton_push32<FM>(P, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+        lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
         ux = fx; uy = fy; uz = fz;
         sx = s0; sy = s1; sz = s2;
       } else {
@@ -999,7 +1030,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
           uz *= 1 - delta;
           ig = sfp_inv_gamma_lepton(ux, uy, uz);
         }
-        lepton_push<FM>(P, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+        lepton_push<FM>(P, FS, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
       }
       sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
       sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
@@ -1040,7 +1071,8 @@ static constexpr uint32_t WAKE_NEVER = 0xffffffffu;
 static constexpr int AW = 16;
 static constexpr int64_t WAKE_PAD = (int64_t)TILE * AW;
 
-__global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, StepArgs a) {
+__global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
@@ -1138,7 +1170,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 }
 
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, StepArgs a) {
+__global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
   const SpeciesDev& sp = P.sp[s];
@@ -1186,9 +1219,11 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
       }
     }
     const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
+    uint32_t shot = 0;
+    const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
     bool hot = false;
     if (live && sp.push) {
-      float S = sfp_field_bound_f32<FM>(&P.fields, x, y, z, a.t);
+      float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
       float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
       hot = qed_possible(P, true, chi_b, ig, a.dt);
     }
@@ -1197,11 +1232,11 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     if (hot) {
       if (KIND == KIND_OD) tau = sp.tau[i];
       if (MIX) {
-        event = qed_event32<true, KIND, FM>(P, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+        event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
                                             z, (float)ux, (float)uy, (float)uz, (float)ig,
                                             sp.id[i], tau, delta);
       } else {
-        event = qed_event<true, KIND, FM>(P, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+        event = qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
                                           ux, uy, uz, ig, sp.id[i], tau, delta);
       }
     }
@@ -1211,10 +1246,10 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     if (event) {
       double dp = 1.0 - delta;
       if (dA)
-        spawn_lepton<FM, MIX>(P, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
+        spawn_lepton<FM, MIX>(P, FS, shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
                               delta * ux, delta * uy, delta * uz);
       if (dB)
-        spawn_lepton<FM, MIX>(P, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
+        spawn_lepton<FM, MIX>(P, FS, shot, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
                               dp * uy, dp * uz);
     }
 
@@ -1230,7 +1265,7 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
         if (!sp.push) {
           nst = a.rel_end - a.rel;
         } else if (!hot && (a.rel & 7u) == 0) {
-          nst = sleep_steps<FM>(P, a, true, x, y, z, ux, uy, uz, ig);
+          nst = sleep_steps<FM>(P, FS, a, true, x, y, z, ux, uy, uz, ig);
         }
         if (sp.push) {
           // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
@@ -1276,7 +1311,8 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
 // prologue of a step: snapshot the counts the species kernels iterate over
 // (and, on the steps where photons fall asleep and wake, restart the count of
 // sleeping photons that the list kernel rebuilds)
-__global__ void k_step_prologue(const Plan* plan, StepArgs a) {
+__global__ void k_step_prologue(const Plan* plan, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
   if (plan->halt->flag) return;
   int s = threadIdx.x;
   if (s < plan->nspecies) {
@@ -1287,7 +1323,7 @@ __global__ void k_step_prologue(const Plan* plan, StepArgs a) {
 }
 
 // ------------------------------------------------------------- dispatch ----
-typedef void (*kfn)(const Plan*, int, StepArgs);
+typedef void (*kfn)(const Plan*, int, const StepArgs*);
 
 template <int FM, bool MIX>
 static kfn sweep_fn(bool photon, bool qed) {
@@ -1326,6 +1362,17 @@ static kfn pick_hot(int fm, bool photon, int kind, bool mixed, bool net = false)
 }
 
 // ------------------------------------------------------------- host side ---
+static constexpr int MAX_CAND = 8;
+static constexpr int64_t SYNC_EVERY = 64;
+
+// events around one kernel launch: for the caller's timing and/or for the
+// launch-shape tuner (candidate index, or -1)
+struct EvRec {
+  hipEvent_t e0, e1;
+  int cand;
+  bool timed;
+};
+
 struct Sim {
   int nsp = 0;
   std::vector<sfp_species_desc> desc;
@@ -1333,8 +1380,24 @@ struct Sim {
   std::vector<int> qed;            // species has a QED process
   kfn sweep[SFP_MAX_SPECIES] = {};
   kfn hot[SFP_MAX_SPECIES] = {};
-  int grid_sweep[SFP_MAX_SPECIES] = {};
-  int grid_hot[SFP_MAX_SPECIES] = {};
+  // launch shapes: candidate persistent grids of each (species, phase) kernel,
+  // the one in use, and a grid the caller pinned (0: tuned)
+  std::vector<int> cand[SFP_MAX_SPECIES][2];
+  int choice[SFP_MAX_SPECIES][2] = {};
+  int pinned[SFP_MAX_SPECIES][2] = {};
+  double tune_ms[SFP_MAX_SPECIES][2][MAX_CAND] = {};
+  int tune_cnt[SFP_MAX_SPECIES][2][MAX_CAND] = {};
+  int64_t tune_n[SFP_MAX_SPECIES][2] = {};  // work of the kernel when last tuned (+1)
+  bool tuned = false;
+  int64_t tune_batches = 0;
+  // the step's stream, the arguments of one batch of steps (device slots,
+  // pinned staging) and the batch captured as a graph
+  hipStream_t stream = nullptr;
+  StepArgs* args_dev = nullptr;
+  StepArgs* args_host = nullptr;
+  hipGraphExec_t graph = nullptr;
+  bool graph_stale = true;
+  int64_t graph_batches = 0, captures = 0;
   std::vector<int64_t> flags_cap[2];
   std::vector<int64_t> hot_cap;
   std::vector<uint64_t*> flags[2];
@@ -1351,11 +1414,15 @@ struct Sim {
   sfp_sim_params params{};
   int fm = SFP_FM_ANY;
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending[SFP_MAX_SPECIES][2];
+  std::vector<EvRec> ev_pending[SFP_MAX_SPECIES][2];
   double ms_acc[SFP_MAX_SPECIES][2] = {};
   int64_t launches[SFP_MAX_SPECIES][2] = {};
   std::vector<hipEvent_t> ev_pool;
   int64_t syncs = 0, replays = 0;
+  int cus = 0;                       // compute units of the device
+  bool net = false;                  // leptons use the learned photon-energy sampler
+  sfp_fieldset* shot_fields = nullptr;  // device, one per shot
+  int32_t nshots = 0;
 };
 
 static int sim_read_counts(Sim* sim) {
@@ -1387,6 +1454,8 @@ static int sim_refresh_plan(Sim* sim) {
     d.sz = (double*)st->p[SFP_SZ];
     d.cnt = st->cnt;
     d.cap = st->cap;
+    // shots take effect once their fields are set
+    d.shot = sim->shot_fields ? (uint32_t*)st->p[SFP_SHOT] : nullptr;
     if (st->cap > (int64_t)UINT32_MAX)
       return set_err(SFP_E_ARG, "species %d: capacity above 2^32 particles", s);
     for (int ph = 0; ph < 2; ++ph) {
@@ -1436,6 +1505,52 @@ static int sim_refresh_plan(Sim* sim) {
   HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
   return SFP_OK;
 }
+
+// Kernel of every (species, phase) for the field mode, precision and sampler,
+// with a persistent grid of (workgroups per CU at full occupancy) x CUs.
+static int sim_pick_kernels(Sim* sim) {
+  if (!sim->cus) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    sim->cus = prop.multiProcessorCount;
+  }
+  const bool mixed = sim->params.precision == SFP_PRECISION_MIXED;
+  const int kind = sim->params.qed_model == SFP_QED_OD ? KIND_OD : KIND_RS;
+  for (int s = 0; s < sim->nsp; ++s) {
+    const bool photon = sim->plan.sp[s].photon;
+    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed);
+    sim->hot[s] =
+        sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, sim->net && !photon) : nullptr;
+    for (int ph = 0; ph < 2; ++ph) {
+      kfn fn = ph ? sim->hot[s] : sim->sweep[s];
+      std::vector<int>& c = sim->cand[s][ph];
+      c.clear();
+      sim->choice[s][ph] = 0;
+      if (!fn) continue;
+      if (sim->pinned[s][ph] > 0) {
+        c.push_back(sim->pinned[s][ph]);
+        continue;
+      }
+      // Tiles are taken from an atomic counter, so the grid only sets how
+      // many workgroups compete for them: full occupancy first (the default),
+      // then fewer per CU (shorter look-back chains and less contention when
+      // the work is small).  Results do not depend on the grid.
+      int per_cu = 0;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, TILE, 0));
+      for (int k = std::max(per_cu, 1); (int)c.size() < MAX_CAND; k /= 2) {
+        c.push_back(k * sim->cus);
+        if (k == 1) break;
+      }
+    }
+  }
+  sim->tuned = false;
+  sim->graph_stale = true;
+  return SFP_OK;
+}
+
+static int sim_grid(const Sim* sim, int s, int ph) { return sim->cand[s][ph][sim->choice[s][ph]]; }
 
 extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
                               const sfp_field* terms, int32_t nterms, void* tables,
@@ -1522,6 +1637,11 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
     P.od_pair_nodes = sim->tables->od_pair_nodes;
   }
   HIPCHK(hipMalloc(&sim->plan_dev, sizeof(Plan)));
+  // default priority; a blocking stream, so the synchronous copies of the
+  // host side order with it
+  HIPCHK(hipStreamCreate(&sim->stream));
+  HIPCHK(hipMalloc(&sim->args_dev, sizeof(StepArgs) * SYNC_EVERY));
+  HIPCHK(hipHostMalloc((void**)&sim->args_host, sizeof(StepArgs) * SYNC_EVERY, 0));
   HIPCHK(hipMalloc(&sim->scan_mem, sizeof(ScanState) * 2 * nspecies));
   HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * 2 * nspecies));
   HIPCHK(hipMalloc(&sim->halt, sizeof(Halt)));
@@ -1537,26 +1657,7 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   sim->hot_cap.assign(nspecies, 0);
   sim->wake_mem.assign(nspecies, nullptr);
   sim->wake_cap.assign(nspecies, 0);
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, dev));
-  int cus = prop.multiProcessorCount;
-  for (int s = 0; s < nspecies; ++s) {
-    bool photon = P.sp[s].photon;
-    const bool mixed = params->precision == SFP_PRECISION_MIXED;
-    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed);
-    sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, params->qed_model == SFP_QED_OD
-                                                                ? KIND_OD : KIND_RS, mixed)
-                              : nullptr;
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->sweep[s], TILE, 0));
-    sim->grid_sweep[s] = std::max(per_cu, 1) * cus;
-    if (sim->hot[s]) {
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->hot[s], TILE, 0));
-      sim->grid_hot[s] = std::max(per_cu, 1) * cus;
-    }
-  }
+  CHK(sim_pick_kernels(sim));
   CHK(sim_refresh_plan(sim));
   *out = sim;
   return SFP_OK;
@@ -1579,19 +1680,38 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
     P.net = nullptr;
     P.net_bins = nullptr;
   }
-  const int kind = sim->params.qed_model == SFP_QED_OD ? KIND_OD : KIND_RS;
-  for (int s = 0; s < sim->nsp; ++s) {
-    if (!sim->qed[s] || P.sp[s].photon) continue;
-    sim->hot[s] = pick_hot(sim->fm, false, kind, true, mlp != nullptr);
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)sim->hot[s], TILE, 0));
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, dev));
-    sim->grid_hot[s] = std::max(per_cu, 1) * prop.multiProcessorCount;
-  }
+  sim->net = mlp != nullptr;
+  CHK(sim_pick_kernels(sim));
   HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_shots(void* p, const sfp_field* terms, const int32_t* nterms,
+                                 int32_t nshots) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  if (nshots < 1 || nshots > (1 << 20)) return set_err(SFP_E_ARG, "between 1 and 2^20 shots");
+  std::vector<sfp_fieldset> fs(nshots);
+  int mode = -1;
+  int64_t off = 0;
+  for (int32_t k = 0; k < nshots; ++k) {
+    CHK(sfp_compile_fields(terms + off, nterms[k], &fs[k]));
+    off += nterms[k];
+    const int m = sfp_field_mode(&fs[k]);
+    mode = (mode < 0 || mode == m) ? m : SFP_FM_ANY;
+  }
+  if (sim->shot_fields) HIPCHK(hipFree(sim->shot_fields));
+  HIPCHK(hipMalloc(&sim->shot_fields, sizeof(sfp_fieldset) * nshots));
+  HIPCHK(hipMemcpy(sim->shot_fields, fs.data(), sizeof(sfp_fieldset) * nshots,
+                   hipMemcpyHostToDevice));
+  sim->nshots = nshots;
+  sim->plan.shot_fields = sim->shot_fields;
+  sim->plan.nshots = (uint32_t)nshots;
+  // one kernel instance for all shots: the common mode, or the general one
+  sim->fm = mode == SFP_FM_ANY ? SFP_FM_ANY : (sfp_field_mode(&sim->plan.fields) == mode ? mode
+                                                                                   : SFP_FM_ANY);
+  CHK(sim_pick_kernels(sim));
+  CHK(sim_refresh_plan(sim));
   return SFP_OK;
 }
 
@@ -1610,12 +1730,17 @@ extern "C" int sfp_sim_destroy(void* p) {
   if (sim->plan_dev) (void)hipFree(sim->plan_dev);
   if (sim->halt) (void)hipFree(sim->halt);
   if (sim->halt_host) (void)hipHostFree(sim->halt_host);
+  if (sim->shot_fields) (void)hipFree(sim->shot_fields);
   if (sim->counts_host) (void)hipHostFree(sim->counts_host);
+  if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
+  if (sim->args_dev) (void)hipFree(sim->args_dev);
+  if (sim->args_host) (void)hipHostFree(sim->args_host);
+  if (sim->stream) (void)hipStreamDestroy(sim->stream);
   for (int s = 0; s < SFP_MAX_SPECIES; ++s)
     for (int ph = 0; ph < 2; ++ph)
       for (auto& e : sim->ev_pending[s][ph]) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.e0);
+        (void)hipEventDestroy(e.e1);
       }
   for (auto e : sim->ev_pool) (void)hipEventDestroy(e);
   delete sim;
@@ -1633,31 +1758,68 @@ static hipEvent_t sim_event(Sim* sim) {
   return e;
 }
 
-static int launch(Sim* sim, kfn fn, int grid, int s, int phase, const StepArgs& a) {
+// Launch the kernel of (species s, phase ph) for the step in argument slot j.
+// tune >= 0: a tuning launch, run with candidate grid (tune mod candidates)
+// and timed for the tuner.
+static int launch(Sim* sim, int s, int ph, int j, int tune) {
+  kfn fn = ph ? sim->hot[s] : sim->sweep[s];
+  const std::vector<int>& cand = sim->cand[s][ph];
+  const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
+  const bool ev = sim->timing || tune >= 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (sim->timing) {
+  if (ev) {
     e0 = sim_event(sim);
     e1 = sim_event(sim);
-    if (e0) HIPCHK(hipEventRecord(e0, 0));
+    if (e0) HIPCHK(hipEventRecord(e0, sim->stream));
   }
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(TILE), 0, 0, (const Plan*)sim->plan_dev, s, a);
+  hipLaunchKernelGGL(fn, dim3(cand[c]), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev, s,
+                     (const StepArgs*)(sim->args_dev + j));
   HIPCHK(hipGetLastError());
-  if (sim->timing && e0 && e1) {
-    HIPCHK(hipEventRecord(e1, 0));
-    sim->ev_pending[s][phase].push_back({e0, e1});
+  if (ev && e0 && e1) {
+    HIPCHK(hipEventRecord(e1, sim->stream));
+    sim->ev_pending[s][ph].push_back({e0, e1, tune >= 0 ? c : -1, sim->timing});
   }
   return SFP_OK;
 }
 
-// Launch the kernels of one step from (species s0, phase ph0) on.
-static int launch_step(Sim* sim, const StepArgs& a, int s0, int ph0) {
-  if (s0 == 0 && ph0 == 0)
-    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, 0, (const Plan*)sim->plan_dev, a);
+// Launch the kernels of the step in slot j from (species s0, phase ph0) on.
+static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
+  if (s0 == 0 && ph0 == 0) {
+    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, sim->stream,
+                       (const Plan*)sim->plan_dev, (const StepArgs*)(sim->args_dev + j));
+    HIPCHK(hipGetLastError());
+  }
   for (int s = s0; s < sim->nsp; ++s) {
     int first = (s == s0) ? ph0 : 0;
-    if (first == 0) CHK(launch(sim, sim->sweep[s], sim->grid_sweep[s], s, 0, a));
-    if (sim->hot[s]) CHK(launch(sim, sim->hot[s], sim->grid_hot[s], s, 1, a));
+    if (first == 0) CHK(launch(sim, s, 0, j, tune));
+    if (sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
   }
+  return SFP_OK;
+}
+
+// Capture one batch of SYNC_EVERY steps (slots 0..SYNC_EVERY-1) as a graph.
+// The kernels read their step from the slots and the plan from device
+// memory, so the graph stays valid while the kernels and grids in use do.
+static int sim_capture(Sim* sim) {
+  if (sim->graph) {
+    HIPCHK(hipGraphExecDestroy(sim->graph));
+    sim->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
+  int rc = SFP_OK;
+  for (int j = 0; j < SYNC_EVERY && rc == SFP_OK; ++j) rc = launch_step(sim, j, 0, 0, -1);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(sim->stream, &g);
+  if (rc != SFP_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return set_err(SFP_E_HIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&sim->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_err(SFP_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  sim->graph_stale = false;
+  sim->captures++;
   return SFP_OK;
 }
 
@@ -1682,12 +1844,88 @@ static int sim_recover(Sim* sim, int s) {
   return SFP_OK;
 }
 
-static constexpr int64_t SYNC_EVERY = 64;
+// Wait for the stream and fold the recorded kernel times into the caller's
+// timing and the tuner's sums.
+static int sim_collect_events(Sim* sim) {
+  HIPCHK(hipStreamSynchronize(sim->stream));
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < 2; ++ph) {
+      for (auto& e : sim->ev_pending[s][ph]) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e.e0, e.e1));
+        if (e.timed) {
+          sim->ms_acc[s][ph] += ms;
+          sim->launches[s][ph] += 1;
+        }
+        if (e.cand >= 0) {
+          sim->tune_ms[s][ph][e.cand] += ms;
+          sim->tune_cnt[s][ph][e.cand] += 1;
+        }
+        sim->ev_pool.push_back(e.e0);
+        sim->ev_pool.push_back(e.e1);
+      }
+      sim->ev_pending[s][ph].clear();
+    }
+  return SFP_OK;
+}
+
+// Work of a kernel as the tuner sees it: particles swept, or listed.
+static int64_t sim_work(const Sim* sim, int s, int ph) {
+  const Counts& c = sim->counts_host[s];
+  return 1 + (ph ? c.n_hot : c.n);
+}
+
+// The tuner runs a batch again when the work of any kernel has changed by
+// more than a factor of two since it last did.
+static bool sim_needs_tune(const Sim* sim) {
+  if (!sim->tuned) return true;
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < 2; ++ph) {
+      if (sim->cand[s][ph].size() < 2) continue;
+      const int64_t w = sim_work(sim, s, ph), w0 = sim->tune_n[s][ph];
+      if (w > 2 * w0 || w0 > 2 * w) return true;
+    }
+  return false;
+}
+
+// After a tuning batch (steps interleaved over the candidates): keep the
+// candidate with the least time per launch.
+static int sim_tune_pick(Sim* sim) {
+  CHK(sim_collect_events(sim));
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < 2; ++ph) {
+      const int nc = (int)sim->cand[s][ph].size();
+      int best = sim->choice[s][ph];
+      double best_ms = 0.0;
+      bool any = false;
+      for (int c = 0; c < nc; ++c) {
+        if (!sim->tune_cnt[s][ph][c]) continue;
+        const double m = sim->tune_ms[s][ph][c] / sim->tune_cnt[s][ph][c];
+        if (!any || m < best_ms) {
+          best = c;
+          best_ms = m;
+          any = true;
+        }
+      }
+      if (best != sim->choice[s][ph]) sim->graph_stale = true;
+      sim->choice[s][ph] = best;
+      sim->tune_n[s][ph] = sim_work(sim, s, ph);
+      for (int c = 0; c < MAX_CAND; ++c) {
+        sim->tune_ms[s][ph][c] = 0.0;
+        sim->tune_cnt[s][ph][c] = 0;
+      }
+    }
+  sim->tuned = true;
+  sim->tune_batches++;
+  return SFP_OK;
+}
 
 extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {
   Sim* sim = (Sim*)p;
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   sim->timing = (flags & SFP_RUN_TIMING) != 0;
+  const bool use_graph = (flags & SFP_RUN_GRAPH) != 0 && !sim->timing;
+  const bool autotune = (flags & SFP_RUN_AUTOTUNE) != 0;
   // stores may have been resized or refilled by the caller since the last run
   bool dirty = false;
   for (int s = 0; s < sim->nsp; ++s)
@@ -1700,17 +1938,16 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   // between runs; the store may have changed)
   for (int s = 0; s < sim->nsp; ++s)
     if (sim->wake_mem[s])
-      HIPCHK(hipMemsetAsync(sim->wake_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], 0));
+      HIPCHK(hipMemsetAsync(sim->wake_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], sim->stream));
+  if (autotune) CHK(sim_read_counts(sim));
   const double dt = sim->params.dt;
-  std::vector<StepArgs> args;
-  args.reserve((size_t)std::min<int64_t>(nsteps, SYNC_EVERY));
   int64_t k = 0;
   double tt = *t;
   while (k < nsteps) {
-    int64_t batch = std::min<int64_t>(SYNC_EVERY, nsteps - k);
-    args.clear();
+    const int64_t batch = std::min<int64_t>(SYNC_EVERY, nsteps - k);
+    // the previous batch has been waited for: the staging slots are free
     for (int64_t j = 0; j < batch; ++j) {
-      StepArgs a;
+      StepArgs& a = sim->args_host[j];
       a.t = tt;
       a.t_half = tt + 0.5 * dt;
       a.dt = dt;
@@ -1720,9 +1957,19 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       a.step = step0 + (uint64_t)(k + j);
       a.rel = (uint32_t)(k + j);
       a.rel_end = (uint32_t)nsteps;
-      args.push_back(a);
       tt = tt + dt;
-      CHK(launch_step(sim, a, 0, 0));
+    }
+    HIPCHK(hipMemcpyAsync(sim->args_dev, sim->args_host, sizeof(StepArgs) * batch,
+                          hipMemcpyHostToDevice, sim->stream));
+    const bool tuning = autotune && sim_needs_tune(sim);
+    if (tuning) {
+      for (int64_t j = 0; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, (int)j));
+    } else if (use_graph && batch == SYNC_EVERY) {
+      if (sim->graph_stale || !sim->graph) CHK(sim_capture(sim));
+      HIPCHK(hipGraphLaunch(sim->graph, sim->stream));
+      sim->graph_batches++;
+    } else {
+      for (int64_t j = 0; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, -1));
     }
     CHK(sim_read_counts(sim));
     // replay from a halted kernel until the batch completes
@@ -1731,30 +1978,36 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       int64_t h = sim->halt_host->step - (int64_t)(step0 + k);
       if (h < 0 || h >= batch) return set_err(SFP_E_STATE, "halt outside the launched steps");
       CHK(sim_recover(sim, s));
-      CHK(launch_step(sim, args[h], s, 1));
-      for (int64_t j = h + 1; j < batch; ++j) CHK(launch_step(sim, args[j], 0, 0));
+      CHK(launch_step(sim, (int)h, s, 1, -1));
+      for (int64_t j = h + 1; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, -1));
       CHK(sim_read_counts(sim));
     }
+    if (tuning) CHK(sim_tune_pick(sim));
     k += batch;
   }
   *t = tt;
   return SFP_OK;
 }
 
-static int sim_collect_timing(Sim* sim) {
-  HIPCHK(hipDeviceSynchronize());
-  for (int s = 0; s < sim->nsp; ++s)
-    for (int ph = 0; ph < 2; ++ph) {
-      for (auto& e : sim->ev_pending[s][ph]) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
-        sim->ms_acc[s][ph] += ms;
-        sim->launches[s][ph] += 1;
-        sim->ev_pool.push_back(e.first);
-        sim->ev_pool.push_back(e.second);
-      }
-      sim->ev_pending[s][ph].clear();
-    }
+static int sim_collect_timing(Sim* sim) { return sim_collect_events(sim); }
+
+extern "C" int sfp_sim_set_grid(void* p, int32_t species, int32_t phase, int32_t grid) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1 || grid < 0)
+    return set_err(SFP_E_ARG, "bad species/phase/grid");
+  sim->pinned[species][phase] = grid;
+  return sim_pick_kernels(sim);
+}
+
+extern "C" int sfp_sim_launch_shape(void* p, int32_t species, int32_t phase, int32_t* grid,
+                                    int32_t* tile, int32_t* ncand) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1)
+    return set_err(SFP_E_ARG, "bad species/phase");
+  const bool has = !sim->cand[species][phase].empty();
+  *grid = has ? sim_grid(sim, species, phase) : 0;
+  *tile = has ? TILE : 0;
+  *ncand = (int32_t)sim->cand[species][phase].size();
   return SFP_OK;
 }
 
@@ -1796,6 +2049,9 @@ extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
   stats[7] = sim->stores[species]->cap;
   stats[8] = c.slept;
   stats[9] = c.sleeps;
+  stats[10] = sim->graph_batches;
+  stats[11] = sim->captures;
+  stats[12] = sim->tune_batches;
   return SFP_OK;
 }
 
@@ -1810,5 +2066,6 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   }
   sim->syncs = 0;
   sim->replays = 0;
+  sim->graph_batches = sim->captures = sim->tune_batches = 0;
   return SFP_OK;
 }

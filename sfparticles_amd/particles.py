@@ -51,7 +51,7 @@ _ATTR = {
     "Ex": (L.EX, np.float64), "Ey": (L.EY, np.float64), "Ez": (L.EZ, np.float64),
     "Bx": (L.BX, np.float64), "By": (L.BY, np.float64), "Bz": (L.BZ, np.float64),
     "event": (L.EVENT, np.bool_), "photon_delta": (L.DELTA, np.float64),
-    "pair_delta": (L.DELTA, np.float64),
+    "pair_delta": (L.DELTA, np.float64), "shot": (L.SHOT, np.uint32),
 }
 
 
@@ -135,6 +135,9 @@ class Particles(object):
         if name in _ATTR:
             dtype = _ATTR[name][1]
             arr = np.ascontiguousarray(np.asarray(value, dtype=dtype))
+            if name == "shot" and name not in self.attrs:
+                # ensemble membership is optional: an attribute once set
+                self.attrs.append(name)
             if self._store is None:
                 self._h[name] = arr
             else:

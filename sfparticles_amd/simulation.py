@@ -13,6 +13,8 @@ import ctypes as C
 from time import perf_counter_ns
 from typing import Callable
 
+import numpy as np
+
 from . import _lib as L
 from . import qed
 from .fields import Fields, terms_to_c
@@ -29,12 +31,19 @@ class Simulation(object):
 
     def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
                  t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None,
-                 precision: str = "fp64", sampler=None) -> None:
+                 precision: str = "fp64", sampler=None, graph: bool = False,
+                 autotune: bool = False, shots=None) -> None:
         """precision: "fp64" (the reference's arithmetic, bit-identical to the CPU
         restatement) or "mixed" (fields, pushers and QED sampling in FP32;
         positions, time and chi in FP64).  sampler (mixed only): a
         qed.mlp.PhotonSampler, or "mlp" for the trained default, that draws the
-        energy of emitted photons in place of the inverse-CDF table."""
+        energy of emitted photons in place of the inverse-CDF table.
+        graph: replay batches of 64 steps as a captured HIP graph.  autotune:
+        pick each kernel's launch grid online.  Neither changes the results.
+        shots: an ensemble of independent shots in one simulation, a list of
+        Fields, one per shot; every particle moves in the field of its
+        ``shot`` index (species without one get shot 0 for their particles)
+        and what it creates inherits the index.  ``fields`` is then unused."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         if sampler is not None and precision != "mixed":
@@ -46,6 +55,21 @@ class Simulation(object):
             sampler = default_sampler()
         self.precision = precision
         self.sampler = sampler
+        self.graph = bool(graph)
+        self.autotune = bool(autotune)
+        self.shots = None if shots is None else list(shots)
+        if self.shots is not None:
+            if not self.shots:
+                raise ValueError("an ensemble needs at least one shot")
+            if fields is None:
+                self.fields = fields = self.shots[0]
+            for p in all_particles:
+                if "shot" not in p.attrs:
+                    p.shot = np.zeros(p.N_buffered, np.uint32)
+                sh = np.asarray(p.shot)[:p.N_buffered]
+                if sh.size and int(sh.max()) >= len(self.shots):
+                    raise ValueError(f"{p.name}: shot index {int(sh.max())} without a field "
+                                     f"({len(self.shots)} shots)")
         self.all_particles = all_particles
         self.particles_push = []
         self.particles_bw = []
@@ -111,6 +135,11 @@ class Simulation(object):
         self._sim_stores = stores
         if self.sampler is not None:
             L.call("sfp_sim_set_sampler", self._sim, self.sampler.handle)
+        if self.shots is not None:
+            per = [f.require_terms() for f in self.shots]
+            flat = [t for ts in per for t in ts]
+            counts = (C.c_int32 * len(per))(*[len(ts) for ts in per])
+            L.call("sfp_sim_set_shots", self._sim, terms_to_c(flat), counts, len(per))
 
     def _destroy(self):
         if self._sim is not None and L._lib is not None:
@@ -123,15 +152,19 @@ class Simulation(object):
         except Exception:
             pass
 
-    def run_steps(self, nsteps: int, timing: bool = False) -> None:
-        """Advance nsteps without callbacks, progress or pruning."""
+    def _prepare(self):
         for p in self.all_particles:
             p._ensure_device()
             p._push_count_to_device()
         self._build()
+
+    def run_steps(self, nsteps: int, timing: bool = False) -> None:
+        """Advance nsteps without callbacks, progress or pruning."""
+        self._prepare()
         t = C.c_double(self.t)
-        L.call("sfp_sim_run", self._sim, C.byref(t), int(self.step), int(nsteps),
-               L.RUN_TIMING if timing else 0)
+        flags = (L.RUN_TIMING if timing else 0) | (L.RUN_GRAPH if self.graph else 0) | \
+            (L.RUN_AUTOTUNE if self.autotune else 0)
+        L.call("sfp_sim_run", self._sim, C.byref(t), int(self.step), int(nsteps), flags)
         self.t = t.value
         self.step += int(nsteps)
         for p in self.all_particles:
@@ -149,15 +182,30 @@ class Simulation(object):
     def reset_timing(self):
         L.call("sfp_sim_reset_timing", self._sim)
 
+    def set_grid(self, species: int, phase: int, grid: int):
+        """Pin the workgroup count of a species' sweep (0) or QED (1) kernel;
+        0 returns it to the tuner."""
+        self._prepare()
+        L.call("sfp_sim_set_grid", self._sim, int(species), int(phase), int(grid))
+
+    def launch_shape(self, species: int, phase: int):
+        """(grid, workgroup size, number of candidate grids) of a kernel."""
+        self._prepare()
+        g, t, n = C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("sfp_sim_launch_shape", self._sim, int(species), int(phase), C.byref(g),
+               C.byref(t), C.byref(n))
+        return int(g.value), int(t.value), int(n.value)
+
     def stats(self, species: int) -> dict:
         """Counters of one species since the last reset: particle-steps (live
         particles after each step), particles read by its step kernel,
         particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 10)()
+        v = (C.c_int64 * 13)()
         L.call("sfp_sim_stats", self._sim, int(species), v)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
                     hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
-                    slept=int(v[8]), sleeps=int(v[9]))
+                    slept=int(v[8]), sleeps=int(v[9]), graph_batches=int(v[10]),
+                    captures=int(v[11]), tune_batches=int(v[12]))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)

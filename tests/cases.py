@@ -70,7 +70,8 @@ def _species_list(case, make):
     return make(case)
 
 
-def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None):
+def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None,
+                 grids=None, **sim_kw):
     """Run a case through sfparticles_amd.Simulation; returns per-species arrays."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
@@ -100,7 +101,9 @@ def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sa
             g.set_pair(e, p)
     sim = sfa.Simulation(*sp, dt=case["dt"], fields=Fields(terms=case["field"]), print_every=None,
                          photon_threshold=case["threshold"], seed=case["seed"], precision=precision,
-                         sampler=sampler)
+                         sampler=sampler, **sim_kw)
+    for (si, ph), grid in (grids or {}).items():
+        sim.set_grid(si, ph, grid)
     steps = case["nsteps"] if nsteps is None else nsteps
     per = steps // chunks
     done = 0
