@@ -333,16 +333,25 @@ __device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, dou
 // Everything the last workgroup reads was produced by atomics (counters,
 // look-back flags) and what it writes is consumed by later launches, so no
 // memory fence is needed.
+// Only the first nwg workgroups of the grid take part (see participants).
 template <typename F>
-__device__ inline void kernel_exit(ScanState* scan, F&& last) {
+__device__ inline void kernel_exit(ScanState* scan, uint32_t nwg, F&& last) {
   if (threadIdx.x == 0) {
     uint32_t prev = atomicAdd(&scan->done_ctr, 1u);
-    if (prev == gridDim.x - 1) {
+    if (prev == nwg - 1) {
       last();
       scan->tile_ctr = 0;
       scan->done_ctr = 0;
     }
   }
+}
+
+// Workgroups of a persistent launch that take part when it has `units` of
+// work: no more than the units (at least one, which runs the exit protocol);
+// the others return at once, before touching any shared counter, so a short
+// list costs a few atomics instead of three per workgroup of the grid.
+__device__ inline uint32_t participants(int64_t units) {
+  return (uint32_t)(units < 1 ? 1 : (units < (int64_t)gridDim.x ? units : (int64_t)gridDim.x));
 }
 
 // Workgroup sums of N per-thread counters; every thread gets the totals.
@@ -427,10 +436,23 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_mask[ROWS][WAVES];
   __shared__ uint64_t sh_pref;
-  constexpr int64_t TSPAN = (int64_t)TILE * ROWS;
 
   const int64_t n_old = sp.cnt->n_old;
-  const int64_t ntiles = (n_old + TSPAN - 1) / TSPAN;
+  // Rows per tile: ROWS once the store fills a tile per workgroup of the
+  // grid; fewer (a power of two) for shorter stores, which then still spread
+  // over many workgroups.  The list order does not depend on the tiling.
+  int rows = ROWS;
+  {
+    const int64_t per = (n_old + (int64_t)TILE * gridDim.x - 1) / ((int64_t)TILE * gridDim.x);
+    if (per < ROWS) {
+      rows = 1;
+      while (rows < per) rows <<= 1;
+    }
+  }
+  const int64_t tspan = (int64_t)TILE * rows;
+  const int64_t ntiles = (n_old + tspan - 1) / tspan;
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
   ScanState* scan = P.scan[s][0];
   const uint32_t epoch = QED ? next_epoch(scan) : 0u;
   const int lane = wave_lane();
@@ -447,7 +469,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     const int64_t tile = sh_tile;
     __syncthreads();
     if (tile >= ntiles) break;
-    const int64_t base = tile * TSPAN;
+    const int64_t base = tile * tspan;
     uint32_t hotbits = 0;
     // photons: the wake steps of all rows of the tile, loaded together (one
     // dependent load per row would leave the sweep latency-bound)
@@ -457,7 +479,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
 #pragma unroll
       for (int row = 0; row < ROWS; ++row) {
         const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
-        w[row] = (i < n_old) ? wake[i] : 0u;
+        w[row] = (row < rows && i < n_old) ? wake[i] : 0u;
       }
 #pragma unroll
       for (int row = 0; row < ROWS; ++row) asleep |= (uint32_t)(w[row] > a.rel) << row;
@@ -468,7 +490,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     }
 
 #pragma unroll 1
-    for (int row = 0; row < ROWS; ++row) {
+    for (int row = 0; row < rows; ++row) {
       const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
       if (PHOTON && wake && !__any(i < n_old && !((asleep >> row) & 1u))) {
         // the whole wave sleeps (or lies past the end) in this row
@@ -561,7 +583,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       __syncthreads();
       if (wv == 0) {
         uint64_t c = 0;
-        for (int k = lane; k < ROWS * WAVES; k += 64) c += (uint64_t)__popcll((&sh_mask[0][0])[k]);
+        for (int k = lane; k < rows * WAVES; k += 64) c += (uint64_t)__popcll((&sh_mask[0][0])[k]);
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
         uint64_t pre = lookback(P.flags[s][0], tile, c, epoch);
         if (lane == 0) sh_pref = pre;
@@ -569,7 +591,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
       __syncthreads();
       uint64_t off = sh_pref;
       const uint64_t lt = (1ull << lane) - 1ull;
-      for (int row = 0; row < ROWS; ++row) {
+      for (int row = 0; row < rows; ++row) {
         uint32_t before = 0, total = 0;
         for (int k = 0; k < WAVES; ++k) {
           uint32_t c = (uint32_t)__popcll(sh_mask[row][k]);
@@ -595,7 +617,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
     count_add(&sp.cnt->slept, tot[2]);
     count_add(&sp.cnt->sleeps, tot[3]);
   }
-  kernel_exit(scan, [&]() {
+  kernel_exit(scan, nwg, [&]() {
     if (QED) {
       uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
       sp.cnt->n_hot = (int64_t)total;
@@ -924,27 +946,37 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   __shared__ uint64_t sh_pref;
 
   const int64_t n_hot = sp.cnt->n_hot;
+  const int64_t ntiles = (n_hot + TILE - 1) / TILE;
+  // workgroups without work leave before reading the destination counts,
+  // which the exit of this launch changes
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
   const SpeciesDev* dA = sp.photon_dst >= 0 ? &P.sp[sp.photon_dst] : nullptr;
   if (!room_or_halt(P, s, a, dA, nullptr, n_hot)) return;
   const int64_t baseA = dA ? dA->cnt->n : 0;
   const int64_t idA = dA ? dA->cnt->next_id : 0;
-  const int64_t ntiles = (n_hot + TILE - 1) / TILE;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
-  stage_tables<KIND, false, MIX>(P, lds_rs, lds_od);
-  if (NET && KIND == KIND_RS) {  // the total photon rate for composition sampling
-    const int64_t nn = P.od.chi_N - 1;
-    for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
-    __syncthreads();
-  }
 
   uint32_t alive = 0;
+  bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
     const int64_t tile = sh_tile;
     __syncthreads();
     if (tile >= ntiles) break;
+    // tables go to LDS once the workgroup has work (a short list leaves most
+    // of the persistent grid without a tile)
+    if (!staged) {
+      stage_tables<KIND, false, MIX>(P, lds_rs, lds_od);
+      if (NET && KIND == KIND_RS) {  // the total photon rate for composition sampling
+        const int64_t nn = P.od.chi_N - 1;
+        for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
+        __syncthreads();
+      }
+      staged = true;
+    }
 
     const int64_t k = tile * TILE + threadIdx.x;
     const bool valid = k < n_hot;
@@ -1044,7 +1076,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   uint64_t tot[1];
   block_sum(cnt, sh_w, tot);
   if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
-  kernel_exit(scan, [&]() {
+  kernel_exit(scan, nwg, [&]() {
     finish_appends(P, s, 1, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
     scan->epoch = epoch;
   });
@@ -1086,10 +1118,12 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     // list of the previous step stays valid (holes it holds are skipped) and
     // is extended by the photons appended since, whose indices are larger.
     const int64_t from = sp.cnt->list_end, n_list = sp.cnt->n_hot;
+    const uint32_t nwg = participants((n_old - from + TILE - 1) / TILE);
+    if (blockIdx.x >= nwg) return;
     for (int64_t j = from + (int64_t)blockIdx.x * TILE + threadIdx.x; j < n_old;
-         j += (int64_t)gridDim.x * TILE)
+         j += (int64_t)nwg * TILE)
       sp.hot[n_list + (j - from)] = (uint32_t)j;
-    kernel_exit(scan, [&]() {
+    kernel_exit(scan, nwg, [&]() {
       Counts* c = sp.cnt;
       c->n_hot = n_list + (n_old - from);
       c->list_end = n_old;
@@ -1099,6 +1133,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     return;
   }
   const int64_t ntiles = (n_old + WAKE_PAD - 1) / WAKE_PAD;
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
   const uint32_t epoch = next_epoch(scan);
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
@@ -1161,7 +1197,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     count_add(&sp.cnt->slept, tot[0]);
     count_add(&sp.cnt->asleep_n, tot[0]);  // zeroed by this step's prologue
   }
-  kernel_exit(scan, [&]() {
+  kernel_exit(scan, nwg, [&]() {
     uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
     sp.cnt->n_hot = (int64_t)total;
     sp.cnt->list_end = n_old;
@@ -1184,23 +1220,31 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
   __shared__ uint64_t sh_pref;
 
   const int64_t n_list = sp.cnt->n_hot;
+  const int64_t ntiles = (n_list + TILE - 1) / TILE;
+  // workgroups without work leave before reading the destination counts,
+  // which the exit of this launch changes
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
   const SpeciesDev* dA = sp.pair_e_dst >= 0 ? &P.sp[sp.pair_e_dst] : nullptr;
   const SpeciesDev* dB = sp.pair_p_dst >= 0 ? &P.sp[sp.pair_p_dst] : nullptr;
   if (!room_or_halt(P, s, a, dA, dB, n_list)) return;
   const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
   const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
-  const int64_t ntiles = (n_list + TILE - 1) / TILE;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
-  stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
+  bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
     const int64_t tile = sh_tile;
     __syncthreads();
     if (tile >= ntiles) break;
+    if (!staged) {  // tables go to LDS once the workgroup has work
+      stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
+      staged = true;
+    }
 
     const int64_t k = tile * TILE + threadIdx.x;
     bool live = false;
@@ -1302,7 +1346,7 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     count_add(&sp.cnt->sleeps, tot[3]);
     count_add(&sp.cnt->asleep_n, tot[3]);
   }
-  kernel_exit(scan, [&]() {
+  kernel_exit(scan, nwg, [&]() {
     finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
     scan->epoch = epoch;
   });
@@ -1876,14 +1920,14 @@ static int64_t sim_work(const Sim* sim, int s, int ph) {
 }
 
 // The tuner runs a batch again when the work of any kernel has changed by
-// more than a factor of two since it last did.
+// more than a factor of four since it last did.
 static bool sim_needs_tune(const Sim* sim) {
   if (!sim->tuned) return true;
   for (int s = 0; s < sim->nsp; ++s)
     for (int ph = 0; ph < 2; ++ph) {
       if (sim->cand[s][ph].size() < 2) continue;
       const int64_t w = sim_work(sim, s, ph), w0 = sim->tune_n[s][ph];
-      if (w > 2 * w0 || w0 > 2 * w) return true;
+      if (w > 4 * w0 || w0 > 4 * w) return true;
     }
   return false;
 }
