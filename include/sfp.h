@@ -321,7 +321,7 @@ int sfp_sim_destroy(void* sim);
 /* Online launch-shape tuner: a batch whose steps alternate between candidate
  * persistent grids of each kernel, timed with events, picks the fastest; the
  * tuner runs such a batch again when the work of a kernel changes by more
- * than a factor of two.  Results do not depend on the grid.  No counterpart
+ * than a factor of four.  Results do not depend on the grid.  No counterpart
  * in the reference. */
 #define SFP_RUN_AUTOTUNE 4
 int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t flags);
