@@ -63,6 +63,9 @@ typedef struct {
   const double* Aip;    /* [z_N] */
   int64_t z_N;
   double z_min, z_max, z_delta;
+  /* Ai(z) on the same grid, for the spin-resolved emission rate; may be NULL
+   * (no counterpart in the reference, whose emission ignores spin) */
+  const double* Ai;
 } sfp_rs_tables;
 
 /* Optical-depth tables, HOST pointers (qed/optical_depth_tables.py:14-32) */
@@ -303,6 +306,11 @@ typedef struct {
   uint64_t seed;
   int32_t qed_model;       /* SFP_QED_* */
   int32_t precision;       /* SFP_PRECISION_* */
+  /* 1: photon emission of spin-carrying leptons under rejection sampling uses
+   * the spin-resolved rate (needs the Ai table).  No counterpart in the
+   * reference (0, its behaviour: emission ignores spin, cpu.py:78-118). */
+  int32_t spin_emission;
+  int32_t pad;
 } sfp_sim_params;
 
 int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies, const sfp_field* terms,

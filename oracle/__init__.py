@@ -49,7 +49,7 @@ class OrcField(C.Structure):
 
 class OrcRSTables(C.Structure):
     _fields_ = [("int_Ai", P), ("Aip", P), ("z_N", I64), ("z_min", D), ("z_max", D),
-                ("z_delta", D)]
+                ("z_delta", D), ("Ai", P)]
 
 
 class OrcODTables(C.Structure):
@@ -84,6 +84,11 @@ def _declare(L):
     qed = [P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P]
     L.orc_photon_from_rejection_sampling.argtypes = qed
     L.orc_pair_from_rejection_sampling.argtypes = qed
+    L.orc_photon_from_rejection_sampling_spin.argtypes = qed + [D] + [P] * 12
+    L.orc_photon_prob_rate_spin.argtypes = [D, D, D, P]
+    L.orc_photon_prob_rate_spin.restype = D
+    L.orc_spin_sigma.argtypes = [D] * 14
+    L.orc_spin_sigma.restype = D
     tau = [P, P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P]
     L.orc_update_tau_e.argtypes = tau
     L.orc_update_tau_gamma.argtypes = tau
@@ -115,7 +120,8 @@ def field_array(terms):
 
 
 def rs_struct(T):
-    s = OrcRSTables(T.int_Ai.ctypes.data, T.Aip.ctypes.data, T.z_N, T.z_min, T.z_max, T.z_delta)
+    s = OrcRSTables(T.int_Ai.ctypes.data, T.Aip.ctypes.data, T.z_N, T.z_min, T.z_max, T.z_delta,
+                    T.Ai.ctypes.data)
     s._keep = T
     return s
 

@@ -573,6 +573,7 @@ typedef struct {
   const double* Aip;
   int64_t z_N;
   double z_min, z_max, z_delta;
+  const double* Ai; /* Ai(z) on the same grid, spin-resolved emission only */
 } orc_rs_tables;
 
 typedef struct {
@@ -644,6 +645,88 @@ void orc_photon_from_rejection_sampling(const double* inv_gamma, const double* c
     double dtau = dt * inv_gamma[ip];
     double r1_3 = o_powi(r1, 3);
     double prob_rate = 3 * o_powi(r1, 2) * orc_photon_prob_rate(chi_e[ip], r1_3, T) * dtau;
+    if (r2 < prob_rate) {
+      delta[ip] = r1_3;
+      event[ip] = 1;
+    } else {
+      delta[ip] = 0.0;
+      event[ip] = 0;
+    }
+  }
+}
+
+/* Spin-resolved emission, an addition with no counterpart in the reference
+ * (its emission ignores spin).  The LCFA emission probability summed over
+ * photon polarisation and final spin carries the initial-spin term
+ * -u (s.zeta) K_1/3(y) (u = delta, y = 2u/(3 chi (1-u))) beside the bracket
+ * (1 - u + 1/(1-u)) K_2/3(y) - int_y^inf K_1/3 that the reference's table
+ * formula restates in Airy form (divided by pi sqrt 3).  With
+ * K_1/3(y) = pi sqrt(3/z) Ai(z) the term becomes -sigma delta Ai(z)/sqrt z,
+ * written as -sigma chi_e' z Ai(z) since delta/sqrt z = chi_e' z. */
+double orc_photon_prob_rate_spin(double chi_e, double delta, double sigma,
+                                 const orc_rs_tables* T) {
+  double chi_gamma = delta * chi_e;
+  double chi_ep = chi_e - chi_gamma;
+  double z = O_POW(chi_gamma / chi_e / chi_ep, 2.0 / 3);
+  double factor = -O_ALPHA * O_ME * (O_C * O_C) / O_HBAR;
+  if (!(z <= T->z_max)) return (z != z) ? z : 0.0;
+  double int_Ai_, Aip_;
+  o_airy_lerp(T, z, &int_Ai_, &Aip_);
+  int64_t idx = (int64_t)((z - T->z_min) / T->z_delta);
+  if (idx > T->z_N - 2) idx = T->z_N - 2;
+  double z_left = T->z_min + (double)idx * T->z_delta;
+  double k = (T->Ai[idx + 1] - T->Ai[idx]) / T->z_delta;
+  double Ai_ = T->Ai[idx] + k * (z - z_left);
+  return factor * (int_Ai_ + (2.0 / z + chi_gamma * O_SQRT(z)) * Aip_ + sigma * chi_ep * z * Ai_);
+}
+
+/* sigma = s.zeta, zeta the unit vector along beta x dbeta/dt under the
+ * Lorentz force on a particle of charge sign q: since dbeta/dt is
+ * q/(gamma m c) (E + c beta x B - beta (beta.E)), beta x dbeta/dt points along
+ * sign(q) [gamma (u x E)/c + u x (u x B)]; 0 without a transverse force. */
+double orc_spin_sigma(double q, double ux, double uy, double uz, double inv_gamma, double Ex,
+                      double Ey, double Ez, double Bx, double By, double Bz, double sx, double sy,
+                      double sz) {
+  double gamma = 1.0 / inv_gamma;
+  double ex = uy * Ez - uz * Ey, ey = uz * Ex - ux * Ez, ez = ux * Ey - uy * Ex;
+  double bx = uy * Bz - uz * By, by = uz * Bx - ux * Bz, bz = ux * By - uy * Bx;
+  double zx = gamma * ex / O_C + (uy * bz - uz * by);
+  double zy = gamma * ey / O_C + (uz * bx - ux * bz);
+  double zz = gamma * ez / O_C + (ux * by - uy * bx);
+  if (q < 0.0) {
+    zx = -zx;
+    zy = -zy;
+    zz = -zz;
+  }
+  double n = O_SQRT(zx * zx + zy * zy + zz * zz);
+  if (!(n > 0.0)) return 0.0;
+  return (sx * zx + sy * zy + sz * zz) / n;
+}
+
+/* qed/rejection_sampling.py:68-85 with the spin-resolved rate (the same
+ * draws, the rate of orc_photon_prob_rate_spin at sigma = s.zeta from the
+ * fields and momentum of this step's QED pass). */
+void orc_photon_from_rejection_sampling_spin(
+    const double* inv_gamma, const double* chi_e, double dt, int64_t N, const uint8_t* pruned,
+    uint8_t* event, double* delta, const uint64_t* ids, uint64_t seed, uint64_t step,
+    uint32_t tag, uint32_t slot_base, const orc_rs_tables* T, double q, const double* ux,
+    const double* uy, const double* uz, const double* Ex, const double* Ey, const double* Ez,
+    const double* Bx, const double* By, const double* Bz, const double* sx, const double* sy,
+    const double* sz) {
+#pragma omp parallel for schedule(static)
+  for (int64_t ip = 0; ip < N; ++ip) {
+    if (pruned[ip] || chi_e[ip] == 0.0) {
+      event[ip] = 0;
+      continue;
+    }
+    sfp_u32x4 d = sfp_draw4(seed, ids[ip], step, tag, slot_base);
+    double r1 = sfp_u01(d.v[0], d.v[1]);
+    double r2 = sfp_u01(d.v[2], d.v[3]);
+    double dtau = dt * inv_gamma[ip];
+    double r1_3 = o_powi(r1, 3);
+    double sigma = orc_spin_sigma(q, ux[ip], uy[ip], uz[ip], inv_gamma[ip], Ex[ip], Ey[ip], Ez[ip],
+                                  Bx[ip], By[ip], Bz[ip], sx[ip], sy[ip], sz[ip]);
+    double prob_rate = 3 * o_powi(r1, 2) * orc_photon_prob_rate_spin(chi_e[ip], r1_3, sigma, T) * dtau;
     if (r2 < prob_rate) {
       delta[ip] = r1_3;
       event[ip] = 1;

@@ -129,8 +129,9 @@ class OSimulation:
     """simulation.py:12-161 on the oracle passes."""
 
     def __init__(self, *all_particles, dt, fields, photon_threshold=2.0, t0=0.0, qed="rs",
-                 seed=0, rs_tables=None, od_tables=None, libm=False):
+                 seed=0, rs_tables=None, od_tables=None, libm=False, spin_emission=False):
         self.all_particles = all_particles
+        self.spin_emission = bool(spin_emission)
         self.dt = dt
         self.terms = list(fields)
         self.fa = field_array(self.terms)
@@ -171,10 +172,17 @@ class OSimulation:
             fn(ptr(p.tau), ptr(p.inv_gamma), ptr(p.chi), dt, p.N_buffered, ptr(p._to_be_pruned),
                ptr(p.event), ptr(delta), ptr(p.id), self.seed, step, p.tag, slot_base, C.byref(self.od))
         else:
-            fn = (self.L.orc_photon_from_rejection_sampling if p.radiating
-                  else self.L.orc_pair_from_rejection_sampling)
-            fn(ptr(p.inv_gamma), ptr(p.chi), dt, p.N_buffered, ptr(p._to_be_pruned),
-               ptr(p.event), ptr(delta), ptr(p.id), self.seed, step, p.tag, slot_base, C.byref(self.rs))
+            args = (ptr(p.inv_gamma), ptr(p.chi), dt, p.N_buffered, ptr(p._to_be_pruned),
+                    ptr(p.event), ptr(delta), ptr(p.id), self.seed, step, p.tag, slot_base,
+                    C.byref(self.rs))
+            if p.radiating and p.spin and self.spin_emission:
+                self.L.orc_photon_from_rejection_sampling_spin(
+                    *args, float(p.q), ptr(p.ux), ptr(p.uy), ptr(p.uz), ptr(p.Ex), ptr(p.Ey),
+                    ptr(p.Ez), ptr(p.Bx), ptr(p.By), ptr(p.Bz), ptr(p.sx), ptr(p.sy), ptr(p.sz))
+            else:
+                fn = (self.L.orc_photon_from_rejection_sampling if p.radiating
+                      else self.L.orc_pair_from_rejection_sampling)
+                fn(*args)
 
     def pick_hard_photon(self, p, threshold):
         self.L.orc_pick_hard_photon(ptr(p.event), ptr(p.photon_delta), ptr(p.inv_gamma),

@@ -45,7 +45,8 @@ class Field(C.Structure):
 
 class RSTablesC(C.Structure):
     _fields_ = [("int_Ai", C.c_void_p), ("Aip", C.c_void_p), ("z_N", C.c_int64),
-                ("z_min", C.c_double), ("z_max", C.c_double), ("z_delta", C.c_double)]
+                ("z_min", C.c_double), ("z_max", C.c_double), ("z_delta", C.c_double),
+                ("Ai", C.c_void_p)]
 
 
 class ODTablesC(C.Structure):
@@ -67,7 +68,8 @@ class SpeciesDesc(C.Structure):
 
 class SimParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("photon_threshold", C.c_double), ("seed", C.c_uint64),
-                ("qed_model", C.c_int32), ("precision", C.c_int32)]
+                ("qed_model", C.c_int32), ("precision", C.c_int32),
+                ("spin_emission", C.c_int32), ("pad", C.c_int32)]
 
 
 P = C.c_void_p

@@ -624,6 +624,56 @@ SFP_HD double sfp_rs_rate(const sfp_rs_node* nodes, int64_t z_N, double z_min, d
   return factor * (int_Ai_ + (2.0 / z - chi_parent * sfp_sqrt(z)) * Aip_);
 }
 
+/* Spin-resolved photon emission rate (no counterpart in the reference, whose
+ * emission ignores spin): the rate above (photon branch) plus the term of the
+ * initial spin in the LCFA emission probability summed over photon
+ * polarisation and final spin, -u (s.zeta) K_1/3(y), y = 2u/(3 chi (1-u)),
+ * u = delta.  With K_1/3(y) = pi sqrt(3/z) Ai(z) and the Airy bracket of the
+ * table being the Bessel bracket / (pi sqrt 3), the term is
+ * -sigma delta Ai(z)/sqrt(z) = -sigma chi_e' z Ai(z) (delta/sqrt z = chi_e' z,
+ * finite at z = 0), sigma = s.zeta.  ai: per bracket, Ai at the left node
+ * and the slope. */
+SFP_HD double sfp_rs_rate_spin(const sfp_rs_node* nodes, const double2* ai, int64_t z_N,
+                               double z_min, double z_max, double z_delta, double chi_e,
+                               double delta, double sigma) {
+  double chi_a = delta * chi_e;
+  double chi_b = chi_e - chi_a;
+  double z = sfp_pow(chi_a / chi_e / chi_b, 2.0 / 3);
+  double factor = -SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR;
+  if (!(z <= z_max)) return (z != z) ? z : 0.0;
+  int64_t idx = (int64_t)((z - z_min) / z_delta);
+  if (idx > z_N - 2) idx = z_N - 2;
+  double z_left = z_min + (double)idx * z_delta;
+  sfp_rs_node nd = nodes[idx];
+  double2 an = ai[idx];
+  double int_Ai_ = nd.ia + nd.ka * (z - z_left);
+  double Aip_ = nd.ap + nd.kp * (z - z_left);
+  double Ai_ = an.x + an.y * (z - z_left);
+  return factor * (int_Ai_ + (2.0 / z + chi_a * sfp_sqrt(z)) * Aip_ + sigma * chi_b * z * Ai_);
+}
+
+/* sigma = s.zeta for the spin-resolved rate: zeta is the unit vector along
+ * beta x dbeta/dt under the Lorentz force, sign(q) [gamma (u x E)/c +
+ * u x (u x B)] (the component of the force along u drops out); 0 where that
+ * vanishes.  F = (Ex, Ey, Ez, Bx, By, Bz), ig = 1/gamma. */
+SFP_HD double sfp_spin_sigma(double q, double ux, double uy, double uz, double ig,
+                             const double* F, double sx, double sy, double sz) {
+  const double g = 1.0 / ig;
+  const double ex = uy * F[2] - uz * F[1], ey = uz * F[0] - ux * F[2], ez = ux * F[1] - uy * F[0];
+  const double bx = uy * F[5] - uz * F[4], by = uz * F[3] - ux * F[5], bz = ux * F[4] - uy * F[3];
+  double zx = g * ex / SFP_C + (uy * bz - uz * by);
+  double zy = g * ey / SFP_C + (uz * bx - ux * bz);
+  double zz = g * ez / SFP_C + (ux * by - uy * bx);
+  if (q < 0.0) {
+    zx = -zx;
+    zy = -zy;
+    zz = -zz;
+  }
+  const double n = sfp_sqrt(zx * zx + zy * zy + zz * zz);
+  if (!(n > 0.0)) return 0.0;
+  return (sx * zx + sy * zy + sz * zz) / n;
+}
+
 /* optical-depth tables as the kernels read them */
 typedef struct {
   const double* photon_total; /* [chi_N] */

@@ -279,6 +279,47 @@ __device__ inline float sfp32_rs_rate(const sfp_rs_node32* nodes, int z_N, float
   return factor * (ia + (2.0f / z - chi_parent * sz) * ap);
 }
 
+/* sfp_rs_rate_spin in FP32 (photon emission) */
+__device__ inline float sfp32_rs_rate_spin(const sfp_rs_node32* nodes, const float2* ai, int z_N,
+                                           float z_min, float z_max, float z_delta, float inv_dz,
+                                           float chi_parent, float delta, float sigma) {
+  const float chi_a = delta * chi_parent;
+  const float chi_b = chi_parent - chi_a;
+  const float q = chi_a / chi_parent / chi_b;
+  const float z = exp2f(2.0f / 3.0f * __log2f(q));
+  const float factor = (float)(SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR);
+  if (!(z <= z_max)) return (z != z) ? z : 0.0f;
+  int idx = (int)((z - z_min) * inv_dz);
+  if (idx > z_N - 2) idx = z_N - 2;
+  const float dz = z - (z_min + (float)idx * z_delta);
+  const sfp_rs_node32 nd = nodes[idx];
+  const float2 an = ai[idx];
+  const float ia = nd.ia + nd.ka * dz;
+  const float ap = nd.ap + nd.kp * dz;
+  const float a = an.x + an.y * dz;
+  return -factor * (ia + (2.0f / z + chi_a * sqrtf(z)) * ap + sigma * chi_b * z * a);
+}
+
+/* sfp_spin_sigma in FP32 */
+__device__ inline float sfp32_spin_sigma(double q, float ux, float uy, float uz, float ig,
+                                         const float* F, float sx, float sy, float sz) {
+  const float g = 1.0f / ig;
+  const float ic = (float)(1.0 / SFP_C);
+  const float ex = uy * F[2] - uz * F[1], ey = uz * F[0] - ux * F[2], ez = ux * F[1] - uy * F[0];
+  const float bx = uy * F[5] - uz * F[4], by = uz * F[3] - ux * F[5], bz = ux * F[4] - uy * F[3];
+  float zx = g * ex * ic + (uy * bz - uz * by);
+  float zy = g * ey * ic + (uz * bx - ux * bz);
+  float zz = g * ez * ic + (ux * by - uy * bx);
+  if (q < 0.0) {
+    zx = -zx;
+    zy = -zy;
+    zz = -zz;
+  }
+  const float n2 = zx * zx + zy * zy + zz * zz;
+  if (!(n2 > 0.0f)) return 0.0f;
+  return (sx * zx + sy * zy + sz * zz) * __frsqrt_rn(n2);
+}
+
 /* uniform in (0, 1) from 24 bits */
 __device__ inline float sfp32_u01(uint32_t w) { return ((float)(w >> 8) + 0.5f) * 0x1p-24f; }
 

@@ -53,6 +53,11 @@ struct Tables {
   float4* rs_nodes32 = nullptr;     // [z_N-1] the same in FP32 (sfp_rs_node32)
   // decreasing envelopes of the table rates over brackets j.. (see sfp_tables_create)
   double2* rs_env = nullptr;        // [z_N-1] {E1, E2}
+  // spin-resolved emission: Ai per bracket {value, slope}, in FP64 and FP32,
+  // and the envelopes with the spin term added to E2
+  double2* rs_ai = nullptr;
+  float2* rs_ai32 = nullptr;
+  double2* rs_env_spin = nullptr;
   int64_t z_N = 0;
   double z_min = 0, z_max = 0, z_delta = 0;
   bool has_rs = false;

@@ -661,6 +661,30 @@ extern "C" int sfp_tables_create(const sfp_rs_tables* rs, const sfp_od_tables* o
     }
     HIPCHK(hipMalloc(&T->rs_env, sizeof(double2) * env.size()));
     HIPCHK(hipMemcpy(T->rs_env, env.data(), sizeof(double2) * env.size(), hipMemcpyHostToDevice));
+    if (rs->Ai) {
+      // Ai nodes for the spin-resolved rate; its spin term is bounded on
+      // bracket j by chi * z_j+1 * max Ai (Ai > 0 and the interpolation stays
+      // between node values), added to E2 as a suffix maximum
+      std::vector<double2> ai(rs->z_N - 1);
+      std::vector<float2> ai32(rs->z_N - 1);
+      std::vector<double2> envs(rs->z_N - 1);
+      double e3 = 0.0;
+      for (int64_t j = rs->z_N - 2; j >= 0; --j) {
+        ai[j] = make_double2(rs->Ai[j], (rs->Ai[j + 1] - rs->Ai[j]) / rs->z_delta);
+        ai32[j] = make_float2((float)ai[j].x, (float)ai[j].y);
+        double zj1 = rs->z_min + (double)(j + 1) * rs->z_delta;
+        e3 = std::max(e3, zj1 * std::max(fabs(rs->Ai[j]), fabs(rs->Ai[j + 1])));
+        envs[j] = make_double2(env[j].x, env[j].y + e3);
+      }
+      HIPCHK(hipMalloc(&T->rs_ai, sizeof(double2) * ai.size()));
+      HIPCHK(hipMemcpy(T->rs_ai, ai.data(), sizeof(double2) * ai.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&T->rs_ai32, sizeof(float2) * ai32.size()));
+      HIPCHK(hipMemcpy(T->rs_ai32, ai32.data(), sizeof(float2) * ai32.size(),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&T->rs_env_spin, sizeof(double2) * envs.size()));
+      HIPCHK(hipMemcpy(T->rs_env_spin, envs.data(), sizeof(double2) * envs.size(),
+                       hipMemcpyHostToDevice));
+    }
     T->z_N = rs->z_N;
     T->z_min = rs->z_min;
     T->z_max = rs->z_max;
@@ -718,6 +742,9 @@ extern "C" int sfp_tables_destroy(void* tables) {
   if (T->rs_nodes) (void)hipFree(T->rs_nodes);
   if (T->rs_nodes32) (void)hipFree(T->rs_nodes32);
   if (T->rs_env) (void)hipFree(T->rs_env);
+  if (T->rs_ai) (void)hipFree(T->rs_ai);
+  if (T->rs_ai32) (void)hipFree(T->rs_ai32);
+  if (T->rs_env_spin) (void)hipFree(T->rs_env_spin);
   if (T->od_mem) (void)hipFree(T->od_mem);
   if (T->od_photon_nodes) (void)hipFree(T->od_photon_nodes);
   if (T->od_pair_nodes) (void)hipFree(T->od_pair_nodes);

@@ -74,6 +74,10 @@ struct Plan {
   const sfp_rs_node* rs_nodes;
   const sfp_rs_node32* rs_nodes32;  // the same nodes in FP32 (mixed precision)
   const double2* rs_env;  // rate envelopes (Tables::rs_env)
+  const double2* rs_ai;        // spin-resolved emission: Ai nodes, FP64 and FP32,
+  const float2* rs_ai32;       // and the envelopes with the spin term
+  const double2* rs_env_spin;
+  int32_t spin_emission;       // leptons with spin use the spin-resolved rate (RS)
   double rate_unit;       // alpha m c^2 / hbar
   int64_t z_N;
   double z_min, z_max, z_delta;
@@ -304,7 +308,7 @@ __device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const S
 // below 2^-53 no event is possible.  Optical depth: skipped below chi_min
 // (optical_depth.py:43), so only that test is exact.
 __device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
-                                    double dt) {
+                                    double dt, bool spin = false) {
   if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
   float num;
   if (photon) {
@@ -322,7 +326,7 @@ __device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, dou
   if (!(fj >= 1.0f)) return true;                       // first bracket: 2/z unbounded
   int64_t j = (int64_t)fj;
   if (j > P.z_N - 2) return false;
-  double2 e = P.rs_env[j];
+  double2 e = spin ? P.rs_env_spin[j] : P.rs_env[j];
   double pbound = P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
   return !(2.0 * pbound < 0x1p-53);
 }
@@ -524,7 +528,7 @@ __global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, i
         // bound and this product carry a small margin for their own rounding
         float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
         float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
-        hot = qed_possible(P, PHOTON, chi_b, ig, a.dt);
+        hot = qed_possible(P, PHOTON, chi_b, ig, a.dt, !PHOTON && P.spin_emission && sp.spin);
       }
       if (QED) {
         uint64_t m = __ballot(hot);
@@ -656,7 +660,8 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
                                  const StepArgs& a,
                                  const sfp_rs_node* lds_rs, const double2* lds_od, double x,
                                  double y, double z, double ux, double uy, double uz, double ig,
-                                 uint64_t pid, double& tau, double& delta) {
+                                 uint64_t pid, double& tau, double& delta, double sx = 0.0,
+                                 double sy = 0.0, double sz = 0.0) {
   bool event = false;
   delta = 0.0;
   double F[6];
@@ -670,8 +675,14 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
       double dtau = a.dt * ig;
       if (!PHOTON) {
         double r1_3 = sfp_powi(r1, 3);
-        double p = 3 * sfp_powi(r1, 2) *
-                   sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1) * dtau;
+        double w;
+        if (P.spin_emission && sp.spin) {
+          const double sig = sfp_spin_sigma(sp.q, ux, uy, uz, ig, F, sx, sy, sz);
+          w = sfp_rs_rate_spin(lds_rs, P.rs_ai, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, sig);
+        } else {
+          w = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1);
+        }
+        double p = 3 * sfp_powi(r1, 2) * w * dtau;
         if (r2 < p) { event = true; delta = r1_3; }
       } else {
         double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
@@ -735,7 +746,8 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
                                    const StepArgs& a,
                                    const sfp_rs_node32* lds_rs, const double2* lds_od, double x,
                                    double y, double z, float ux, float uy, float uz, float ig,
-                                   uint64_t pid, double& tau, double& delta) {
+                                   uint64_t pid, double& tau, double& delta, float sx = 0.0f,
+                                   float sy = 0.0f, float sz = 0.0f) {
   bool event = false;
   delta = 0.0;
   float F[6];
@@ -751,9 +763,14 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
       const float izd = (float)(1.0 / P.z_delta);
       if (!PHOTON) {
         const float r1_3 = r1 * r1 * r1;
-        const float p = 3.0f * (r1 * r1) *
-                        sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1_3, true) *
-                        dtau;
+        float w;
+        if (P.spin_emission && sp.spin) {
+          const float sig = sfp32_spin_sigma(sp.q, ux, uy, uz, ig, F, sx, sy, sz);
+          w = sfp32_rs_rate_spin(lds_rs, P.rs_ai32, (int)P.z_N, zmin, zmax, zd, izd, chif, r1_3, sig);
+        } else {
+          w = sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1_3, true);
+        }
+        const float p = 3.0f * (r1 * r1) * w * dtau;
         if (r2 < p) { event = true; delta = r1_3; }
       } else {
         const float p =
@@ -1022,10 +1039,11 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
     } else if (valid) {
       if (MIX) {
         event = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
-                                             z, fx, fy, fz, igf, pid, tau, delta);
+                                             z, fx, fy, fz, igf, pid, tau, delta, (float)sx,
+                                             (float)sy, (float)sz);
       } else {
         event = qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
-                                           ux, uy, uz, ig, pid, tau, delta);
+                                           ux, uy, uz, ig, pid, tau, delta, sx, sy, sz);
       }
     }
     if (!dA) event = false;
@@ -1661,12 +1679,22 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
     P.rs_nodes = sim->tables->rs_nodes;
     P.rs_nodes32 = (const sfp_rs_node32*)sim->tables->rs_nodes32;
     P.rs_env = sim->tables->rs_env;
+    P.rs_ai = sim->tables->rs_ai;
+    P.rs_ai32 = sim->tables->rs_ai32;
+    P.rs_env_spin = sim->tables->rs_env_spin;
     P.rate_unit = SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR;
     P.z_N = sim->tables->z_N;
     P.z_min = sim->tables->z_min;
     P.z_max = sim->tables->z_max;
     P.z_delta = sim->tables->z_delta;
     P.zmax15 = sim->tables->z_max * sqrt(sim->tables->z_max);
+  }
+  if (params->spin_emission) {
+    if (params->qed_model != SFP_QED_RS)
+      return set_err(SFP_E_ARG, "spin-resolved emission is defined for rejection sampling");
+    if (!sim->tables || !sim->tables->rs_ai)
+      return set_err(SFP_E_ARG, "spin-resolved emission needs the Ai table");
+    P.spin_emission = 1;
   }
   if (any_qed && params->qed_model == SFP_QED_OD) {
     if (!sim->tables || !sim->tables->has_od)
@@ -1712,6 +1740,11 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   if (sim->params.precision != SFP_PRECISION_MIXED)
     return set_err(SFP_E_ARG, "the learned sampler runs in the mixed-precision step");
+  if (mlp && sim->plan.spin_emission)
+    for (int s = 0; s < sim->nsp; ++s)
+      if (sim->plan.sp[s].spin && sim->desc[s].photon_dst >= 0)
+        return set_err(SFP_E_ARG, "the learned sampler draws the unpolarised spectrum: not with "
+                                  "spin-resolved emission");
   if (!sim->tables || !sim->tables->has_od || !sim->plan.od_photon_nodes)
     return set_err(SFP_E_ARG, "the learned sampler needs the optical-depth tables");
   Plan& P = sim->plan;

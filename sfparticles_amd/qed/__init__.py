@@ -49,7 +49,7 @@ def device_tables():
         rs = load_rs_tables()
         od = load_od_tables()
         rsc = L.RSTablesC(rs.int_Ai.ctypes.data, rs.Aip.ctypes.data, rs.z_N, rs.z_min, rs.z_max,
-                          rs.z_delta)
+                          rs.z_delta, rs.Ai.ctypes.data)
         odc = L.ODTablesC(od.photon_total.ctypes.data, od.pair_total.ctypes.data,
                           od.photon_cdf.ctypes.data, od.pair_cdf.ctypes.data, od.chi_N,
                           od.log_chi_min, od.log_chi_max, od.log_chi_delta, od.delta_N,

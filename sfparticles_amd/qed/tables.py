@@ -80,6 +80,14 @@ def rs_table_gen(z_N=RS_Z_N, z_min=RS_Z_MIN, z_max=RS_Z_MAX, processes=None):
     return ia, ap, attrs
 
 
+def rs_ai_gen(z_N=RS_Z_N, z_min=RS_Z_MIN, z_max=RS_Z_MAX):
+    """Ai(z) on the rejection-sampling grid: the spin term of the emission
+    rate, -alpha m c^2/hbar * sigma * delta * Ai(z) / sqrt(z) (no counterpart
+    in the reference)."""
+    z = np.linspace(z_min, z_max, z_N)
+    return np.asarray(_Ai(z), dtype=np.float64)
+
+
 def photon_rate_for_delta(chi_e):
     """dW/d(delta) per unit proper time for photon emission at fixed chi_e."""
     factor = -K.alpha * K.m_e * K.c ** 2 / K.hbar
@@ -169,9 +177,12 @@ def od_table_gen(chi_N=OD_CHI_N, log_chi_min=OD_LOG_CHI_MIN, log_chi_max=OD_LOG_
 class RSTables:
     """Rejection-sampling tables as the kernels consume them."""
 
-    def __init__(self, int_Ai, Aip, z_N, z_min, z_max, z_delta):
+    def __init__(self, int_Ai, Aip, z_N, z_min, z_max, z_delta, Ai=None):
         self.int_Ai = np.ascontiguousarray(int_Ai, dtype=np.float64)
         self.Aip = np.ascontiguousarray(Aip, dtype=np.float64)
+        if Ai is None:
+            Ai = rs_ai_gen(int(z_N), float(z_min), float(z_max))
+        self.Ai = np.ascontiguousarray(Ai, dtype=np.float64)
         self.z_N = int(z_N)
         self.z_min = float(z_min)
         self.z_max = float(z_max)
@@ -216,7 +227,7 @@ def load_rs_tables():
     if _rs_cache is None:
         d = np.load(os.path.join(TABLE_DIR, "rs_tables.npz"))
         _rs_cache = RSTables(d["int_Ai"], d["Aip"], d["z_N"], d["z_range"][0], d["z_range"][1],
-                             d["z_delta"])
+                             d["z_delta"], d["Ai"] if "Ai" in d else None)
     return _rs_cache
 
 
@@ -238,7 +249,8 @@ def main():
     os.makedirs(TABLE_DIR, exist_ok=True)
     ia, ap, a = rs_table_gen()
     np.savez(os.path.join(TABLE_DIR, "rs_tables.npz"), int_Ai=ia, Aip=ap, z_N=a["z_N"],
-             z_range=np.asarray(a["z_range"]), z_delta=a["z_delta"])
+             z_range=np.asarray(a["z_range"]), z_delta=a["z_delta"],
+             Ai=rs_ai_gen(RS_Z_N, RS_Z_MIN, RS_Z_MAX))
     print("rejection-sampling tables written", flush=True)
     pt, qt, pc, qc, a = od_table_gen()
     np.savez(os.path.join(TABLE_DIR, "od_tables.npz"), photon_prob_rate_total=pt,

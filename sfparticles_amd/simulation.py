@@ -32,7 +32,7 @@ class Simulation(object):
     def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
                  t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None,
                  precision: str = "fp64", sampler=None, graph: bool = False,
-                 autotune: bool = False, shots=None) -> None:
+                 autotune: bool = False, shots=None, spin_emission: bool = False) -> None:
         """precision: "fp64" (the reference's arithmetic, bit-identical to the CPU
         restatement) or "mixed" (fields, pushers and QED sampling in FP32;
         positions, time and chi in FP64).  sampler (mixed only): a
@@ -43,7 +43,11 @@ class Simulation(object):
         shots: an ensemble of independent shots in one simulation, a list of
         Fields, one per shot; every particle moves in the field of its
         ``shot`` index (species without one get shot 0 for their particles)
-        and what it creates inherits the index.  ``fields`` is then unused."""
+        and what it creates inherits the index.  ``fields`` is then unused.
+        spin_emission: photon emission of SpinParticles (rejection sampling)
+        uses the spin-resolved rate, which depends on the projection of the
+        spin on the direction of beta x dbeta/dt; the reference's emission
+        ignores spin."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         if sampler is not None and precision != "mixed":
@@ -55,6 +59,7 @@ class Simulation(object):
             sampler = default_sampler()
         self.precision = precision
         self.sampler = sampler
+        self.spin_emission = bool(spin_emission)
         self.graph = bool(graph)
         self.autotune = bool(autotune)
         self.shots = None if shots is None else list(shots)
@@ -126,7 +131,7 @@ class Simulation(object):
             d.pair_p_dst = idx[id(p.bw_positron_ref())] if p.bw else -1
         params = L.SimParams(float(self.dt), float(self.photon_threshold), int(self.seed),
                              L.QED_OD if self.qed_model == "od" else L.QED_RS,
-                             1 if self.precision == "mixed" else 0)
+                             1 if self.precision == "mixed" else 0, int(self.spin_emission), 0)
         arr = terms_to_c(terms)
         h = C.c_void_p()
         L.call("sfp_sim_create", descs, n, arr, len(terms), qed.device_tables(), C.byref(params),

@@ -129,7 +129,7 @@ def _collect(sp):
     return out
 
 
-def run_oracle_case(case, nsteps=None, chunks=1, libm=False):
+def run_oracle_case(case, nsteps=None, chunks=1, libm=False, spin_emission=False):
     import oracle
     from oracle.sim import OParticles, OSimulation
     from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
@@ -156,7 +156,7 @@ def run_oracle_case(case, nsteps=None, chunks=1, libm=False):
             g.set_pair(e, p, od=od)
     sim = OSimulation(*sp, dt=case["dt"], fields=case["field"], photon_threshold=case["threshold"],
                       qed=case["qed"], seed=case["seed"], rs_tables=load_rs_tables(),
-                      od_tables=load_od_tables(), libm=libm)
+                      od_tables=load_od_tables(), libm=libm, spin_emission=spin_emission)
     steps = case["nsteps"] if nsteps is None else nsteps
     per = steps // chunks
     done = 0

@@ -39,7 +39,7 @@ def test_shot_index_without_field_is_rejected():
     from sfparticles_amd.fields import Fields
     e = sfa.Particles("electron", -1, 1, N=4, props=tuple(np.ones(4) for _ in range(6)))
     e.shot = np.array([0, 1, 2, 1], np.uint32)
-    f = Fields(terms=[(4, (0, 0, 0, 0, 0, 1.0))])
+    f = Fields(terms=[(0, (0, 0, 0, 0, 0, 1.0))])
     with pytest.raises(ValueError, match="shot index 2"):
         sfa.Simulation(e, dt=1e-17, fields=None, shots=[f, f], print_every=None)
 
