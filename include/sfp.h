@@ -360,6 +360,13 @@ int sfp_sim_set_shots(void* sim, const sfp_field* terms, const int32_t* nterms, 
  * the optical-depth tables.  The network must outlive its use by the
  * simulation.  No counterpart in the reference. */
 int sfp_sim_set_sampler(void* sim, void* mlp);
+/* Use the learned spin-resolved emission rate (a network from
+ * sfp_mlp_create_rate) in place of the rate table for the photon emission of
+ * spin-carrying leptons, or stop using it (NULL).  Mixed precision,
+ * rejection sampling and spin_emission only; bins of log10 chi flagged for
+ * fallback, and chi outside the network's range, use the table.  No
+ * counterpart in the reference. */
+int sfp_sim_set_rate_net(void* sim, void* mlp);
 
 /* Counters of one species since the last reset (synchronous), stats[13]:
  * [0] particle-steps (live particles after each step, the reference's
@@ -376,26 +383,42 @@ int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 
 /* ---------------------------------------------------------------------------
- * 4. Learned photon-energy sampler (no counterpart in the reference)
+ * 4. Learned QED models (no counterpart in the reference)
  *
- * An MLP (log10 chi, r) -> log delta, 2 -> 64 -> 64 -> 64 -> 1 with SiLU, BF16
- * matrix-core products with FP32 accumulation, trained on the GPU on samples
- * of the optical-depth photon-emission inverse CDF (the table of
- * qed/optical_depth_tables.py:59-101, which it replaces in the mixed-precision
- * step and which stays its fallback per bin of log10 chi).
+ * Photon-energy sampler: an MLP (log10 chi, r) -> log delta,
+ * 2 -> 64 -> 64 -> 64 -> 1 with SiLU, BF16 matrix-core products with FP32
+ * accumulation, trained on the GPU on samples of the optical-depth
+ * photon-emission inverse CDF (the table of qed/optical_depth_tables.py:59-101,
+ * which it replaces in the mixed-precision step and which stays its fallback
+ * per bin of log10 chi).
  * Parameters: SFP_MLP_NPARAM floats, W1[64][2] b1 W2[64][64] b2 W3[64][64] b3
  * W4[64] b4; norm[4] = {lc_min, lc_max, mu, sd}: inputs (log10 chi - centre) /
  * half-width and 2 r - 1, output (log delta - mu) / sd.
+ *
+ * Spin-resolved emission rate: the same network with three inputs
+ * (log10 chi, r1, sigma = s.zeta) -> log(f + 1e-6), f = 3 r1^2 w(chi, r1^3,
+ * sigma) / (alpha m c^2 / hbar) the rejection-sampling acceptance density of
+ * qed/rejection_sampling.py:68-85 with the spin-resolved rate;
+ * SFP_MLP_NPARAM_RATE parameters, W1[64][3]; inputs as above plus
+ * sigma unscaled.  Created by sfp_mlp_create_rate; every sfp_mlp_* call below
+ * takes either kind (x arrays have 2 or 3 columns), except sfp_mlp_sample
+ * (sampler) and sfp_mlp_rate (rate).
  * ------------------------------------------------------------------------- */
 #define SFP_MLP_NPARAM 8577
+#define SFP_MLP_NPARAM_RATE 8641
 int sfp_mlp_create(void** mlp);
+int sfp_mlp_create_rate(void** mlp);
+/* number of inputs (2 or 3) and of parameters of a network */
+int sfp_mlp_shape(void* mlp, int32_t* nin, int32_t* nparam);
 int sfp_mlp_destroy(void* mlp);
 int sfp_mlp_set_params(void* mlp, const float* params, const float* norm);
 int sfp_mlp_get_params(void* mlp, float* params, float* norm);
 /* bit i set: bin i of 64 equal bins of log10 chi over [lc_min, lc_max] uses the table */
 int sfp_mlp_set_fallback(void* mlp, uint64_t mask);
 /* n samples: x_dev [n][2] = (log10 chi ~ U(lc_min, lc_max), r ~ U(0,1)) from
- * Philox(seed), y_dev [n] = log delta of the table's inverse CDF (synchronous) */
+ * Philox(seed), y_dev [n] = log delta of the table's inverse CDF; for the rate
+ * network x_dev [n][3] adds sigma ~ U(-1, 1) and y_dev is log f of the
+ * spin-resolved rate table (synchronous) */
 int sfp_mlp_make_data(void* mlp, void* tables, int64_t n, uint64_t seed, float* x_dev,
                       float* y_dev);
 /* `steps` Adam steps on batches of `batch` samples (a multiple of 32) taken in
@@ -413,6 +436,11 @@ int sfp_mlp_forward(void* mlp, const float* x_dev, float* out_dev, int64_t n);
  * inverse CDF (method 1); *ms = kernel time (synchronous) */
 int sfp_mlp_sample(void* mlp, void* tables, const float* x_dev, int64_t n, int32_t method,
                    double* delta_dev, float* ms);
+/* f = 3 r1^2 w(chi, r1^3, sigma) / (alpha m c^2 / hbar) for raw inputs x_dev
+ * [n][3] by the rate network (method 0, FP32) or the spin-resolved rate table
+ * (method 1, FP64); *ms = kernel time (synchronous) */
+int sfp_mlp_rate(void* mlp, void* tables, const float* x_dev, int64_t n, int32_t method,
+                 double* f_dev, float* ms);
 /* device copy of the network and of its fallback bins, for the simulation */
 int sfp_mlp_device(void* mlp, const void** dev, const void** bins);
 

@@ -1,38 +1,49 @@
-"""CPU oracle of the learned photon-energy sampler (test infrastructure).
+"""CPU oracle of the learned QED models (test infrastructure).
 
-The network has no counterpart in the reference: it is defined by the
-mathematics it implements, (log10 chi, r) -> log delta,
-2 -> 64 -> 64 -> 64 -> 1 with SiLU, mean squared error on the normalised
-output.  This module restates its forward pass and its gradient in numpy
+The networks have no counterpart in the reference: they are defined by the
+mathematics they implement, NIN -> 64 -> 64 -> 64 -> 1 with SiLU (NIN = 2:
+the photon-energy sampler (log10 chi, r) -> log delta; NIN = 3: the
+emission rate (log10 chi, r1, sigma) -> log f), mean squared error on the
+normalised output.  This module restates its forward pass and its gradient in numpy
 (float64), optionally rounding the operands of the two 64 x 64 products to
 BF16 exactly where the GPU feeds them to the matrix cores, so that the GPU's
 forward and backward kernels can be compared with it.  The gradient itself is
 pinned by central finite differences of the float64 loss (tests).
 
-Parameter layout (flat, float32 on the device): W1[64][2] b1[64] W2[64][64]
+Parameter layout (flat, float32 on the device): W1[64][NIN] b1[64] W2[64][64]
 b2[64] W3[64][64] b3[64] W4[64] b4.
 """
 import numpy as np
 
-NPARAM = 64 * 2 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
-SLICES = {}
-_o = 0
-for _name, _shape in (("W1", (64, 2)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)),
-                      ("W3", (64, 64)), ("b3", (64,)), ("W4", (64,)), ("b4", (1,))):
-    _n = int(np.prod(_shape))
-    SLICES[_name] = (_o, _o + _n, _shape)
-    _o += _n
-assert _o == NPARAM
+def _slices(nin):
+    out, o = {}, 0
+    for name, shape in (("W1", (64, nin)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)),
+                        ("W3", (64, 64)), ("b3", (64,)), ("W4", (64,)), ("b4", (1,))):
+        n = int(np.prod(shape))
+        out[name] = (o, o + n, shape)
+        o += n
+    return out, o
+
+
+SLICES, NPARAM = _slices(2)
+SLICES3, NPARAM3 = _slices(3)
+assert NPARAM == 8577 and NPARAM3 == 8641
+
+
+def _sl(n):
+    return SLICES if n == NPARAM else SLICES3
 
 
 def unpack(flat):
     flat = np.asarray(flat, np.float64)
-    return {k: flat[a:b].reshape(s) for k, (a, b, s) in SLICES.items()}
+    return {k: flat[a:b].reshape(s) for k, (a, b, s) in _sl(flat.size).items()}
 
 
 def pack(P):
-    out = np.zeros(NPARAM)
-    for k, (a, b, s) in SLICES.items():
+    nin = np.asarray(P["W1"]).shape[1]
+    sl = SLICES if nin == 2 else SLICES3
+    out = np.zeros(NPARAM if nin == 2 else NPARAM3)
+    for k, (a, b, s) in sl.items():
         out[a:b] = np.asarray(P[k], np.float64).reshape(-1)
     return out
 
@@ -55,11 +66,13 @@ def dsilu(z):
 
 
 def normalise(x, norm):
-    """raw (log10 chi, r) -> network inputs; norm = (lc_min, lc_max, mu, sd)"""
+    """raw (log10 chi, r[, sigma]) -> network inputs; norm = (lc_min, lc_max,
+    mu, sd); sigma passes unscaled"""
     lc_min, lc_max = norm[0], norm[1]
     x = np.asarray(x, np.float64)
     x0 = (x[:, 0] - 0.5 * (lc_min + lc_max)) * (2.0 / (lc_max - lc_min))
-    return np.stack([x0, 2.0 * x[:, 1] - 1.0], 1)
+    cols = [x0, 2.0 * x[:, 1] - 1.0] + ([x[:, 2]] if x.shape[1] == 3 else [])
+    return np.stack(cols, 1)
 
 
 def forward(flat, X, emulate=True):
@@ -98,10 +111,10 @@ def loss_grad(flat, X, t, emulate=True):
     return loss, pack(G)
 
 
-def init_params(seed=0):
+def init_params(seed=0, nin=2):
     """He-style initial weights, zero biases."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    P = {"W1": rng.normal(0, 1 / np.sqrt(2), (64, 2)), "b1": np.zeros(64),
+    P = {"W1": rng.normal(0, 1 / np.sqrt(nin), (64, nin)), "b1": np.zeros(64),
          "W2": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b2": np.zeros(64),
          "W3": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b3": np.zeros(64),
          "W4": rng.normal(0, 1 / np.sqrt(64), 64), "b4": np.zeros(1)}

@@ -112,6 +112,10 @@ SIGNATURES = {
     "sfp_math_eval": [I32, P, P, P, I64, P],
     "sfp_field_bound_eval": [P, I32, P, P, P, D, P, P, P, D, I64, P, P],
     "sfp_mlp_create": [C.POINTER(P)],
+    "sfp_mlp_create_rate": [C.POINTER(P)],
+    "sfp_mlp_shape": [P, C.POINTER(I32), C.POINTER(I32)],
+    "sfp_mlp_rate": [P, P, P, I64, I32, P, P],
+    "sfp_sim_set_rate_net": [P, P],
     "sfp_mlp_destroy": [P],
     "sfp_mlp_set_params": [P, P, P],
     "sfp_mlp_get_params": [P, P, P],

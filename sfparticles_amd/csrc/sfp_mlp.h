@@ -1,15 +1,23 @@
 /*
- * sfp_mlp.h -- the learned photon-energy sampler: an MLP
- * (log10 chi, r) -> log delta, 2 -> 64 -> 64 -> 64 -> 1 with SiLU, that
- * replaces the optical-depth inverse-CDF lookup (qed/optical_depth_tables.py:
- * 59-101) for photon emission.  The two 64 x 64 layers run on the matrix
- * cores (v_mfma_f32_16x16x32_bf16: BF16 inputs, FP32 accumulation) in tiles
- * of 16 samples per wave; the 2 -> 64 and 64 -> 1 layers are plain FP32.
+ * sfp_mlp.h -- the learned QED models, MLPs NIN -> 64 -> 64 -> 64 -> 1 with
+ * SiLU:
+ *  - the photon-energy sampler (NIN = 2): (log10 chi, r) -> log delta, which
+ *    replaces the optical-depth inverse-CDF lookup (qed/optical_depth_tables.py:
+ *    59-101) for photon emission;
+ *  - the spin-resolved emission rate (NIN = 3): (log10 chi, r1, sigma) ->
+ *    log(f + SFP_RATE_EPS), f = 3 r1^2 w(chi, r1^3, sigma) / (alpha m c^2 /
+ *    hbar) the rejection-sampling acceptance density, in place of the rate
+ *    table lookup.
+ * The two 64 x 64 layers run on the matrix cores (v_mfma_f32_16x16x32_bf16:
+ * BF16 inputs, FP32 accumulation) in tiles of 16 samples per wave; the
+ * NIN -> 64 and 64 -> 1 layers are plain FP32.
  *
- * Parameters, FP32 master copy, flat (SFP_MLP_NPARAM floats):
- *   W1[64][2] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
+ * Parameters, FP32 master copy, flat (SFP_MLP_NPARAM floats for NIN = 2,
+ * SFP_MLP_NPARAM_RATE for NIN = 3):
+ *   W1[64][NIN] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
  * Inputs are normalised x0 = (log10 chi - 0.5 (lc_min + lc_max)) /
- * (0.5 (lc_max - lc_min)), x1 = 2 r - 1; the output is (log delta - mu) / sd.
+ * (0.5 (lc_max - lc_min)), x1 = 2 r - 1, x2 = sigma; the output is
+ * (target - mu) / sd.
  *
  * MFMA operand layout (16 x 16 x 32, 64 lanes, lane = 16 g + c, g = 0..3):
  *   A (16 x 32): lane holds A[c][8 g .. 8 g + 7]
@@ -26,30 +34,28 @@
 #include "../../include/sfp.h"
 #include "sfp_device.h"
 
-static_assert(SFP_MLP_NPARAM == 64 * 2 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1,
-              "parameter count of the 2-64-64-64-1 network");
-enum {
-  MLP_W1 = 0,
-  MLP_B1 = MLP_W1 + 128,
-  MLP_W2 = MLP_B1 + 64,
-  MLP_B2 = MLP_W2 + 4096,
-  MLP_W3 = MLP_B2 + 64,
-  MLP_B3 = MLP_W3 + 4096,
-  MLP_W4 = MLP_B3 + 64,
-  MLP_B4 = MLP_W4 + 64,
+// offsets of the flat parameter blocks of a network with NIN inputs
+template <int NIN>
+struct MlpOff {
+  static constexpr int W1 = 0, B1 = 64 * NIN, W2 = B1 + 64, B2 = W2 + 4096, W3 = B2 + 64,
+                       B3 = W3 + 4096, W4 = B3 + 64, B4 = W4 + 64, N = B4 + 1;
 };
+static_assert(SFP_MLP_NPARAM == MlpOff<2>::N, "parameter count of the 2-64-64-64-1 network");
+static_assert(SFP_MLP_NPARAM_RATE == MlpOff<3>::N, "parameter count of the 3-64-64-64-1 network");
 
 typedef __bf16 sfp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float sfp_f32x4 __attribute__((ext_vector_type(4)));
 
 /* Device copy used by the kernels: the two square layers in BF16 (row-major
- * [out][in]), everything else FP32, plus the normalisation constants. */
+ * [out][in]), everything else FP32, plus the normalisation constants.  The
+ * first layer is stored as W1[64][2] for the first two inputs and W1c[64]
+ * for the third (0 for NIN = 2). */
 typedef struct {
   __bf16 W2[64 * 64];
   __bf16 W3[64 * 64];
   __bf16 W2p[64 * 64];  // input columns permuted to the register order of sfp_mlp_eval16
   __bf16 W3p[64 * 64];
-  float W1[128], b1[64], b2[64], b3[64], W4[64], b4;
+  float W1[128], W1c[64], b1[64], b2[64], b3[64], W4[64], b4;
   float x0_shift, x0_scale, out_mu, out_sd;
   float pad[3];
 } sfp_mlp_dev;
@@ -111,16 +117,20 @@ __device__ inline void sfp_mlp_layer16(const float* in, const __bf16* W, const f
   __builtin_amdgcn_wave_barrier();
 }
 
-/* Forward pass of 16 samples by one wave.  x0[r], x1[r] are the normalised
- * inputs of row r (rows >= n are padding); z1, z2 and z3 (16 x 64 FP32 each,
- * LDS) receive the pre-activations; returns the normalised output of row
- * (lane & 15) in every lane. */
+/* Forward pass of 16 samples by one wave.  x0[r], x1[r] (and x2[r] for
+ * NIN = 3) are the normalised inputs of row r (rows >= n are padding); z1, z2
+ * and z3 (16 x 64 FP32 each, LDS) receive the pre-activations; returns the
+ * normalised output of row (lane & 15) in every lane. */
+template <int NIN = 2>
 __device__ inline float sfp_mlp_forward16(const sfp_mlp_dev* M, const float* x0, const float* x1,
-                                          float* z1, float* z2, float* z3) {
+                                          const float* x2, float* z1, float* z2, float* z3) {
   const int l = __lane_id();
-  // layer 1 (2 -> 64) on the vector ALU: lane l computes units l of all rows
-  for (int r = 0; r < 16; ++r)
-    z1[r * 64 + l] = M->W1[2 * l] * x0[r] + M->W1[2 * l + 1] * x1[r] + M->b1[l];
+  // layer 1 (NIN -> 64) on the vector ALU: lane l computes units l of all rows
+  for (int r = 0; r < 16; ++r) {
+    float v = M->W1[2 * l] * x0[r] + M->W1[2 * l + 1] * x1[r];
+    if (NIN == 3) v += M->W1c[l] * x2[r];
+    z1[r * 64 + l] = v + M->b1[l];
+  }
   __builtin_amdgcn_wave_barrier();
   sfp_mlp_layer16<true>(z1, M->W2, M->b2, z2);
   sfp_mlp_layer16<true>(z2, M->W3, M->b3, z3);
@@ -162,9 +172,10 @@ __device__ inline const __bf16* sfp_mlp_opaque(const __bf16* p) {
   return p;
 }
 
-template <int NG>
+template <int NG, int NIN = 2>
 __device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_mlp_dev* M, const __bf16* W2p, const __bf16* W3p,
-                                     const float* x0, const float* x1, float* out) {
+                                     const float* x0, const float* x1, float* out,
+                                     const float* x2 = nullptr) {
   W2p = sfp_mlp_opaque(W2p);
   W3p = sfp_mlp_opaque(W3p);
   const int l = __lane_id();
@@ -177,7 +188,11 @@ __device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_ml
       const int n = 16 * t + 4 * g + e;
       const float w0 = M->W1[2 * n], w1 = M->W1[2 * n + 1], bb = M->b1[n];
 #pragma unroll
-      for (int q = 0; q < NG; ++q) h[q][4 * t + e] = w0 * x0[q] + w1 * x1[q] + bb;
+      for (int q = 0; q < NG; ++q) {
+        float v = w0 * x0[q] + w1 * x1[q];
+        if (NIN == 3) v += M->W1c[n] * x2[q];
+        h[q][4 * t + e] = v + bb;
+      }
     }
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
@@ -242,6 +257,24 @@ __device__ inline float sfp_mlp_eval16(const sfp_mlp_dev* M, const __bf16* W2p,
 __device__ inline double sfp_mlp_delta(const sfp_mlp_dev* M, float o) {
   const double d = (double)__expf(o * M->out_sd + M->out_mu);
   return d > SFP_DELTA_FLOOR ? d : SFP_DELTA_FLOOR;
+}
+
+/* NG = 1 with three inputs (the emission-rate network) */
+__device__ inline float sfp_mlp_eval16_3(const sfp_mlp_dev* M, float x0, float x1, float x2) {
+  float o;
+  sfp_mlp_evalN<1, 3>(M, M->W2p, M->W3p, &x0, &x1, &o, &x2);
+  return o;
+}
+
+/* The emission-rate network learns log(f + SFP_RATE_EPS): a smooth floor
+ * that keeps the target's range to about [-14, 3] while the densities that
+ * matter (f above ~1e-4; f = 1e-6 is an emission probability below 1e-7 per
+ * step) keep their relative accuracy; f = max(exp(out) - SFP_RATE_EPS, 0). */
+#define SFP_RATE_EPS 1e-6f
+
+__device__ inline float sfp_rate_from_net(const sfp_mlp_dev* M, float o) {
+  const float f = __expf(o * M->out_sd + M->out_mu) - SFP_RATE_EPS;
+  return f > 0.0f ? f : 0.0f;
 }
 
 /* log10 chi -> normalised input */

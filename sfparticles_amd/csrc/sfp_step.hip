@@ -93,6 +93,8 @@ struct Plan {
   uint32_t nshots;
   const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
   const sfp_mlp_bins* net_bins;  // its per-bin table fallback
+  const sfp_mlp_dev* rate_net;   // learned spin-resolved emission rate, or null
+  const sfp_mlp_bins* rate_bins;
   int32_t nspecies;
   int32_t qed;
 };
@@ -881,6 +883,75 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
   return event;
 }
 
+// The spin-resolved lepton emission decision of the mixed-precision step with
+// the learned rate (rejection sampling, qed/rejection_sampling.py:68-85): the
+// draws and sigma as in qed_event32; where log10 chi lies in the network's
+// range and its bin is not flagged for fallback, `net` is set with the
+// normalised inputs and the acceptance density comes from the wave's network
+// evaluation afterwards; elsewhere p is the table's.  Returns r2, and p where
+// the table decides.
+template <int FM>
+__device__ inline void qed_rate32_prep(const Plan& P, const sfp_fieldset* FS, const SpeciesDev& sp,
+                                       const StepArgs& a, const sfp_rs_node32* lds_rs, double x,
+                                       double y, double z, float ux, float uy, float uz, float ig,
+                                       uint64_t pid, float sx, float sy, float sz, bool& net,
+                                       float& in0, float& in1, float& in2, float& r1_3, float& r2,
+                                       float& dtau, float& p_table) {
+  net = false;
+  p_table = 0.0f;
+  r2 = 1.0f;
+  float F[6];
+  sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
+  const double chi = sfp32_chi(F, ux, uy, uz);
+  if (chi == 0.0) return;
+  sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+  const float r1 = sfp32_u01(d.v[0]);
+  r2 = sfp32_u01(d.v[2]);
+  r1_3 = r1 * r1 * r1;
+  dtau = (float)a.dt * ig;
+  const float sig = sfp32_spin_sigma(sp.q, ux, uy, uz, ig, F, sx, sy, sz);
+  const sfp_mlp_bins& nb = *P.rate_bins;
+  const float lc = __log10f((float)chi);
+  const float fb = (lc - nb.lc_min) / (nb.lc_max - nb.lc_min) * (float)SFP_MLP_BINS;
+  const int bin = fb < 0.0f ? 0 : (fb >= (float)SFP_MLP_BINS ? SFP_MLP_BINS - 1 : (int)fb);
+  if (lc >= nb.lc_min && lc <= nb.lc_max && !((nb.fallback >> bin) & 1ull)) {
+    net = true;
+    in0 = sfp_mlp_x0(P.rate_net, lc);
+    in1 = 2.0f * r1 - 1.0f;
+    in2 = sig;
+  } else {
+    p_table = 3.0f * (r1 * r1) *
+              sfp32_rs_rate_spin(lds_rs, P.rs_ai32, (int)P.z_N, (float)P.z_min, (float)P.z_max,
+                                 (float)P.z_delta, (float)(1.0 / P.z_delta), (float)chi, r1_3, sig) *
+              dtau;
+  }
+}
+
+// Normalised output of the rate network for the lanes with `net` set,
+// sixteen at a time on the matrix cores.  Called by all lanes of the wave.
+__device__ inline float net_rate_wave(const Plan& P, bool net, float in0, float in1, float in2) {
+  const sfp_mlp_dev* M = P.rate_net;
+  const int lane = wave_lane(), c = lane & 15;
+  uint64_t m = __ballot(net);
+  const uint64_t below = m & ((1ull << lane) - 1ull);
+  int slot = (int)__popcll(below);
+  float out = 0.0f;
+  while (m) {
+    const int cnt = __popcll(m) < 16 ? (int)__popcll(m) : 16;
+    uint64_t mm = m;
+    const int cc = c < cnt ? c : 0;
+    for (int j = 0; j < cc; ++j) mm &= mm - 1ull;
+    const int src = (int)__builtin_ctzll(mm);
+    const float a0 = __shfl(in0, src), a1 = __shfl(in1, src), a2 = __shfl(in2, src);
+    const float o = sfp_mlp_eval16_3(M, a0, a1, a2);
+    const float res = __shfl(o, slot >= 0 && slot < 16 ? slot : 0);
+    if (net && slot >= 0 && slot < 16) out = res;
+    for (int j = 0; j < cnt; ++j) m &= m - 1ull;
+    slot -= 16;
+  }
+  return out;
+}
+
 // delta = exp(network(in0, in1)) for the lanes with `net` set, sixteen at a
 // time on the matrix cores.  Called by all lanes of the wave.
 __device__ inline void net_delta_wave(const Plan& P, bool net, float in0, float in1,
@@ -948,7 +1019,9 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // (wave64 ballot + workgroup scan + decoupled look-back), the new photon
 // appended already pushed (the reference pushes particles in the step that
 // creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
-template <int KIND, int FM, bool MIX, bool NET>
+// NETK: 0 tables, 1 the learned photon-energy sampler, 2 the learned
+// spin-resolved emission rate (mixed precision, rejection sampling)
+template <int KIND, int FM, bool MIX, int NETK>
 __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
@@ -957,6 +1030,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
 
   __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  constexpr bool NET = NETK == 1;
   __shared__ double2 lds_od[(KIND == KIND_OD || NET) ? 255 : 1];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[2 * WAVES];
@@ -1036,6 +1110,21 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
                                   : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, chi_n, u_tab);
       // hard-photon cut (cpu.py:227-232)
       if (event && (float)delta / igf < (float)P.threshold) event = false;
+    } else if (NETK == 2) {
+      bool net = false;
+      float in0 = 0.0f, in1 = 0.0f, in2 = 0.0f, r1_3 = 0.0f, r2 = 1.0f, dtau = 0.0f, p = 0.0f;
+      if (valid)
+        qed_rate32_prep<FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, x, y, z, fx, fy, fz, igf,
+                            pid, (float)sx, (float)sy, (float)sz, net, in0, in1, in2, r1_3, r2,
+                            dtau, p);
+      const float o = net_rate_wave(P, net, in0, in1, in2);
+      if (net)
+        p = sfp_rate_from_net(P.rate_net, o) * (float)P.rate_unit * dtau;
+      if (valid && r2 < p) {
+        event = true;
+        delta = r1_3;
+        if ((float)delta / igf < (float)P.threshold) event = false;  // cpu.py:227-232
+      }
     } else if (valid) {
       if (MIX) {
         event = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
@@ -1394,10 +1483,12 @@ static kfn sweep_fn(bool photon, bool qed) {
 }
 
 template <int FM, bool MIX>
-static kfn hot_fn(bool photon, int kind, bool net) {
+static kfn hot_fn(bool photon, int kind, int netk) {
   if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM, MIX> : k_photon<KIND_RS, FM, MIX>;
-  if (MIX && net) return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, MIX> : k_hot<KIND_RS, FM, MIX, MIX>;
-  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, false> : k_hot<KIND_RS, FM, MIX, false>;
+  if (MIX && netk == 2 && kind == KIND_RS) return k_hot<KIND_RS, FM, MIX, (MIX ? 2 : 0)>;
+  if (MIX && netk == 1)
+    return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, (MIX ? 1 : 0)> : k_hot<KIND_RS, FM, MIX, (MIX ? 1 : 0)>;
+  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, 0> : k_hot<KIND_RS, FM, MIX, 0>;
 }
 
 template <bool MIX>
@@ -1408,19 +1499,19 @@ static kfn pick_sweep_p(int fm, bool photon, bool qed) {
 }
 
 template <bool MIX>
-static kfn pick_hot_p(int fm, bool photon, int kind, bool net) {
-  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind, net);
-  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind, net);
-  return hot_fn<SFP_FM_ANY, MIX>(photon, kind, net);
+static kfn pick_hot_p(int fm, bool photon, int kind, int netk) {
+  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind, netk);
+  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind, netk);
+  return hot_fn<SFP_FM_ANY, MIX>(photon, kind, netk);
 }
 
 static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed) {
   return mixed ? pick_sweep_p<true>(fm, photon, qed) : pick_sweep_p<false>(fm, photon, qed);
 }
 
-static kfn pick_hot(int fm, bool photon, int kind, bool mixed, bool net = false) {
-  return mixed ? pick_hot_p<true>(fm, photon, kind, net)
-               : pick_hot_p<false>(fm, photon, kind, false);
+static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk = 0) {
+  return mixed ? pick_hot_p<true>(fm, photon, kind, netk)
+               : pick_hot_p<false>(fm, photon, kind, 0);
 }
 
 // ------------------------------------------------------------- host side ---
@@ -1483,6 +1574,7 @@ struct Sim {
   int64_t syncs = 0, replays = 0;
   int cus = 0;                       // compute units of the device
   bool net = false;                  // leptons use the learned photon-energy sampler
+  bool rate_net = false;             // spin leptons use the learned emission rate
   sfp_fieldset* shot_fields = nullptr;  // device, one per shot
   int32_t nshots = 0;
 };
@@ -1583,8 +1675,15 @@ static int sim_pick_kernels(Sim* sim) {
   for (int s = 0; s < sim->nsp; ++s) {
     const bool photon = sim->plan.sp[s].photon;
     sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed);
-    sim->hot[s] =
-        sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, sim->net && !photon) : nullptr;
+    // learned models: the rate network for spin-carrying leptons under
+    // spin-resolved emission, else the photon-energy sampler
+    int netk = 0;
+    if (!photon && sim->rate_net && sim->plan.spin_emission && sim->plan.sp[s].spin &&
+        kind == KIND_RS)
+      netk = 2;
+    else if (!photon && sim->net)
+      netk = 1;
+    sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, netk) : nullptr;
     for (int ph = 0; ph < 2; ++ph) {
       kfn fn = ph ? sim->hot[s] : sim->sweep[s];
       std::vector<int>& c = sim->cand[s][ph];
@@ -1749,6 +1848,9 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
     return set_err(SFP_E_ARG, "the learned sampler needs the optical-depth tables");
   Plan& P = sim->plan;
   if (mlp) {
+    int32_t nin = 0, np = 0;
+    CHK(sfp_mlp_shape(mlp, &nin, &np));
+    if (nin != 2) return set_err(SFP_E_ARG, "not a photon-energy sampler");
     const void *dev = nullptr, *bins = nullptr;
     CHK(sfp_mlp_device(mlp, &dev, &bins));
     P.net = (const sfp_mlp_dev*)dev;
@@ -1758,6 +1860,33 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
     P.net_bins = nullptr;
   }
   sim->net = mlp != nullptr;
+  CHK(sim_pick_kernels(sim));
+  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_rate_net(void* p, void* mlp) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  Plan& P = sim->plan;
+  if (mlp) {
+    if (sim->params.precision != SFP_PRECISION_MIXED)
+      return set_err(SFP_E_ARG, "the learned rate runs in the mixed-precision step");
+    if (!P.spin_emission || sim->params.qed_model != SFP_QED_RS)
+      return set_err(SFP_E_ARG, "the learned rate needs spin-resolved emission under rejection "
+                                "sampling");
+    int32_t nin = 0, np = 0;
+    CHK(sfp_mlp_shape(mlp, &nin, &np));
+    if (nin != 3) return set_err(SFP_E_ARG, "not an emission-rate network");
+    const void *dev = nullptr, *bins = nullptr;
+    CHK(sfp_mlp_device(mlp, &dev, &bins));
+    P.rate_net = (const sfp_mlp_dev*)dev;
+    P.rate_bins = (const sfp_mlp_bins*)bins;
+  } else {
+    P.rate_net = nullptr;
+    P.rate_bins = nullptr;
+  }
+  sim->rate_net = mlp != nullptr;
   CHK(sim_pick_kernels(sim));
   HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
   return SFP_OK;

@@ -32,7 +32,8 @@ class Simulation(object):
     def __init__(self, *all_particles: Particles, dt: float, fields: Fields, print_every: int = 100,
                  t0=0.0, photon_threshold=2.0, seed: int = None, qed_model: str = None,
                  precision: str = "fp64", sampler=None, graph: bool = False,
-                 autotune: bool = False, shots=None, spin_emission: bool = False) -> None:
+                 autotune: bool = False, shots=None, spin_emission: bool = False,
+                 rate_net=None) -> None:
         """precision: "fp64" (the reference's arithmetic, bit-identical to the CPU
         restatement) or "mixed" (fields, pushers and QED sampling in FP32;
         positions, time and chi in FP64).  sampler (mixed only): a
@@ -47,7 +48,9 @@ class Simulation(object):
         spin_emission: photon emission of SpinParticles (rejection sampling)
         uses the spin-resolved rate, which depends on the projection of the
         spin on the direction of beta x dbeta/dt; the reference's emission
-        ignores spin."""
+        ignores spin.  rate_net (mixed, spin_emission): a qed.mlp.EmissionRateNet,
+        or "mlp" for the trained default, that gives the spin-resolved emission
+        rate in place of the table."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         if sampler is not None and precision != "mixed":
@@ -60,6 +63,16 @@ class Simulation(object):
         self.precision = precision
         self.sampler = sampler
         self.spin_emission = bool(spin_emission)
+        if rate_net is not None:
+            if precision != "mixed" or not self.spin_emission:
+                raise ValueError("the learned rate runs in the mixed-precision step with "
+                                 "spin_emission=True")
+            if isinstance(rate_net, str):
+                if rate_net != "mlp":
+                    raise ValueError(f"unknown rate network {rate_net!r}")
+                from .qed.mlp import default_rate_net
+                rate_net = default_rate_net()
+        self.rate_net = rate_net
         self.graph = bool(graph)
         self.autotune = bool(autotune)
         self.shots = None if shots is None else list(shots)
@@ -140,6 +153,8 @@ class Simulation(object):
         self._sim_stores = stores
         if self.sampler is not None:
             L.call("sfp_sim_set_sampler", self._sim, self.sampler.handle)
+        if self.rate_net is not None:
+            L.call("sfp_sim_set_rate_net", self._sim, self.rate_net.handle)
         if self.shots is not None:
             per = [f.require_terms() for f in self.shots]
             flat = [t for ts in per for t in ts]

@@ -129,38 +129,44 @@ def _expected_probability(chi, gamma, dt, sigma, threshold=2.0, m=100000):
     return float(np.mean(np.where(r ** 3 * gamma >= threshold, p, 0.0)))
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_spin_asymmetry_in_static_field(precision):
+def _static_field_run(precision, rate_net=None, n=400000, gamma=1000.0, chi=1.0):
+    """One step of n electrons along +x in a static B along +z (zeta = +z),
+    half with spin +z, half -z; returns the counts of emitters of each half
+    (electrons that recoiled: the field only rotates u, a hard photon takes
+    at least 2/gamma of |u|) and the parameters."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
     from sfparticles_amd.fields import Fields
     Q.set_model("rs")
-    n, gamma, chi = 400000, 1000.0, 1.0
     # chi = gamma |beta x B| c / E_s for u along x, B along z
     E_s = 1.32328547e18
     B = chi * E_s / C_LIGHT / np.sqrt(gamma ** 2 - 1)
     dt = 0.8e-6 / C_LIGHT / 100
     ux = np.full(n, np.sqrt(gamma ** 2 - 1))
     z0 = np.zeros(n)
-    sz = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)  # zeta = +z for electrons here
+    sz = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     e = sfa.SpinParticles("electron", -1, 1, N=n,
                           props=(z0, z0.copy(), z0.copy(), ux, z0.copy(), z0.copy(), z0.copy(),
                                  z0.copy(), sz))
     g = sfa.Particles("photon", 0, 0)
     e.set_photon(g)
     sim = sfa.Simulation(e, g, dt=dt, fields=Fields(terms=[(0, (0, 0, 0, 0, 0, B))]),
-                         print_every=None, seed=3, precision=precision, spin_emission=True)
+                         print_every=None, seed=3, precision=precision, spin_emission=True,
+                         rate_net=rate_net)
     sim.start(1)
-    # emitters are the electrons that recoiled: the magnetic field only
-    # rotates u, the recoil of a hard photon takes at least 2/gamma of |u|
     u_after = np.sqrt(np.array(e.ux)[:n] ** 2 + np.array(e.uy)[:n] ** 2 + np.array(e.uz)[:n] ** 2)
     emitted = u_after < ux * (1 - 1e-4)
     k_up, k_dn = int(emitted[sz > 0].sum()), int(emitted[sz < 0].sum())
     assert k_up + k_dn == g.N_buffered
+    return k_up, k_dn, n // 2, chi, gamma, dt
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spin_asymmetry_in_static_field(precision):
+    k_up, k_dn, m, chi, gamma, dt = _static_field_run(precision)
     p_up = _expected_probability(chi, gamma, dt, 1.0)
     p_dn = _expected_probability(chi, gamma, dt, -1.0)
-    m = n // 2
     print(f"{precision}: emitters spin +zeta {k_up} (expected {m * p_up:.0f}), "
           f"-zeta {k_dn} (expected {m * p_dn:.0f})")
     for k, p in ((k_up, p_up), (k_dn, p_dn)):
