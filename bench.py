@@ -212,7 +212,7 @@ def calibrate_copy(n_doubles, reps):
                       "reps": reps, "copy_GBps": round(16 * n_doubles / dt / 1e9, 1)}), flush=True)
 
 
-def ensemble_run(nshots, count_max, steps, check_shots, seed=0):
+def ensemble_run(nshots, count_max, steps, check_shots, seed=0, ppe=64):
     """The batched-shot workload; returns the JSON line's fields."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed, _lib as L
@@ -244,7 +244,7 @@ def ensemble_run(nshots, count_max, steps, check_shots, seed=0):
                              photon_threshold=2.0, seed=0, precision=precision,
                              spin_emission=True, rate_net=rate_net, graph=graph,
                              autotune=graph)
-        for s_, cap in zip((e, p, g), (2 * m + 4096, m + 4096, 64 * m + 4096)):
+        for s_, cap in zip((e, p, g), (2 * m + 4096, m + 4096, ppe * m + 4096)):
             s_._ensure_device()
             s_._extend(cap - s_.N_buffered)
         sim._prepare()
@@ -264,9 +264,9 @@ def ensemble_run(nshots, count_max, steps, check_shots, seed=0):
     elapsed = time.perf_counter() - t0
     st = [sim.stats(i) for i in range(3)]
     psteps = sum(x["psteps"] for x in st)
-    shapes = {f"{nm}_{ph}": sim.launch_shape(i, ph)[0]
-              for i, nm in enumerate(("electron", "positron", "photon")) for ph in ("sweep", "hot")
-              if sim.launch_shape(i, 0 if ph == "sweep" else 1)[0]}
+    shapes = {f"{nm}_{phn}": sim.launch_shape(i, ph)[0]
+              for i, nm in enumerate(("electron", "positron", "photon"))
+              for ph, phn in ((0, "sweep"), (1, "hot")) if sim.launch_shape(i, ph)[0]}
     ens = [shot_histogram(s_, nshots)[0] for s_ in sp]
     counts = [s_.N_buffered for s_ in sp]
     del sim, sp
@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--count-max", type=float, default=1e6)
     ap.add_argument("--check-shots", type=int, default=8)
+    ap.add_argument("--photons-per-electron", type=int, default=64,
+                    help="ensemble: photon store capacity reserved per seed electron")
     ap.add_argument("--path", choices=["fp64", "mixed", "mixed+mlp", "all"], default="all",
                     help="fp64: the reference's arithmetic; mixed: FP32 fields, push and QED "
                          "with FP64 positions, time and chi; mixed+mlp: mixed with the learned "
@@ -325,7 +327,8 @@ def main():
         return
 
     if args.workload == "ensemble":
-        r = ensemble_run(args.shots, args.count_max, args.steps, args.check_shots)
+        r = ensemble_run(args.shots, args.count_max, args.steps, args.check_shots,
+                         ppe=args.photons_per_electron)
         line = {"metric": "particle-steps/s (ensemble of shots, QED on, spin-resolved)",
                 "value": r.pop("value"), "unit": "particle-steps/s", "n_gpus": 1,
                 "steps": args.steps, "warmup": 0, "ms_per_step": r.pop("ms_per_step"),

@@ -2037,10 +2037,18 @@ static int sim_recover(Sim* sim, int s) {
   for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst}) {
     if (dst < 0) continue;
     Store* st = sim->stores[dst];
-    int64_t need = sim->counts_host[dst].n + n_hot;
+    const int64_t have = sim->counts_host[dst].n;
+    int64_t need = have + n_hot;
     if (need > st->cap) {
-      int64_t want = need + need / 2 + 4096;
-      CHK(sfp_store_reserve(st, want));
+      // the room the halted kernel asks for (one append per listed particle,
+      // mostly slack) plus half the particles already there; where HBM cannot
+      // hold that, a sixteenth
+      int64_t want = need + have / 2 + 4096;
+      if (sfp_store_reserve(st, want) != SFP_OK) {
+        (void)hipGetLastError();
+        want = need + have / 16 + 4096;
+        CHK(sfp_store_reserve(st, want));
+      }
     }
   }
   Halt clear{};

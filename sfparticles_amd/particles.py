@@ -55,6 +55,10 @@ _ATTR = {
 }
 
 
+# per-step scratch of the reference's passes, recomputed before every use
+_SCRATCH = frozenset(("inv_gamma", "chi", "Ex", "Ey", "Ez", "Bx", "By", "Bz"))
+
+
 class Particles(object):
     """particles.py:43-516"""
 
@@ -154,7 +158,13 @@ class Particles(object):
         self._cache.clear()
 
     def _ensure_device(self):
-        """Move the host arrays into a device store (particles.py:148-155)."""
+        """Move the host arrays into a device store (particles.py:148-155).
+
+        The per-step scratch arrays of the reference (1/gamma, chi, E, B) are
+        not uploaded: the fused step keeps them in registers, and the per-op
+        entry points allocate them on first use (1/gamma is then recomputed
+        from u), so a store costs 57 bytes per particle plus its optional
+        attributes instead of 129."""
         if self._store is not None:
             return
         L.lib()
@@ -165,8 +175,9 @@ class Particles(object):
         host = self._h
         object.__setattr__(self, "_h", None)
         for name in self.attrs:
-            if name in host:
+            if name in host and name not in _SCRATCH:
                 self._upload(name, host[name])
+        object.__setattr__(self, "_ig_stale", True)
         L.call("sfp_store_set_count", self._store, int(self.N_buffered), int(self.next_id))
         self._touch()
 
