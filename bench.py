@@ -59,6 +59,10 @@ HBM_PEAK_GBS = 8000.0
 #            (8 B); writes x, y, z (24 B) of every photon it pushes, a 4-byte
 #            wake step per multi-step push, and per conversion the pruned flag
 #            and wake step (5 B).
+#   leptons with the learned sampler: the QED kernel writes a 40-byte record
+#            per emission candidate instead of its state; the emission kernel
+#            reads the record and the state (x, y, z, u: 48 B) of every
+#            candidate and writes its state (48 B).
 #   Optical depth: + tau read and written (16 B) per exact QED evaluation.
 #   Per appended particle: x, y, z, u, id and flag (57 B) in each destination.
 def kernel_bytes(photon, phase, st, od=False):
@@ -71,6 +75,11 @@ def kernel_bytes(photon, phase, st, od=False):
                 + 4 * st["sleeps"] + 5 * st["spawned"] + 57 * 2 * st["spawned"])
     if phase == 0:
         return 49 * st["processed"] + 48 * (st["processed"] - hot) + 4 * hot
+    emit = st.get("emit", 0)
+    if phase == 2:
+        return (40 + 48 + 48) * emit + 57 * st["spawned"]
+    if emit:
+        return 60 * hot + 48 * (hot - emit) + 40 * emit + tau
     return 60 * hot + 48 * hot + tau + 57 * st["spawned"]
 
 
@@ -154,7 +163,7 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64"):
     L.call("sfp_synchronize")
     t1 = time.perf_counter()
     stats = [sim.stats(i) for i in range(3)]
-    ktime = [[sim.kernel_time(i, ph) for ph in (0, 1)] for i in range(3)]
+    ktime = [[sim.kernel_time(i, ph) for ph in (0, 1, 2)] for i in range(3)]
     spectrum = photon_energy_sample(species[2])
     counts = [s.N_buffered for s in species]
     del sim, species
@@ -353,7 +362,7 @@ def main():
         # dominant kernel: largest accumulated time over (species, phase); one
         # launch of each kernel per timed step does the work (launches replayed
         # after a capacity halt return at entry), so averages are per step
-        dom, ph = max(((i, p) for i in range(3) for p in (0, 1)),
+        dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)),
                       key=lambda ip: ktime[ip[0]][ip[1]][0])
         ms, _ = ktime[dom][ph]
         per_launch = kernel_bytes(dom == 2, ph, stats[dom]) / args.steps
@@ -361,13 +370,15 @@ def main():
         achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
         kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>",
                   (1, False): "k_hot<RS, gaussian field>", (0, True): "k_awake",
-                  (1, True): "k_photon<RS, gaussian field>"}[(ph, dom == 2)]
+                  (1, True): "k_photon<RS, gaussian field>",
+                  (2, False): "k_emit<RS, gaussian field>"}[(ph, dom == 2)]
                  + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
         kfile = ({(0, False): "k_sweep", (1, False): "k_hot", (0, True): "k_awake",
+                  (2, False): "k_emit",
                   (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_"
                  + precision.replace("+", "_"))
         traffic = load_traffic(kfile)
-        total_ms = sum(k[p][0] for k in ktime for p in (0, 1))
+        total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
@@ -381,8 +392,8 @@ def main():
                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
                   "photon_steps_in_multistep_push": stats[2]["slept"],
                   "photon_multistep_pushes": stats[2]["sleeps"],
-                  "kernel_ms": {f"{names[i]}_{'sweep' if p == 0 else 'hot'}": round(ktime[i][p][0], 2)
-                                for i in range(3) for p in (0, 1)},
+                  "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0], 2)
+                                for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
         return detail, roofline, counts, spectrum
 

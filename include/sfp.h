@@ -334,7 +334,7 @@ int sfp_sim_destroy(void* sim);
 #define SFP_RUN_AUTOTUNE 4
 int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t flags);
 /* Pin the persistent grid (workgroups of 256 threads) of one species' sweep
- * (phase 0) or QED (phase 1) kernel; 0 returns it to the tuner, whose default
+ * (phase 0), QED (1) or emission (2) kernel; 0 returns it to the tuner, whose default
  * is full occupancy.  No counterpart in the reference. */
 int sfp_sim_set_grid(void* sim, int32_t species, int32_t phase, int32_t grid);
 /* Grid in use, workgroup size and number of candidate grids of a kernel
@@ -344,7 +344,9 @@ int sfp_sim_launch_shape(void* sim, int32_t species, int32_t phase, int32_t* gri
 /* Accumulated time (ms) and launch count of one species' step kernel since
  * the last reset (requires SFP_RUN_TIMING; synchronous).  phase 0 is the
  * sweep kernel (envelope bound, push of particles that cannot undergo a QED
- * event, QED list), phase 1 the QED kernel over that list. */
+ * event, QED list), phase 1 the QED kernel over that list, phase 2 the
+ * emission kernel of leptons with the learned photon-energy sampler (photon
+ * energy of the emission candidates, append, recoil and push). */
 int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
@@ -368,7 +370,7 @@ int sfp_sim_set_sampler(void* sim, void* mlp);
  * counterpart in the reference. */
 int sfp_sim_set_rate_net(void* sim, void* mlp);
 
-/* Counters of one species since the last reset (synchronous), stats[13]:
+/* Counters of one species since the last reset (synchronous), stats[14]:
  * [0] particle-steps (live particles after each step, the reference's
  *     ns/particle denominator, simulation.py:150-151)
  * [1] particles read by the sweep kernel   [2] particles appended by the species
@@ -378,7 +380,8 @@ int sfp_sim_set_rate_net(void* sim, void* mlp);
  * [8] photon-steps inside multi-step pushes (the sweep reads only a 4-byte
  *     wake step for these)                 [9] multi-step pushes started
  * [10] batches run as a graph replay       [11] graph captures
- * [12] tuning batches (10-12: of the simulation, not the species) */
+ * [12] tuning batches (10-12: of the simulation, not the species)
+ * [13] emission candidates handed to the emission kernel (sum over steps) */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 

@@ -83,7 +83,9 @@ struct Counts {
   int64_t sleeps;     // multi-step photon pushes started
   int64_t list_end;   // photons: n_old when the visit list was last built or extended
   int64_t asleep_n;   // photons: live photons inside a multi-step push
-  int64_t pad[3];
+  int64_t n_emit;     // emission candidates handed to the emission kernel this step
+  int64_t emit_total; // sum of n_emit over steps
+  int64_t pad[1];
 };
 
 extern const size_t kAttrSize[SFP_NATTR];

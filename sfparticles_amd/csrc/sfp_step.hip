@@ -33,6 +33,20 @@
 
 static constexpr int TILE = 256;
 static constexpr int WAVES = TILE / 64;
+// kernels of a species and step: 0 sweep, 1 QED, 2 emission (leptons with the
+// learned photon-energy sampler)
+static constexpr int NPH = 3;
+
+// An emission candidate of a lepton whose photon energy is sampled by the
+// emission kernel: particle index, whether the network gives delta (else the
+// table value `delta`), the network's normalised inputs, chi and the table's
+// uniform (for the hard tail and fallback bins).
+struct EmitRec {
+  uint32_t i;
+  uint32_t net;
+  float in0, in1;
+  double chi, u_tab, delta;
+};
 #define SFP_MAX_SPECIES 8
 
 enum { KIND_RS = 0, KIND_OD = 1 };
@@ -46,6 +60,7 @@ struct SpeciesDev {
   uint32_t* hot;   // this step's QED list (ascending particle indices)
   uint32_t* wake;  // photons: run-relative step of the next visit (multi-step push), or null
   uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
+  EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
   Counts* cnt;
   int64_t cap;
   double q, ae;
@@ -68,8 +83,8 @@ struct Halt {
 
 struct Plan {
   SpeciesDev sp[SFP_MAX_SPECIES];
-  ScanState* scan[SFP_MAX_SPECIES][2];  // [species][0 sweep list, 1 event ranks]
-  uint64_t* flags[SFP_MAX_SPECIES][2];
+  ScanState* scan[SFP_MAX_SPECIES][NPH];  // [species][0 sweep list, 1 event ranks, 2 emissions]
+  uint64_t* flags[SFP_MAX_SPECIES][NPH];
   sfp_fieldset fields;
   const sfp_rs_node* rs_nodes;
   const sfp_rs_node32* rs_nodes32;  // the same nodes in FP32 (mixed precision)
@@ -954,8 +969,8 @@ __device__ inline float net_rate_wave(const Plan& P, bool net, float in0, float 
 
 // delta = exp(network(in0, in1)) for the lanes with `net` set, sixteen at a
 // time on the matrix cores.  Called by all lanes of the wave.
-__device__ inline void net_delta_wave(const Plan& P, bool net, float in0, float in1,
-                                      double& delta) {
+__device__ inline void net_delta_wave(const Plan& P, const __bf16* W2p, const __bf16* W3p,
+                                      bool net, float in0, float in1, double& delta) {
   const sfp_mlp_dev* M = P.net;
   const int lane = wave_lane(), c = lane & 15;
   uint64_t m = __ballot(net);
@@ -968,7 +983,7 @@ __device__ inline void net_delta_wave(const Plan& P, bool net, float in0, float 
     for (int j = 0; j < cc; ++j) mm &= mm - 1ull;
     const int src = (int)__builtin_ctzll(mm);
     const float a0 = __shfl(in0, src), a1 = __shfl(in1, src);
-    const float o = sfp_mlp_eval16(M, M->W2p, M->W3p, a0, a1);
+    const float o = sfp_mlp_eval16(M, W2p, W3p, a0, a1);
     const float res = __shfl(o, slot >= 0 && slot < 16 ? slot : 0);
     if (net && slot >= 0 && slot < 16) delta = sfp_mlp_delta(M, res);
     for (int j = 0; j < cnt; ++j) m &= m - 1ull;
@@ -1097,19 +1112,39 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
     bool event = false;
     double delta = 0.0;
     if (NET) {
-      bool net = false;
+      // An emission candidate goes to the emission kernel, which samples its
+      // energy on the matrix cores in full waves, applies the hard-photon cut,
+      // appends the photon and pushes the lepton (queue order = index order,
+      // so the appends keep their order); the others are pushed here.
+      bool net = false, cand = false;
       float in0 = 0.0f, in1 = 0.0f;
       double chi_n = 0.0, u_tab = 0.0;
       if (valid)
-        event = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
-                                          fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
-                                          u_tab);
-      net_delta_wave(P, net, in0, in1, delta);
-      if (net && delta > SFP_MLP_DELTA_CUT)  // the hard tail from the table, same uniform
-        delta = (KIND == KIND_RS) ? sfp_od_sample_from0(P.od.photon_cdf, &P.od, chi_n, u_tab)
-                                  : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, chi_n, u_tab);
-      // hard-photon cut (cpu.py:227-232)
-      if (event && (float)delta / igf < (float)P.threshold) event = false;
+        cand = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+                                         fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
+                                         u_tab);
+      const int64_t q = tile_rank(cand, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
+      if (cand) {
+        EmitRec r;
+        r.i = (uint32_t)i;
+        r.net = net ? 1u : 0u;
+        r.in0 = in0;
+        r.in1 = in1;
+        r.chi = chi_n;
+        r.u_tab = u_tab;
+        r.delta = delta;
+        sp.emit[q] = r;
+        if (KIND == KIND_OD) sp.tau[i] = tau;
+      } else if (valid) {
+        float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+        lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+        sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+        sp.ux[i] = fx; sp.uy[i] = fy; sp.uz[i] = fz;
+        if (sp.spin) { sp.sx[i] = s0; sp.sy[i] = s1; sp.sz[i] = s2; }
+        if (KIND == KIND_OD) sp.tau[i] = tau;
+        alive += 1;
+      }
+      continue;
     } else if (NETK == 2) {
       bool net = false;
       float in0 = 0.0f, in1 = 0.0f, in2 = 0.0f, r1_3 = 0.0f, r2 = 1.0f, dtau = 0.0f, p = 0.0f;
@@ -1184,7 +1219,120 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   block_sum(cnt, sh_w, tot);
   if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
   kernel_exit(scan, nwg, [&]() {
-    finish_appends(P, s, 1, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
+    if (NET) {
+      const uint64_t c = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+      sp.cnt->n_emit = (int64_t)c;
+      sp.cnt->emit_total += (int64_t)c;
+    } else {
+      finish_appends(P, s, 1, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
+    }
+    scan->epoch = epoch;
+  });
+}
+
+// --------------------------------------------------- lepton emission kernel --
+// Mixed precision with the learned photon-energy sampler: over this step's
+// emission candidates (from k_hot, in particle index order), delta from the
+// network (whole waves of candidates on the matrix cores, weights in LDS) or
+// the table, the hard tail from the table, the hard-photon cut (cpu.py:
+// 227-232), the stable rank and append of the photon, then recoil
+// (cpu.py:78-88) and push of the lepton.  The room for the appends was
+// checked by k_hot (one per listed particle).
+template <int KIND, int FM>
+__global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, int s,
+                                               const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+  __shared__ uint4 lds_w[2 * 4096 * 2 / 16];
+  __shared__ int64_t sh_tile;
+  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint64_t sh_pref;
+  const int64_t n_c = sp.cnt->n_emit;
+  const int64_t ntiles = (n_c + TILE - 1) / TILE;
+  // workgroups without work leave before reading the destination counts,
+  // which the exit of this launch changes
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
+  const SpeciesDev* dA = sp.photon_dst >= 0 ? &P.sp[sp.photon_dst] : nullptr;
+  const int64_t baseA = dA ? dA->cnt->n : 0;
+  const int64_t idA = dA ? dA->cnt->next_id : 0;
+  ScanState* scan = P.scan[s][2];
+  const uint32_t epoch = next_epoch(scan);
+  uint32_t alive = 0;
+  bool staged = false;
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    if (!staged) {
+      const uint4* w2 = (const uint4*)P.net->W2p;
+      const uint4* w3 = (const uint4*)P.net->W3p;
+      constexpr int NQ = 4096 * 2 / 16;
+      for (int k = threadIdx.x; k < NQ; k += TILE) {
+        lds_w[k] = w2[k];
+        lds_w[NQ + k] = w3[k];
+      }
+      __syncthreads();
+      staged = true;
+    }
+    const int64_t k = tile * TILE + threadIdx.x;
+    const bool valid = k < n_c;
+    EmitRec r;
+    r.i = 0; r.net = 0; r.in0 = 0.0f; r.in1 = 0.0f; r.chi = 0.0; r.u_tab = 0.0; r.delta = 0.0;
+    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, sx = 0, sy = 0, sz = 0;
+    int64_t i = 0;
+    if (valid) {
+      r = sp.emit[k];
+      i = (int64_t)r.i;
+      x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+      ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+      if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+    }
+    uint32_t shot = 0;
+    const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
+    float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+    float igf = valid ? __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz) : 1.0f;
+    double delta = r.delta;
+    const bool net = valid && r.net;
+    net_delta_wave(P, (const __bf16*)lds_w, (const __bf16*)lds_w + 4096, net, r.in0, r.in1, delta);
+    if (net && delta > SFP_MLP_DELTA_CUT)  // the hard tail from the table, same uniform
+      delta = (KIND == KIND_RS) ? sfp_od_sample_from0(P.od.photon_cdf, &P.od, r.chi, r.u_tab)
+                                : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, r.chi, r.u_tab);
+    // hard-photon cut (cpu.py:227-232)
+    bool event = valid && !((float)delta / igf < (float)P.threshold);
+    if (!dA) event = false;
+    const int64_t rank = tile_rank(event, tile, P.flags[s][2], epoch, sh_w, &sh_pref);
+    if (event) {
+      const float df = (float)delta;
+      spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
+                   df * fz);
+    }
+    if (valid) {
+      if (event && sp.rr == SFP_RR_PHOTON) {
+        const float f = 1.0f - (float)delta;
+        fx *= f;
+        fy *= f;
+        fz *= f;
+        igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+      }
+      float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+      lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+      sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+      sp.ux[i] = fx; sp.uy[i] = fy; sp.uz[i] = fz;
+      if (sp.spin) { sp.sx[i] = s0; sp.sy[i] = s1; sp.sz[i] = s2; }
+      alive += 1;
+    }
+  }
+  uint32_t cnt[1] = {alive};
+  uint64_t tot[1];
+  block_sum(cnt, sh_w, tot);
+  if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
+  kernel_exit(scan, nwg, [&]() {
+    finish_appends(P, s, 2, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
     scan->epoch = epoch;
   });
 }
@@ -1514,6 +1662,12 @@ static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk = 0) {
                : pick_hot_p<false>(fm, photon, kind, 0);
 }
 
+static kfn pick_emit(int fm, int kind) {
+  if (fm == SFP_FM_GAUSSIAN) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_GAUSSIAN> : k_emit<KIND_RS, SFP_FM_GAUSSIAN>;
+  if (fm == SFP_FM_PLANE) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_PLANE> : k_emit<KIND_RS, SFP_FM_PLANE>;
+  return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_ANY> : k_emit<KIND_RS, SFP_FM_ANY>;
+}
+
 // ------------------------------------------------------------- host side ---
 static constexpr int MAX_CAND = 8;
 static constexpr int64_t SYNC_EVERY = 64;
@@ -1533,14 +1687,15 @@ struct Sim {
   std::vector<int> qed;            // species has a QED process
   kfn sweep[SFP_MAX_SPECIES] = {};
   kfn hot[SFP_MAX_SPECIES] = {};
+  kfn emit[SFP_MAX_SPECIES] = {};
   // launch shapes: candidate persistent grids of each (species, phase) kernel,
   // the one in use, and a grid the caller pinned (0: tuned)
-  std::vector<int> cand[SFP_MAX_SPECIES][2];
-  int choice[SFP_MAX_SPECIES][2] = {};
-  int pinned[SFP_MAX_SPECIES][2] = {};
-  double tune_ms[SFP_MAX_SPECIES][2][MAX_CAND] = {};
-  int tune_cnt[SFP_MAX_SPECIES][2][MAX_CAND] = {};
-  int64_t tune_n[SFP_MAX_SPECIES][2] = {};  // work of the kernel when last tuned (+1)
+  std::vector<int> cand[SFP_MAX_SPECIES][NPH];
+  int choice[SFP_MAX_SPECIES][NPH] = {};
+  int pinned[SFP_MAX_SPECIES][NPH] = {};
+  double tune_ms[SFP_MAX_SPECIES][NPH][MAX_CAND] = {};
+  int tune_cnt[SFP_MAX_SPECIES][NPH][MAX_CAND] = {};
+  int64_t tune_n[SFP_MAX_SPECIES][NPH] = {};  // work of the kernel when last tuned (+1)
   bool tuned = false;
   int64_t tune_batches = 0;
   // the step's stream, the arguments of one batch of steps (device slots,
@@ -1551,10 +1706,12 @@ struct Sim {
   hipGraphExec_t graph = nullptr;
   bool graph_stale = true;
   int64_t graph_batches = 0, captures = 0;
-  std::vector<int64_t> flags_cap[2];
+  std::vector<int64_t> flags_cap[NPH];
   std::vector<int64_t> hot_cap;
-  std::vector<uint64_t*> flags[2];
+  std::vector<uint64_t*> flags[NPH];
   std::vector<uint32_t*> hot_mem;
+  std::vector<EmitRec*> emit_mem;   // leptons with the learned sampler
+  std::vector<int64_t> emit_cap;
   std::vector<uint32_t*> wake_mem;  // photon species: multi-step push bookkeeping
   std::vector<int64_t> wake_cap;
   ScanState* scan_mem = nullptr;
@@ -1567,9 +1724,9 @@ struct Sim {
   sfp_sim_params params{};
   int fm = SFP_FM_ANY;
   bool timing = false;
-  std::vector<EvRec> ev_pending[SFP_MAX_SPECIES][2];
-  double ms_acc[SFP_MAX_SPECIES][2] = {};
-  int64_t launches[SFP_MAX_SPECIES][2] = {};
+  std::vector<EvRec> ev_pending[SFP_MAX_SPECIES][NPH];
+  double ms_acc[SFP_MAX_SPECIES][NPH] = {};
+  int64_t launches[SFP_MAX_SPECIES][NPH] = {};
   std::vector<hipEvent_t> ev_pool;
   int64_t syncs = 0, replays = 0;
   int cus = 0;                       // compute units of the device
@@ -1612,7 +1769,7 @@ static int sim_refresh_plan(Sim* sim) {
     d.shot = sim->shot_fields ? (uint32_t*)st->p[SFP_SHOT] : nullptr;
     if (st->cap > (int64_t)UINT32_MAX)
       return set_err(SFP_E_ARG, "species %d: capacity above 2^32 particles", s);
-    for (int ph = 0; ph < 2; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
       int64_t need = st->cap / TILE + 2;
       if (need > sim->flags_cap[ph][s]) {
         if (sim->flags[ph][s]) HIPCHK(hipFree(sim->flags[ph][s]));
@@ -1622,7 +1779,7 @@ static int sim_refresh_plan(Sim* sim) {
         sim->flags_cap[ph][s] = want;
       }
       P.flags[s][ph] = sim->flags[ph][s];
-      P.scan[s][ph] = sim->scan_mem + 2 * s + ph;
+      P.scan[s][ph] = sim->scan_mem + NPH * s + ph;
     }
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
       // photons keep their visit list from step to step: grown inside a run,
@@ -1639,6 +1796,15 @@ static int sim_refresh_plan(Sim* sim) {
       sim->hot_cap[s] = want;
     }
     d.hot = sim->hot_mem[s];
+    if (sim->emit[s] && st->cap > sim->emit_cap[s]) {
+      // one step's emission candidates: written by k_hot, read by k_emit
+      if (sim->emit_mem[s]) HIPCHK(hipFree(sim->emit_mem[s]));
+      sim->emit_mem[s] = nullptr;
+      const int64_t want = std::max<int64_t>(st->cap, 1);
+      HIPCHK(hipMalloc(&sim->emit_mem[s], sizeof(EmitRec) * want));
+      sim->emit_cap[s] = want;
+    }
+    d.emit = sim->emit[s] ? sim->emit_mem[s] : nullptr;
     if (d.photon && st->cap > sim->wake_cap[s]) {
       // grown inside a run: sleeping photons keep their wake steps; whole
       // tiles of k_awake, so that its vector loads stay inside the array
@@ -1684,8 +1850,9 @@ static int sim_pick_kernels(Sim* sim) {
     else if (!photon && sim->net)
       netk = 1;
     sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, netk) : nullptr;
-    for (int ph = 0; ph < 2; ++ph) {
-      kfn fn = ph ? sim->hot[s] : sim->sweep[s];
+    sim->emit[s] = (sim->qed[s] && mixed && netk == 1) ? pick_emit(sim->fm, kind) : nullptr;
+    for (int ph = 0; ph < NPH; ++ph) {
+      kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
       std::vector<int>& c = sim->cand[s][ph];
       c.clear();
       sim->choice[s][ph] = 0;
@@ -1813,18 +1980,20 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   HIPCHK(hipStreamCreate(&sim->stream));
   HIPCHK(hipMalloc(&sim->args_dev, sizeof(StepArgs) * SYNC_EVERY));
   HIPCHK(hipHostMalloc((void**)&sim->args_host, sizeof(StepArgs) * SYNC_EVERY, 0));
-  HIPCHK(hipMalloc(&sim->scan_mem, sizeof(ScanState) * 2 * nspecies));
-  HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * 2 * nspecies));
+  HIPCHK(hipMalloc(&sim->scan_mem, sizeof(ScanState) * NPH * nspecies));
+  HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * NPH * nspecies));
   HIPCHK(hipMalloc(&sim->halt, sizeof(Halt)));
   HIPCHK(hipMemset(sim->halt, 0, sizeof(Halt)));
   HIPCHK(hipHostMalloc((void**)&sim->halt_host, sizeof(Halt), 0));
   HIPCHK(hipHostMalloc((void**)&sim->counts_host, sizeof(Counts) * nspecies, 0));
   P.halt = sim->halt;
-  for (int ph = 0; ph < 2; ++ph) {
+  for (int ph = 0; ph < NPH; ++ph) {
     sim->flags[ph].assign(nspecies, nullptr);
     sim->flags_cap[ph].assign(nspecies, 0);
   }
   sim->hot_mem.assign(nspecies, nullptr);
+  sim->emit_mem.assign(nspecies, nullptr);
+  sim->emit_cap.assign(nspecies, 0);
   sim->hot_cap.assign(nspecies, 0);
   sim->wake_mem.assign(nspecies, nullptr);
   sim->wake_cap.assign(nspecies, 0);
@@ -1861,7 +2030,7 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
   }
   sim->net = mlp != nullptr;
   CHK(sim_pick_kernels(sim));
-  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+  CHK(sim_refresh_plan(sim));  // the emission queue of the kernels picked
   return SFP_OK;
 }
 
@@ -1888,7 +2057,7 @@ extern "C" int sfp_sim_set_rate_net(void* p, void* mlp) {
   }
   sim->rate_net = mlp != nullptr;
   CHK(sim_pick_kernels(sim));
-  HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
+  CHK(sim_refresh_plan(sim));
   return SFP_OK;
 }
 
@@ -1925,11 +2094,13 @@ extern "C" int sfp_sim_destroy(void* p) {
   Sim* sim = (Sim*)p;
   if (!sim) return SFP_OK;
   (void)hipDeviceSynchronize();
-  for (int ph = 0; ph < 2; ++ph)
+  for (int ph = 0; ph < NPH; ++ph)
     for (auto f : sim->flags[ph])
       if (f) (void)hipFree(f);
   for (auto h : sim->hot_mem)
     if (h) (void)hipFree(h);
+  for (auto q : sim->emit_mem)
+    if (q) (void)hipFree(q);
   for (auto w : sim->wake_mem)
     if (w) (void)hipFree(w);
   if (sim->scan_mem) (void)hipFree(sim->scan_mem);
@@ -1943,7 +2114,7 @@ extern "C" int sfp_sim_destroy(void* p) {
   if (sim->args_host) (void)hipHostFree(sim->args_host);
   if (sim->stream) (void)hipStreamDestroy(sim->stream);
   for (int s = 0; s < SFP_MAX_SPECIES; ++s)
-    for (int ph = 0; ph < 2; ++ph)
+    for (int ph = 0; ph < NPH; ++ph)
       for (auto& e : sim->ev_pending[s][ph]) {
         (void)hipEventDestroy(e.e0);
         (void)hipEventDestroy(e.e1);
@@ -1968,7 +2139,7 @@ static hipEvent_t sim_event(Sim* sim) {
 // tune >= 0: a tuning launch, run with candidate grid (tune mod candidates)
 // and timed for the tuner.
 static int launch(Sim* sim, int s, int ph, int j, int tune) {
-  kfn fn = ph ? sim->hot[s] : sim->sweep[s];
+  kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
   const std::vector<int>& cand = sim->cand[s][ph];
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
   const bool ev = sim->timing || tune >= 0;
@@ -1999,6 +2170,7 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
     int first = (s == s0) ? ph0 : 0;
     if (first == 0) CHK(launch(sim, s, 0, j, tune));
     if (sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
+    if (sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
   }
   return SFP_OK;
 }
@@ -2037,16 +2209,15 @@ static int sim_recover(Sim* sim, int s) {
   for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst}) {
     if (dst < 0) continue;
     Store* st = sim->stores[dst];
-    const int64_t have = sim->counts_host[dst].n;
-    int64_t need = have + n_hot;
+    const int64_t need = sim->counts_host[dst].n + n_hot;
     if (need > st->cap) {
-      // the room the halted kernel asks for (one append per listed particle,
-      // mostly slack) plus half the particles already there; where HBM cannot
-      // hold that, a sixteenth
-      int64_t want = need + have / 2 + 4096;
+      // the room the halted kernel asks for (one append per listed particle)
+      // and a quarter more, so that growing lists do not halt every step;
+      // where HBM cannot hold that, a sixteenth more
+      int64_t want = need + need / 4 + 4096;
       if (sfp_store_reserve(st, want) != SFP_OK) {
         (void)hipGetLastError();
-        want = need + have / 16 + 4096;
+        want = need + need / 16 + 4096;
         CHK(sfp_store_reserve(st, want));
       }
     }
@@ -2063,7 +2234,7 @@ static int sim_recover(Sim* sim, int s) {
 static int sim_collect_events(Sim* sim) {
   HIPCHK(hipStreamSynchronize(sim->stream));
   for (int s = 0; s < sim->nsp; ++s)
-    for (int ph = 0; ph < 2; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
       for (auto& e : sim->ev_pending[s][ph]) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e.e0, e.e1));
@@ -2086,7 +2257,7 @@ static int sim_collect_events(Sim* sim) {
 // Work of a kernel as the tuner sees it: particles swept, or listed.
 static int64_t sim_work(const Sim* sim, int s, int ph) {
   const Counts& c = sim->counts_host[s];
-  return 1 + (ph ? c.n_hot : c.n);
+  return 1 + (ph == 0 ? c.n : (ph == 1 ? c.n_hot : c.n_emit));
 }
 
 // The tuner runs a batch again when the work of any kernel has changed by
@@ -2094,7 +2265,7 @@ static int64_t sim_work(const Sim* sim, int s, int ph) {
 static bool sim_needs_tune(const Sim* sim) {
   if (!sim->tuned) return true;
   for (int s = 0; s < sim->nsp; ++s)
-    for (int ph = 0; ph < 2; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
       if (sim->cand[s][ph].size() < 2) continue;
       const int64_t w = sim_work(sim, s, ph), w0 = sim->tune_n[s][ph];
       if (w > 4 * w0 || w0 > 4 * w) return true;
@@ -2107,7 +2278,7 @@ static bool sim_needs_tune(const Sim* sim) {
 static int sim_tune_pick(Sim* sim) {
   CHK(sim_collect_events(sim));
   for (int s = 0; s < sim->nsp; ++s)
-    for (int ph = 0; ph < 2; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
       const int nc = (int)sim->cand[s][ph].size();
       int best = sim->choice[s][ph];
       double best_ms = 0.0;
@@ -2207,7 +2378,7 @@ static int sim_collect_timing(Sim* sim) { return sim_collect_events(sim); }
 
 extern "C" int sfp_sim_set_grid(void* p, int32_t species, int32_t phase, int32_t grid) {
   Sim* sim = (Sim*)p;
-  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1 || grid < 0)
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH || grid < 0)
     return set_err(SFP_E_ARG, "bad species/phase/grid");
   sim->pinned[species][phase] = grid;
   return sim_pick_kernels(sim);
@@ -2216,7 +2387,7 @@ extern "C" int sfp_sim_set_grid(void* p, int32_t species, int32_t phase, int32_t
 extern "C" int sfp_sim_launch_shape(void* p, int32_t species, int32_t phase, int32_t* grid,
                                     int32_t* tile, int32_t* ncand) {
   Sim* sim = (Sim*)p;
-  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1)
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH)
     return set_err(SFP_E_ARG, "bad species/phase");
   const bool has = !sim->cand[species][phase].empty();
   *grid = has ? sim_grid(sim, species, phase) : 0;
@@ -2228,7 +2399,7 @@ extern "C" int sfp_sim_launch_shape(void* p, int32_t species, int32_t phase, int
 extern "C" int sfp_sim_kernel_time(void* p, int32_t species, int32_t phase, double* ms,
                                    int64_t* launches) {
   Sim* sim = (Sim*)p;
-  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase > 1)
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH)
     return set_err(SFP_E_ARG, "bad species/phase");
   CHK(sim_collect_timing(sim));
   *ms = sim->ms_acc[species][phase];
@@ -2241,7 +2412,7 @@ extern "C" int sfp_sim_reset_timing(void* p) {
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   CHK(sim_collect_timing(sim));
   for (int s = 0; s < sim->nsp; ++s)
-    for (int ph = 0; ph < 2; ++ph) {
+    for (int ph = 0; ph < NPH; ++ph) {
       sim->ms_acc[s][ph] = 0.0;
       sim->launches[s][ph] = 0;
     }
@@ -2266,6 +2437,7 @@ extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
   stats[10] = sim->graph_batches;
   stats[11] = sim->captures;
   stats[12] = sim->tune_batches;
+  stats[13] = c.emit_total;
   return SFP_OK;
 }
 
@@ -2275,7 +2447,7 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   CHK(sim_read_counts(sim));
   for (int s = 0; s < sim->nsp; ++s) {
     Counts c = sim->counts_host[s];
-    c.psteps = c.processed = c.spawned = c.hot_total = c.slept = c.sleeps = 0;
+    c.psteps = c.processed = c.spawned = c.hot_total = c.slept = c.sleeps = c.emit_total = 0;
     HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   }
   sim->syncs = 0;

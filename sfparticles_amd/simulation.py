@@ -192,7 +192,7 @@ class Simulation(object):
             object.__setattr__(p, "_ig_stale", True)
 
     def kernel_time(self, species: int, phase: int):
-        """(ms, launches) of a species' sweep (phase 0) or QED (phase 1) kernel."""
+        """(ms, launches) of a species' sweep (phase 0), QED (1) or emission (2) kernel."""
         ms = C.c_double(0)
         n = C.c_int64(0)
         L.call("sfp_sim_kernel_time", self._sim, int(species), int(phase), C.byref(ms),
@@ -220,12 +220,12 @@ class Simulation(object):
         """Counters of one species since the last reset: particle-steps (live
         particles after each step), particles read by its step kernel,
         particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 13)()
+        v = (C.c_int64 * 14)()
         L.call("sfp_sim_stats", self._sim, int(species), v)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
                     hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
                     slept=int(v[8]), sleeps=int(v[9]), graph_batches=int(v[10]),
-                    captures=int(v[11]), tune_batches=int(v[12]))
+                    captures=int(v[11]), tune_batches=int(v[12]), emit=int(v[13]))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)
