@@ -61,6 +61,11 @@ struct SpeciesDev {
   uint32_t* wake;  // photons: run-relative step of the next visit (multi-step push), or null
   uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
   EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
+  // photons: state of the first `ccap` entries of the visit list, by list
+  // position, for the 8-step window the list is kept (see k_photon)
+  double *cx, *cy, *cz, *cux, *cuy, *cuz;
+  uint32_t* cwake;
+  int64_t ccap;
   Counts* cnt;
   int64_t cap;
   double q, ae;
@@ -1376,8 +1381,15 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     const uint32_t nwg = participants((n_old - from + TILE - 1) / TILE);
     if (blockIdx.x >= nwg) return;
     for (int64_t j = from + (int64_t)blockIdx.x * TILE + threadIdx.x; j < n_old;
-         j += (int64_t)nwg * TILE)
-      sp.hot[n_list + (j - from)] = (uint32_t)j;
+         j += (int64_t)nwg * TILE) {
+      const int64_t k = n_list + (j - from);
+      sp.hot[k] = (uint32_t)j;
+      if (k < sp.ccap) {  // into the window cache (the new photons are contiguous)
+        sp.cx[k] = sp.x[j]; sp.cy[k] = sp.y[j]; sp.cz[k] = sp.z[j];
+        sp.cux[k] = sp.ux[j]; sp.cuy[k] = sp.uy[j]; sp.cuz[k] = sp.uz[j];
+        sp.cwake[k] = sp.pruned[j] ? WAKE_NEVER : sp.wake[j];
+      }
+    }
     kernel_exit(scan, nwg, [&]() {
       Counts* c = sp.cnt;
       c->n_hot = n_list + (n_old - from);
@@ -1505,16 +1517,36 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
     bool live = false;
     int64_t i = 0;
     double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
+    // Window cache: the list is rebuilt every 8th step and kept in between,
+    // so the state of its first ccap entries lives in arrays indexed by list
+    // position for the window (coalesced), gathered from the store at the
+    // rebuild (or by the list kernel for appended photons) and written back
+    // at the window's last step; photons falling asleep or converting go
+    // straight to the store.
+    const bool cached = k < sp.ccap;
+    const bool rebuild = (a.rel & 7u) == 0;
+    const bool last = (a.rel & 7u) == 7u || a.rel + 1 == a.rel_end;
     if (k < n_list) {
       i = (int64_t)sp.hot[k];
       // a list kept from an earlier step holds photons that have fallen
       // asleep since (counted by the list kernel) and holes: skipped
-      if (sp.wake[i] <= a.rel) {
+      if (cached && !rebuild) {
+        if (sp.cwake[k] <= a.rel) {
+          x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
+          ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
+          live = true;
+        }
+      } else if (sp.wake[i] <= a.rel) {
         const uint8_t pr = sp.pruned[i];
         x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
         ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
         live = pr == 0;
-        if (!live) sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
+        if (!live) {
+          sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
+          if (cached) sp.cwake[k] = WAKE_NEVER;
+        }
+      } else if (cached) {
+        sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
       }
     }
     const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
@@ -1559,6 +1591,7 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
         // the converted photon becomes a hole (cpu.py:189-190), not pushed
         sp.pruned[i] = 1;
         sp.wake[i] = WAKE_NEVER;
+        if (cached) sp.cwake[k] = WAKE_NEVER;
       } else {
         uint32_t nst = 1;
         if (!sp.push) {
@@ -1580,11 +1613,21 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
             y = sfp_repeat_add(y, hy, 2 * nst);
             z = sfp_repeat_add(z, hz, 2 * nst);
           }
-          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          if (!cached || nst > 1 || last) {
+            sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          }
+          if (cached && nst == 1 && !last) {
+            sp.cx[k] = x; sp.cy[k] = y; sp.cz[k] = z;
+            if (rebuild) {
+              sp.cux[k] = ux; sp.cuy[k] = uy; sp.cuz[k] = uz;
+              sp.cwake[k] = 0u;
+            }
+          }
         }
         if (hot && KIND == KIND_OD) sp.tau[i] = tau;
         if (nst > 1) {
           sp.wake[i] = a.rel + nst;
+          if (cached) sp.cwake[k] = a.rel + nst;
           cnt[3] += 1;
         }
         cnt[0] += 1;
@@ -1714,6 +1757,8 @@ struct Sim {
   std::vector<int64_t> emit_cap;
   std::vector<uint32_t*> wake_mem;  // photon species: multi-step push bookkeeping
   std::vector<int64_t> wake_cap;
+  std::vector<char*> cache_mem;     // photon species: window cache of the visit list
+  std::vector<int64_t> cache_cap;
   ScanState* scan_mem = nullptr;
   Halt* halt = nullptr;            // device
   Halt* halt_host = nullptr;       // pinned
@@ -1821,6 +1866,15 @@ static int sim_refresh_plan(Sim* sim) {
       sim->wake_cap[s] = want;
     }
     d.wake = sim->wake_mem[s];
+    {
+      const int64_t cc = sim->cache_cap[s];
+      double* c = (double*)sim->cache_mem[s];
+      d.cx = c; d.cy = c ? c + cc : nullptr; d.cz = c ? c + 2 * cc : nullptr;
+      d.cux = c ? c + 3 * cc : nullptr; d.cuy = c ? c + 4 * cc : nullptr;
+      d.cuz = c ? c + 5 * cc : nullptr;
+      d.cwake = c ? (uint32_t*)(c + 6 * cc) : nullptr;
+      d.ccap = c ? cc : 0;
+    }
   }
   HIPCHK(hipMemcpy(sim->plan_dev, &P, sizeof(Plan), hipMemcpyHostToDevice));
   return SFP_OK;
@@ -1997,6 +2051,8 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   sim->hot_cap.assign(nspecies, 0);
   sim->wake_mem.assign(nspecies, nullptr);
   sim->wake_cap.assign(nspecies, 0);
+  sim->cache_mem.assign(nspecies, nullptr);
+  sim->cache_cap.assign(nspecies, 0);
   CHK(sim_pick_kernels(sim));
   CHK(sim_refresh_plan(sim));
   *out = sim;
@@ -2103,6 +2159,8 @@ extern "C" int sfp_sim_destroy(void* p) {
     if (q) (void)hipFree(q);
   for (auto w : sim->wake_mem)
     if (w) (void)hipFree(w);
+  for (auto c : sim->cache_mem)
+    if (c) (void)hipFree(c);
   if (sim->scan_mem) (void)hipFree(sim->scan_mem);
   if (sim->plan_dev) (void)hipFree(sim->plan_dev);
   if (sim->halt) (void)hipFree(sim->halt);
@@ -2316,9 +2374,25 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   for (int s = 0; s < sim->nsp; ++s)
     if (sim->plan.sp[s].cap != sim->stores[s]->cap || sim->plan.sp[s].x != sim->stores[s]->p[SFP_X])
       dirty = true;
-  if (dirty) CHK(sim_refresh_plan(sim));
   if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
     return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
+  // window cache of the photon visit lists: a quarter of the store (longer
+  // lists keep their tail in the store); sized at the start of a run only, so
+  // that it stays valid while stores grow inside the run
+  for (int s = 0; s < sim->nsp; ++s) {
+    if (!sim->plan.sp[s].photon || !sim->qed[s]) continue;
+    const int64_t want = sim->stores[s]->cap / 4;
+    if (want > sim->cache_cap[s]) {
+      if (sim->cache_mem[s]) HIPCHK(hipFree(sim->cache_mem[s]));
+      sim->cache_mem[s] = nullptr;
+      sim->cache_cap[s] = 0;
+      const int64_t cc = want + want / 2;
+      HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * (6 * sizeof(double) + sizeof(uint32_t))));
+      sim->cache_cap[s] = cc;
+      dirty = true;
+    }
+  }
+  if (dirty) CHK(sim_refresh_plan(sim));
   // every photon is visited at the first step of a run (positions are current
   // between runs; the store may have changed)
   for (int s = 0; s < sim->nsp; ++s)
