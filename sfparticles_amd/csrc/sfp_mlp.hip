@@ -7,7 +7,7 @@
 // qed/optical_depth_tables.py:59-101 for photon emission, the emission-rate
 // network the rejection-sampling rate lookup (qed/rejection_sampling_tables.py:
 // 32-53, with the spin term); each is trained on its table, which stays as the
-// error judge and as the fallback.
+// reference for its error and as the fallback.
 
 #include "sfp_internal.h"
 #include "sfp_mlp.h"
