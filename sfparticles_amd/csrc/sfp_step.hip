@@ -2431,11 +2431,20 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       for (int64_t j = 0; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, -1));
     }
     CHK(sim_read_counts(sim));
-    // replay from a halted kernel until the batch completes
+    // replay from a halted kernel until the batch completes; a halt that
+    // comes back at the same kernel and step means the recovery did not make
+    // the room the kernel asks for: an error, not another replay
+    int32_t last_s = -1;
+    int64_t last_h = -1;
     while (sim->halt_host->flag) {
       int s = sim->halt_host->species;
       int64_t h = sim->halt_host->step - (int64_t)(step0 + k);
       if (h < 0 || h >= batch) return set_err(SFP_E_STATE, "halt outside the launched steps");
+      if (s == last_s && h == last_h)
+        return set_err(SFP_E_CAPACITY, "species %d halted twice at step %lld: no room after growth",
+                       s, (long long)sim->halt_host->step);
+      last_s = s;
+      last_h = h;
       CHK(sim_recover(sim, s));
       CHK(launch_step(sim, (int)h, s, 1, -1));
       for (int64_t j = h + 1; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, -1));
