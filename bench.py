@@ -17,6 +17,11 @@ algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around each of
 its launches) and the CPU oracle timed on this machine's cores on a bounded
 sample of the same workload.
 
+--workload cascade runs the cascade configuration instead: 1e4 electrons at
+rest at the node of a rotating-E standing wave of two counter-propagating
+circularly polarised pulses (a0 1000), optical-depth model, 6 ctau / (c dt)
+steps, FP64 and mixed precision.
+
 --workload ensemble runs the batched-shot workload instead (not part of the
 default run): --shots independent head-on shots with a0 ~ U(20, 500),
 electron counts log-uniform in [1e3, --count-max], gamma_s ~ U(500, 5000)
@@ -221,6 +226,63 @@ def calibrate_copy(n_doubles, reps):
                       "reps": reps, "copy_GBps": round(16 * n_doubles / dt / 1e9, 1)}), flush=True)
 
 
+def cascade_run(n, steps, precision, seed=0, x0_ctau=3.0, reserve=(0, 0, 0)):
+    """The cascade workload: n electrons at rest uniformly in a ball of radius
+    0.1 um at the node of a rotating-E standing wave (two counter-propagating
+    circularly polarised plane pulses, each the sum of two linear pulses with
+    pol 0 / pi/2 and cep 0 / pi/2; lambda 0.8 um, a0 1000, w0 3 um, tau_FWHM
+    30 fs, starting x0_ctau ctau from the node), optical-depth model, photon
+    threshold 20 m c^2.  Draw order (numpy PCG64(seed)): radius, direction."""
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed, _lib as L
+    from sfparticles_amd.fields import Fields
+    qed.set_model("od")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r = 0.1e-6 * rng.random(n) ** (1 / 3)
+    v = rng.normal(size=(3, n))
+    v /= np.linalg.norm(v, axis=0)
+    z0 = np.zeros(n)
+    props = (r * v[0], r * v[1], r * v[2], z0, z0.copy(), z0.copy())
+    x0 = x0_ctau * CTAU
+    field = [(1, (1000.0, 3e-6, CTAU, 2, d, -d * x0, LAMBDA, pol, cep))
+             for d in (1, -1) for pol, cep in ((0.0, 0.0), (math.pi / 2, math.pi / 2))]
+    e = sfa.Particles("electron", -1, 1, N=n, props=props)
+    p = sfa.Particles("positron", 1, 1)
+    g = sfa.Particles("photon", 0, 0)
+    e.set_photon(g)
+    p.set_photon(g)
+    g.set_pair(e, p)
+    sim = sfa.Simulation(e, p, g, dt=DT, fields=Fields(terms=field), print_every=None,
+                         photon_threshold=20.0, seed=0, precision=precision)
+    for s_ in (e, p, g):
+        s_._ensure_device()
+    # preallocated SoA capacity (the reference's _extend), so that the growth
+    # of the cascade needs no capacity replay
+    for s_, cap in zip((e, p, g), reserve):
+        if cap > s_.buffer_size:
+            s_._extend(int(cap) - s_.N_buffered)
+    sim._prepare()
+    L.call("sfp_synchronize")
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        m = min(500, steps - done)
+        sim.run_steps(m)
+        done += m
+        print(f"[bench] cascade {precision} {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
+              f"{[s_.N_buffered for s_ in (e, p, g)]}", file=sys.stderr, flush=True)
+    L.call("sfp_synchronize")
+    elapsed = time.perf_counter() - t0
+    st = [sim.stats(i) for i in range(3)]
+    return dict(value=round(sum(x["psteps"] for x in st) / elapsed, 1),
+                ms_per_step=round(elapsed / steps * 1e3, 4),
+                final_counts={nm: s_.N_buffered for nm, s_ in
+                              zip(("electron", "positron", "photon"), (e, p, g))},
+                particle_steps={nm: x["psteps"] for nm, x in
+                                zip(("electron", "positron", "photon"), st)},
+                capacity_replays=st[0]["replays"], seconds=round(elapsed, 2))
+
+
 def ensemble_run(nshots, count_max, steps, check_shots, seed=0, ppe=64):
     """The batched-shot workload; returns the JSON line's fields."""
     import sfparticles_amd as sfa
@@ -314,10 +376,14 @@ def main():
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--workload", choices=["head-on", "ensemble"], default="head-on")
+    ap.add_argument("--workload", choices=["head-on", "ensemble", "cascade"], default="head-on")
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--count-max", type=float, default=1e6)
     ap.add_argument("--check-shots", type=int, default=8)
+    ap.add_argument("--x0-ctau", type=float, default=3.0,
+                    help="cascade: distance of the pulses' start from the node, in ctau")
+    ap.add_argument("--reserve", default="0,0,0",
+                    help="cascade: electron, positron, photon capacity reserved up front")
     ap.add_argument("--photons-per-electron", type=int, default=64,
                     help="ensemble: photon store capacity reserved per seed electron")
     ap.add_argument("--path", choices=["fp64", "mixed", "mixed+mlp", "all"], default="all",
@@ -333,6 +399,25 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(1 << 27, 5)
+        return
+
+    if args.workload == "cascade":
+        steps = args.steps if args.steps != 3000 else int(6 * CTAU / (C_LIGHT * DT))
+        paths = ["fp64", "mixed"] if args.path == "all" else [args.path.replace("+mlp", "")]
+        res = {pth: cascade_run(10000, steps, pth, x0_ctau=args.x0_ctau,
+                                reserve=tuple(float(v) for v in args.reserve.split(",")))
+               for pth in paths}
+        head = res[paths[-1]]
+        line = {"metric": "particle-steps/s (cascade, QED on, optical depth)",
+                "value": head["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+                "warmup": 0, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64" if paths[-1] == "fp64" else "f32+f64", "data": "synthetic",
+                "config": dict(workload="rotating-E standing wave, a0 1000, 1e4 e- at rest at "
+                                        f"the node, pulses from {args.x0_ctau:g} ctau, optical "
+                                        "depth, photon threshold 20",
+                               **{pth: r for pth, r in res.items()})}
+        print(json.dumps(line), flush=True)
         return
 
     if args.workload == "ensemble":
