@@ -357,6 +357,19 @@ int sfp_sim_reset_timing(void* sim);
  * particle id, so shots are independent streams. */
 int sfp_sim_set_shots(void* sim, const sfp_field* terms, const int32_t* nterms, int32_t nshots);
 
+/* Device-side trajectory recorder (the callback pattern of
+ * example/trajectory.py:62-70 without a host round trip per step): after
+ * every `every`-th step counted from step0, x, y, z, ux, uy, uz of particles
+ * [0, n) of a lepton species are stored on the device ([max_rows][n][6],
+ * NaN for slots past the species' count) with the time after that step.
+ * n = 0 removes the recorder.  Runs as one small kernel at the end of each
+ * step, also inside graph replays. */
+int sfp_sim_set_recorder(void* sim, int32_t species, int64_t n, int64_t every, int64_t max_rows,
+                         uint64_t step0);
+/* rows recorded so far (synchronous); t_host [rows], rows_host [rows][n][6]
+ * (either may be NULL to query the count) */
+int sfp_sim_get_records(void* sim, double* t_host, double* rows_host, int64_t* rows);
+
 /* Use the learned photon-energy sampler (sfp_mlp_*) for the photon emission
  * of the leptons, or stop using it (mlp NULL).  Mixed precision only; needs
  * the optical-depth tables.  The network must outlive its use by the

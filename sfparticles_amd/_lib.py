@@ -116,6 +116,8 @@ SIGNATURES = {
     "sfp_mlp_shape": [P, C.POINTER(I32), C.POINTER(I32)],
     "sfp_mlp_rate": [P, P, P, I64, I32, P, P],
     "sfp_sim_set_rate_net": [P, P],
+    "sfp_sim_set_recorder": [P, I32, I64, I64, I64, U64],
+    "sfp_sim_get_records": [P, P, P, C.POINTER(I64)],
     "sfp_mlp_destroy": [P],
     "sfp_mlp_set_params": [P, P, P],
     "sfp_mlp_get_params": [P, P, P],

@@ -86,6 +86,17 @@ struct Halt {
   int64_t step;
 };
 
+// Device-side trajectory recorder: after every `every`-th step from step0,
+// x, y, z, ux, uy, uz of particles [0, n) of one species go to row r of buf
+// ([max_rows][n][6]) and the time to t[r].
+struct Recorder {
+  double* buf;
+  double* t;
+  int64_t n, every, max_rows;
+  uint64_t step0;
+  int32_t species, on;
+};
+
 struct Plan {
   SpeciesDev sp[SFP_MAX_SPECIES];
   ScanState* scan[SFP_MAX_SPECIES][NPH];  // [species][0 sweep list, 1 event ranks, 2 emissions]
@@ -115,6 +126,7 @@ struct Plan {
   const sfp_mlp_bins* net_bins;  // its per-bin table fallback
   const sfp_mlp_dev* rate_net;   // learned spin-resolved emission rate, or null
   const sfp_mlp_bins* rate_bins;
+  Recorder rec;
   int32_t nspecies;
   int32_t qed;
 };
@@ -1664,6 +1676,35 @@ __global__ void k_step_prologue(const Plan* plan, const StepArgs* __restrict__ a
   }
 }
 
+// end of a step: the recorder's row when this step is one it records
+// (particles past the species' count are recorded as NaN)
+__global__ void k_record(const Plan* __restrict__ plan, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
+  const Plan& P = *plan;
+  if (P.halt->flag || !P.rec.on) return;
+  const Recorder& R = P.rec;
+  if (a.step + 1 <= R.step0) return;
+  const uint64_t k1 = a.step + 1 - R.step0;
+  if (k1 % (uint64_t)R.every) return;
+  const int64_t r = (int64_t)(k1 / (uint64_t)R.every) - 1;
+  if (r >= R.max_rows) return;
+  const SpeciesDev& sp = P.sp[R.species];
+  const int64_t n = sp.cnt->n;
+  const double nan = __builtin_nan("");
+  double* row = R.buf + r * R.n * 6;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const bool in = i < n;
+    row[6 * i + 0] = in ? sp.x[i] : nan;
+    row[6 * i + 1] = in ? sp.y[i] : nan;
+    row[6 * i + 2] = in ? sp.z[i] : nan;
+    row[6 * i + 3] = in ? sp.ux[i] : nan;
+    row[6 * i + 4] = in ? sp.uy[i] : nan;
+    row[6 * i + 5] = in ? sp.uz[i] : nan;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) R.t[r] = a.t + a.dt;
+}
+
 // ------------------------------------------------------------- dispatch ----
 typedef void (*kfn)(const Plan*, int, const StepArgs*);
 
@@ -1776,6 +1817,7 @@ struct Sim {
   int64_t syncs = 0, replays = 0;
   int cus = 0;                       // compute units of the device
   bool net = false;                  // leptons use the learned photon-energy sampler
+  uint64_t rec_end = 0;              // recorder: one past the last step run
   bool rate_net = false;             // spin leptons use the learned emission rate
   sfp_fieldset* shot_fields = nullptr;  // device, one per shot
   int32_t nshots = 0;
@@ -2170,6 +2212,8 @@ extern "C" int sfp_sim_destroy(void* p) {
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   if (sim->args_dev) (void)hipFree(sim->args_dev);
   if (sim->args_host) (void)hipHostFree(sim->args_host);
+  if (sim->plan.rec.buf) (void)hipFree(sim->plan.rec.buf);
+  if (sim->plan.rec.t) (void)hipFree(sim->plan.rec.t);
   if (sim->stream) (void)hipStreamDestroy(sim->stream);
   for (int s = 0; s < SFP_MAX_SPECIES; ++s)
     for (int ph = 0; ph < NPH; ++ph)
@@ -2229,6 +2273,12 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
     if (first == 0) CHK(launch(sim, s, 0, j, tune));
     if (sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
     if (sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
+  }
+  if (sim->plan.rec.on) {
+    const int g = (int)std::min<int64_t>((sim->plan.rec.n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_record, dim3(std::max(g, 1)), dim3(256), 0, sim->stream,
+                       (const Plan*)sim->plan_dev, (const StepArgs*)(sim->args_dev + j));
+    HIPCHK(hipGetLastError());
   }
   return SFP_OK;
 }
@@ -2454,10 +2504,59 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
     k += batch;
   }
   *t = tt;
+  sim->rec_end = std::max<uint64_t>(sim->rec_end, step0 + (uint64_t)nsteps);
   return SFP_OK;
 }
 
 static int sim_collect_timing(Sim* sim) { return sim_collect_events(sim); }
+
+extern "C" int sfp_sim_set_recorder(void* p, int32_t species, int64_t n, int64_t every,
+                                    int64_t max_rows, uint64_t step0) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
+  if (sim->plan.sp[species].photon)
+    return set_err(SFP_E_ARG, "the recorder takes a lepton species (photon state is cached "
+                              "between the 8-step list rebuilds)");
+  if (n > 0 && (every < 1 || max_rows < 1))
+    return set_err(SFP_E_ARG, "every and max_rows must be >= 1");
+  Recorder& R = sim->plan.rec;
+  HIPCHK(hipDeviceSynchronize());
+  if (R.buf) HIPCHK(hipFree(R.buf));
+  if (R.t) HIPCHK(hipFree(R.t));
+  R = Recorder{};
+  if (n > 0) {
+    HIPCHK(hipMalloc(&R.buf, sizeof(double) * 6 * (size_t)n * (size_t)max_rows));
+    HIPCHK(hipMalloc(&R.t, sizeof(double) * (size_t)max_rows));
+    R.n = n;
+    R.every = every;
+    R.max_rows = max_rows;
+    R.step0 = step0;
+    R.species = species;
+    R.on = 1;
+  }
+  sim->rec_end = step0;
+  sim->graph_stale = true;
+  HIPCHK(hipMemcpy(sim->plan_dev, &sim->plan, sizeof(Plan), hipMemcpyHostToDevice));
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_get_records(void* p, double* t_host, double* rows_host, int64_t* rows) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  const Recorder& R = sim->plan.rec;
+  int64_t nr = 0;
+  if (R.on && sim->rec_end > R.step0)
+    nr = std::min<int64_t>(R.max_rows, (int64_t)((sim->rec_end - R.step0) / (uint64_t)R.every));
+  *rows = nr;
+  if (nr > 0) {
+    HIPCHK(hipDeviceSynchronize());
+    if (t_host) HIPCHK(hipMemcpy(t_host, R.t, sizeof(double) * nr, hipMemcpyDeviceToHost));
+    if (rows_host)
+      HIPCHK(hipMemcpy(rows_host, R.buf, sizeof(double) * 6 * (size_t)R.n * nr,
+                       hipMemcpyDeviceToHost));
+  }
+  return SFP_OK;
+}
 
 extern "C" int sfp_sim_set_grid(void* p, int32_t species, int32_t phase, int32_t grid) {
   Sim* sim = (Sim*)p;

@@ -202,6 +202,34 @@ class Simulation(object):
     def reset_timing(self):
         L.call("sfp_sim_reset_timing", self._sim)
 
+    def record(self, particles, steps: int, n: int = None, every: int = 1):
+        """Record x, y, z, ux, uy, uz of the first n particles of a lepton
+        species on the device after every `every`-th of the next `steps`
+        steps (the store_trajectory callback of example/trajectory.py:62-70,
+        without a host round trip per step); read them with records()."""
+        self._prepare()
+        sp = self.all_particles.index(particles)
+        n = int(particles.N_buffered if n is None else n)
+        rows = max(1, int(steps) // int(every))
+        self._rec_n = n
+        L.call("sfp_sim_set_recorder", self._sim, sp, n, int(every), rows, int(self.step))
+
+    def records(self) -> dict:
+        """Recorded rows: t [rows] and x, y, z, ux, uy, uz [rows, n]."""
+        r = C.c_int64(0)
+        if self._sim is not None:
+            L.call("sfp_sim_get_records", self._sim, None, None, C.byref(r))
+        rows = int(r.value)
+        n = getattr(self, "_rec_n", 0)
+        t = np.empty(rows)
+        buf = np.empty((rows, n, 6))
+        if rows:
+            L.call("sfp_sim_get_records", self._sim, t.ctypes.data, buf.ctypes.data, C.byref(r))
+        out = {"t": t}
+        for j, k in enumerate(("x", "y", "z", "ux", "uy", "uz")):
+            out[k] = buf[:, :, j].copy()
+        return out
+
     def set_grid(self, species: int, phase: int, grid: int):
         """Pin the workgroup count of a species' sweep (0) or QED (1) kernel;
         0 returns it to the tuner."""
