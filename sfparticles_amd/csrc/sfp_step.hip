@@ -217,6 +217,49 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
   return (int64_t)(*sh_pref) + woff + lrank;
 }
 
+// Rows of 256 list entries per tile of the QED kernels.  A tile's events are
+// ranked with one look-back after all of its rows and then appended from a
+// workgroup queue; an event leaves its particle untouched in the row pass and
+// is recomputed from that unchanged state when it is appended, so a
+// look-back (a round trip through the device-coherent flags, several
+// microseconds) is paid once per QROWS rows, not once per row.
+static constexpr int QROWS = 4;
+
+// After the rows of a tile have left their event ballots in mask[row][wave]:
+// the tile's event count goes through the look-back (exclusive prefix to
+// *sh_pref), and the tile positions (row * TILE + thread) of its events go to
+// q in (row, wave, lane) order, i.e. list order.  Returns the event count.
+// Called by every thread; q and *sh_pref are visible on return.
+__device__ inline uint32_t tile_events(const uint64_t (*mask)[WAVES], int rows, uint64_t* flags,
+                                       int64_t tile, uint32_t epoch, uint32_t* q,
+                                       uint64_t* sh_pref) {
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  __syncthreads();
+  if (wv == 0) {
+    uint64_t c = 0;
+    for (int k = lane; k < rows * WAVES; k += 64) c += (uint64_t)__popcll((&mask[0][0])[k]);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    const uint64_t pre = lookback(flags, tile, c, epoch);
+    if (lane == 0) *sh_pref = pre;
+  }
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t off = 0;
+  for (int row = 0; row < rows; ++row) {
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < WAVES; ++k) {
+      const uint32_t c = (uint32_t)__popcll(mask[row][k]);
+      if (k < wv) before += c;
+      total += c;
+    }
+    const uint64_t m = mask[row][wv];
+    if ((m >> lane) & 1ull) q[off + before + (uint32_t)__popcll(m & lt)] = (uint32_t)(row * TILE) + threadIdx.x;
+    off += total;
+  }
+  __syncthreads();
+  return off;
+}
+
 // The field set of particle i: its shot's in an ensemble (indices at or above
 // the number of shots take the last shot), else the common one.
 __device__ inline const sfp_fieldset* fields_of(const Plan& P, const SpeciesDev& sp, int64_t i,
@@ -465,7 +508,7 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1054,7 +1097,7 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // NETK: 0 tables, 1 the learned photon-energy sampler, 2 the learned
 // spin-resolved emission rate (mixed precision, rejection sampling)
 template <int KIND, int FM, bool MIX, int NETK>
-__global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1067,9 +1110,13 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[2 * WAVES];
   __shared__ uint64_t sh_pref;
+  __shared__ uint64_t sh_mask[NETK == 2 ? 1 : QROWS][WAVES];
+  __shared__ uint32_t sh_q[NETK == 2 ? 1 : QROWS * TILE];
 
   const int64_t n_hot = sp.cnt->n_hot;
-  const int64_t ntiles = (n_hot + TILE - 1) / TILE;
+  // tiles of QROWS rows, or of one row for the rate network
+  const int64_t span = NETK == 2 ? (int64_t)TILE : (int64_t)QROWS * TILE;
+  const int64_t ntiles = (n_hot + span - 1) / span;
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
   const uint32_t nwg = participants(ntiles);
@@ -1083,6 +1130,184 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
 
   uint32_t alive = 0;
   bool staged = false;
+  if constexpr (NETK != 2) {
+  constexpr int64_t SPAN = (int64_t)QROWS * TILE;
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    // tables go to LDS once the workgroup has work (a short list leaves most
+    // of the persistent grid without a tile)
+    if (!staged) {
+      stage_tables<KIND, false, MIX>(P, lds_rs, lds_od);
+      if (NET && KIND == KIND_RS) {  // the total photon rate for composition sampling
+        const int64_t nn = P.od.chi_N - 1;
+        for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
+        __syncthreads();
+      }
+      staged = true;
+    }
+
+    // row pass: the QED decision of every listed lepton; the ones without an
+    // event (emission, or with the learned sampler an emission candidate)
+    // are pushed and stored, the others are left untouched for the event pass
+#pragma unroll 1
+    for (int row = 0; row < QROWS; ++row) {
+      const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
+      const bool valid = k < n_hot;
+      int64_t i = 0;
+      double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
+      uint64_t pid = 0;
+      if (valid) {
+        i = (int64_t)sp.hot[k];
+        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+        if (KIND == KIND_OD) tau = sp.tau[i];
+        if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+        pid = sp.id[i];
+      }
+      uint32_t shot = 0;
+      const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
+      float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+      float igf = 1.0f;
+      double ig = 1.0;
+      if (valid) {
+        if (MIX) {
+          igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+        } else {
+          ig = sfp_inv_gamma_lepton(ux, uy, uz);
+        }
+      }
+      bool ev = false;
+      double delta = 0.0;
+      if (valid) {
+        if (NET) {
+          bool net = false;
+          float in0 = 0.0f, in1 = 0.0f;
+          double chi_n = 0.0, u_tab = 0.0;
+          ev = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+                                         fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
+                                         u_tab);
+        } else if (MIX) {
+          ev = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+                                            z, fx, fy, fz, igf, pid, tau, delta, (float)sx,
+                                            (float)sy, (float)sz);
+        } else {
+          ev = qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+                                          ux, uy, uz, ig, pid, tau, delta, sx, sy, sz);
+        }
+        if (!NET && !dA) ev = false;
+      }
+      const uint64_t m = __ballot(ev);
+      if (lane == 0) sh_mask[row][wv] = m;
+      if (valid && !ev) {
+        if (MIX) {
+          float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+          lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+          ux = fx; uy = fy; uz = fz;
+          sx = s0; sy = s1; sz = s2;
+        } else {
+          lepton_push<FM>(P, FS, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+        }
+        sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+        sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
+        if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
+        if (KIND == KIND_OD) sp.tau[i] = tau;
+        alive += 1;
+      }
+    }
+
+    // event pass, in list order, each decision recomputed from the lepton's
+    // unchanged state.  Tables: the photon appended already pushed (the
+    // reference pushes particles in the step that creates them), then recoil
+    // (cpu.py:78-88) and push (simulation.py:128-137).  Learned sampler: the
+    // candidate goes to the emission kernel, which samples its energy on the
+    // matrix cores in full waves, applies the hard-photon cut, appends the
+    // photon and pushes the lepton (queue order = index order, so the
+    // appends keep their order).
+    const uint32_t nev = tile_events(sh_mask, QROWS, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
+    for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
+      const int64_t k = tile * SPAN + (int64_t)sh_q[e];
+      const int64_t rank = (int64_t)sh_pref + (int64_t)e;
+      const int64_t i = (int64_t)sp.hot[k];
+      double x = sp.x[i], y = sp.y[i], z = sp.z[i];
+      double ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
+      double tau = KIND == KIND_OD ? sp.tau[i] : 0.0, sx = 0, sy = 0, sz = 0;
+      if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+      const uint64_t pid = sp.id[i];
+      uint32_t shot = 0;
+      const sfp_fieldset* FS = fields_of(P, sp, i, shot);
+      float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+      float igf = 1.0f;
+      double ig = 1.0;
+      if (MIX) {
+        igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+      } else {
+        ig = sfp_inv_gamma_lepton(ux, uy, uz);
+      }
+      double delta = 0.0;
+      if (NET) {
+        bool net = false;
+        float in0 = 0.0f, in1 = 0.0f;
+        double chi_n = 0.0, u_tab = 0.0;
+        (void)qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+                                        fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
+                                        u_tab);
+        EmitRec r;
+        r.i = (uint32_t)i;
+        r.net = net ? 1u : 0u;
+        r.in0 = in0;
+        r.in1 = in1;
+        r.chi = chi_n;
+        r.u_tab = u_tab;
+        r.delta = delta;
+        sp.emit[rank] = r;
+        if (KIND == KIND_OD) sp.tau[i] = tau;
+        continue;
+      }
+      if (MIX) {
+        (void)qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+                                           z, fx, fy, fz, igf, pid, tau, delta, (float)sx,
+                                           (float)sy, (float)sz);
+        const float df = (float)delta;
+        spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
+                     df * fz);
+        if (sp.rr == SFP_RR_PHOTON) {
+          const float f = 1.0f - (float)delta;
+          fx *= f;
+          fy *= f;
+          fz *= f;
+          igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
+        }
+        float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+        lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
+        ux = fx; uy = fy; uz = fz;
+        sx = s0; sy = s1; sz = s2;
+      } else {
+        (void)qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+                                         ux, uy, uz, ig, pid, tau, delta, sx, sy, sz);
+        spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, delta * ux,
+                     delta * uy, delta * uz);
+        if (sp.rr == SFP_RR_PHOTON) {
+          ux *= 1 - delta;
+          uy *= 1 - delta;
+          uz *= 1 - delta;
+          ig = sfp_inv_gamma_lepton(ux, uy, uz);
+        }
+        lepton_push<FM>(P, FS, sp, a, x, y, z, ux, uy, uz, sx, sy, sz, ig);
+      }
+      sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+      sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
+      if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
+      if (KIND == KIND_OD) sp.tau[i] = tau;
+      alive += 1;
+    }
+  }
+  } else {
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
@@ -1229,6 +1454,7 @@ __global__ void __launch_bounds__(TILE) k_hot(const Plan* __restrict__ plan, int
       if (KIND == KIND_OD) sp.tau[i] = tau;
       alive += 1;
     }
+  }
   }
 
   uint32_t cnt[1] = {alive};
@@ -1485,7 +1711,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 }
 
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1497,9 +1723,12 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
+  __shared__ uint64_t sh_mask[QROWS][WAVES];
+  __shared__ uint32_t sh_q[QROWS * TILE];
 
   const int64_t n_list = sp.cnt->n_hot;
-  const int64_t ntiles = (n_list + TILE - 1) / TILE;
+  constexpr int64_t SPAN = (int64_t)QROWS * TILE;
+  const int64_t ntiles = (n_list + SPAN - 1) / SPAN;
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
   const uint32_t nwg = participants(ntiles);
@@ -1511,6 +1740,16 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
   const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  // Window cache: the list is rebuilt every 8th step and kept in between,
+  // so the state of its first ccap entries lives in arrays indexed by list
+  // position for the window (coalesced), gathered from the store at the
+  // rebuild (or by the list kernel for appended photons) and written back
+  // at the window's last step; photons falling asleep or converting go
+  // straight to the store.
+  const bool rebuild = (a.rel & 7u) == 0;
+  const bool last = (a.rel & 7u) == 7u || a.rel + 1 == a.rel_end;
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
   bool staged = false;
@@ -1525,86 +1764,67 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
       staged = true;
     }
 
-    const int64_t k = tile * TILE + threadIdx.x;
-    bool live = false;
-    int64_t i = 0;
-    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
-    // Window cache: the list is rebuilt every 8th step and kept in between,
-    // so the state of its first ccap entries lives in arrays indexed by list
-    // position for the window (coalesced), gathered from the store at the
-    // rebuild (or by the list kernel for appended photons) and written back
-    // at the window's last step; photons falling asleep or converting go
-    // straight to the store.
-    const bool cached = k < sp.ccap;
-    const bool rebuild = (a.rel & 7u) == 0;
-    const bool last = (a.rel & 7u) == 7u || a.rel + 1 == a.rel_end;
-    if (k < n_list) {
-      i = (int64_t)sp.hot[k];
-      // a list kept from an earlier step holds photons that have fallen
-      // asleep since (counted by the list kernel) and holes: skipped
-      if (cached && !rebuild) {
-        if (sp.cwake[k] <= a.rel) {
-          x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
-          ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
-          live = true;
+#pragma unroll 1
+    for (int row = 0; row < QROWS; ++row) {
+      const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
+      bool live = false;
+      int64_t i = 0;
+      double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
+      const bool cached = k < sp.ccap;
+      if (k < n_list) {
+        i = (int64_t)sp.hot[k];
+        // a list kept from an earlier step holds photons that have fallen
+        // asleep since (counted by the list kernel) and holes: skipped
+        if (cached && !rebuild) {
+          if (sp.cwake[k] <= a.rel) {
+            x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
+            ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
+            live = true;
+          }
+        } else if (sp.wake[i] <= a.rel) {
+          const uint8_t pr = sp.pruned[i];
+          x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+          ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+          live = pr == 0;
+          if (!live) {
+            sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
+            if (cached) sp.cwake[k] = WAKE_NEVER;
+          }
+        } else if (cached) {
+          sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
         }
-      } else if (sp.wake[i] <= a.rel) {
-        const uint8_t pr = sp.pruned[i];
-        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-        live = pr == 0;
-        if (!live) {
-          sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
-          if (cached) sp.cwake[k] = WAKE_NEVER;
+      }
+      const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
+      uint32_t shot = 0;
+      const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
+      bool hot = false;
+      if (live && sp.push) {
+        float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
+        float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+        hot = qed_possible(P, true, chi_b, ig, a.dt);
+      }
+      bool event = false;
+      double delta = 0.0;
+      if (hot) {
+        if (KIND == KIND_OD) tau = sp.tau[i];
+        if (MIX) {
+          event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x,
+                                              y, z, (float)ux, (float)uy, (float)uz, (float)ig,
+                                              sp.id[i], tau, delta);
+        } else {
+          event = qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y,
+                                            z, ux, uy, uz, ig, sp.id[i], tau, delta);
         }
-      } else if (cached) {
-        sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
       }
-    }
-    const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
-    uint32_t shot = 0;
-    const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
-    bool hot = false;
-    if (live && sp.push) {
-      float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
-      float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
-      hot = qed_possible(P, true, chi_b, ig, a.dt);
-    }
-    bool event = false;
-    double delta = 0.0;
-    if (hot) {
-      if (KIND == KIND_OD) tau = sp.tau[i];
-      if (MIX) {
-        event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
-                                            z, (float)ux, (float)uy, (float)uz, (float)ig,
-                                            sp.id[i], tau, delta);
-      } else {
-        event = qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
-                                          ux, uy, uz, ig, sp.id[i], tau, delta);
+      if (!dA && !dB) event = false;
+      const uint64_t m = __ballot(event);
+      if (lane == 0) sh_mask[row][wv] = m;
+
+      if (live) {
+        cnt[1] += 1;
+        cnt[2] += hot ? 1u : 0u;
       }
-    }
-    if (!dA && !dB) event = false;
-
-    int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
-    if (event) {
-      double dp = 1.0 - delta;
-      if (dA)
-        spawn_lepton<FM, MIX>(P, FS, shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
-                              delta * ux, delta * uy, delta * uz);
-      if (dB)
-        spawn_lepton<FM, MIX>(P, FS, shot, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z, dp * ux,
-                              dp * uy, dp * uz);
-    }
-
-    if (live) {
-      cnt[1] += 1;
-      cnt[2] += hot ? 1u : 0u;
-      if (event) {
-        // the converted photon becomes a hole (cpu.py:189-190), not pushed
-        sp.pruned[i] = 1;
-        sp.wake[i] = WAKE_NEVER;
-        if (cached) sp.cwake[k] = WAKE_NEVER;
-      } else {
+      if (live && !event) {
         uint32_t nst = 1;
         if (!sp.push) {
           nst = a.rel_end - a.rel;
@@ -1644,6 +1864,49 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
         }
         cnt[0] += 1;
       }
+    }
+
+    // the tile's conversions in list order, recomputed from the photons'
+    // unchanged state: the pair at the photon's position, electron with
+    // delta of its momentum, positron with 1 - delta (cpu.py:142-191); the
+    // photon becomes a hole (cpu.py:189-190), not pushed
+    const uint32_t nev = tile_events(sh_mask, QROWS, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
+    for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
+      const int64_t k = tile * SPAN + (int64_t)sh_q[e];
+      const int64_t rank = (int64_t)sh_pref + (int64_t)e;
+      const int64_t i = (int64_t)sp.hot[k];
+      const bool cached = k < sp.ccap;
+      double x, y, z, ux, uy, uz;
+      if (cached && !rebuild) {
+        x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
+        ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
+      } else {
+        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+      }
+      const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+      uint32_t shot = 0;
+      const sfp_fieldset* FS = fields_of(P, sp, i, shot);
+      double tau = KIND == KIND_OD ? sp.tau[i] : 0.0;
+      double delta = 0.0;
+      if (MIX) {
+        (void)qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
+                                          z, (float)ux, (float)uy, (float)uz, (float)ig, sp.id[i],
+                                          tau, delta);
+      } else {
+        (void)qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
+                                        ux, uy, uz, ig, sp.id[i], tau, delta);
+      }
+      const double dp = 1.0 - delta;
+      if (dA)
+        spawn_lepton<FM, MIX>(P, FS, shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
+                              delta * ux, delta * uy, delta * uz);
+      if (dB)
+        spawn_lepton<FM, MIX>(P, FS, shot, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z,
+                              dp * ux, dp * uy, dp * uz);
+      sp.pruned[i] = 1;
+      sp.wake[i] = WAKE_NEVER;
+      if (cached) sp.cwake[k] = WAKE_NEVER;
     }
   }
 
