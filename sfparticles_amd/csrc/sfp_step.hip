@@ -222,8 +222,15 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
 // workgroup queue; an event leaves its particle untouched in the row pass and
 // is recomputed from that unchanged state when it is appended, so a
 // look-back (a round trip through the device-coherent flags, several
-// microseconds) is paid once per QROWS rows, not once per row.
-static constexpr int QROWS = 4;
+// microseconds) is paid once per tile, not once per row.  Short lists take
+// fewer rows per tile, so that every workgroup of the grid has a tile.
+static constexpr int QROWS = 8;
+
+__device__ inline int qed_rows(int64_t n_list) {
+  const int64_t per_round = (int64_t)gridDim.x * TILE;
+  const int64_t r = (n_list + per_round - 1) / per_round;
+  return r < 1 ? 1 : (r > QROWS ? QROWS : (int)r);
+}
 
 // After the rows of a tile have left their event ballots in mask[row][wave]:
 // the tile's event count goes through the look-back (exclusive prefix to
@@ -1114,8 +1121,9 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
   __shared__ uint32_t sh_q[NETK == 2 ? 1 : QROWS * TILE];
 
   const int64_t n_hot = sp.cnt->n_hot;
-  // tiles of QROWS rows, or of one row for the rate network
-  const int64_t span = NETK == 2 ? (int64_t)TILE : (int64_t)QROWS * TILE;
+  // tiles of up to QROWS rows, or of one row for the rate network
+  const int rows = NETK == 2 ? 1 : qed_rows(n_hot);
+  const int64_t span = (int64_t)rows * TILE;
   const int64_t ntiles = (n_hot + span - 1) / span;
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
@@ -1131,7 +1139,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
   uint32_t alive = 0;
   bool staged = false;
   if constexpr (NETK != 2) {
-  constexpr int64_t SPAN = (int64_t)QROWS * TILE;
+  const int64_t SPAN = span;
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
   while (true) {
@@ -1156,7 +1164,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
     // event (emission, or with the learned sampler an emission candidate)
     // are pushed and stored, the others are left untouched for the event pass
 #pragma unroll 1
-    for (int row = 0; row < QROWS; ++row) {
+    for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
       const bool valid = k < n_hot;
       int64_t i = 0;
@@ -1229,7 +1237,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
     // matrix cores in full waves, applies the hard-photon cut, appends the
     // photon and pushes the lepton (queue order = index order, so the
     // appends keep their order).
-    const uint32_t nev = tile_events(sh_mask, QROWS, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
+    const uint32_t nev = tile_events(sh_mask, rows, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t k = tile * SPAN + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
@@ -1727,7 +1735,8 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
   __shared__ uint32_t sh_q[QROWS * TILE];
 
   const int64_t n_list = sp.cnt->n_hot;
-  constexpr int64_t SPAN = (int64_t)QROWS * TILE;
+  const int rows = qed_rows(n_list);
+  const int64_t SPAN = (int64_t)rows * TILE;
   const int64_t ntiles = (n_list + SPAN - 1) / SPAN;
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
@@ -1765,7 +1774,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
     }
 
 #pragma unroll 1
-    for (int row = 0; row < QROWS; ++row) {
+    for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
       bool live = false;
       int64_t i = 0;
@@ -1870,7 +1879,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
     // unchanged state: the pair at the photon's position, electron with
     // delta of its momentum, positron with 1 - delta (cpu.py:142-191); the
     // photon becomes a hole (cpu.py:189-190), not pushed
-    const uint32_t nev = tile_events(sh_mask, QROWS, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
+    const uint32_t nev = tile_events(sh_mask, rows, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t k = tile * SPAN + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
