@@ -148,7 +148,7 @@ def photon_energy_sample(g, chunks=100, size=2000, seed=1):
     return np.concatenate(out)
 
 
-def gpu_run(n, steps, warmup, nsteps_config, precision="fp64"):
+def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250):
     from sfparticles_amd import _lib as L
     if warmup > 0:
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision)
@@ -160,7 +160,7 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64"):
     t0 = time.perf_counter()
     done = 0
     while done < steps:  # chunks, so that progress is visible during long runs
-        m = min(250, steps - done)
+        m = min(chunk, steps - done)
         sim.run_steps(m, timing=True)
         done += m
         print(f"[bench] {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--chunk", type=int, default=250,
+                    help="head-on: steps per Simulation.run_steps call (a progress line each)")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--workload", choices=["head-on", "ensemble", "cascade"], default="head-on")
     ap.add_argument("--shots", type=int, default=256)
@@ -442,7 +444,7 @@ def main():
 
     def measure(precision):
         elapsed, stats, ktime, counts, spectrum = gpu_run(n, args.steps, args.warmup,
-                                                          nsteps_config, precision)
+                                                          nsteps_config, precision, args.chunk)
         psteps = sum(s["psteps"] for s in stats)
         # dominant kernel: largest accumulated time over (species, phase); one
         # launch of each kernel per timed step does the work (launches replayed

@@ -383,7 +383,7 @@ int sfp_sim_set_sampler(void* sim, void* mlp);
  * counterpart in the reference. */
 int sfp_sim_set_rate_net(void* sim, void* mlp);
 
-/* Counters of one species since the last reset (synchronous), stats[14]:
+/* Counters of one species since the last reset (synchronous), stats[15]:
  * [0] particle-steps (live particles after each step, the reference's
  *     ns/particle denominator, simulation.py:150-151)
  * [1] particles read by the sweep kernel   [2] particles appended by the species
@@ -394,7 +394,10 @@ int sfp_sim_set_rate_net(void* sim, void* mlp);
  *     wake step for these)                 [9] multi-step pushes started
  * [10] batches run as a graph replay       [11] graph captures
  * [12] tuning batches (10-12: of the simulation, not the species)
- * [13] emission candidates handed to the emission kernel (sum over steps) */
+ * [13] emission candidates handed to the emission kernel (sum over steps)
+ * [14] run calls that restarted the photon sleep clock (every photon visited
+ *      at their first step): the first one, and those that do not continue
+ *      the previous call or follow a change of a store from the host */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
 int sfp_sim_reset_stats(void* sim);
 

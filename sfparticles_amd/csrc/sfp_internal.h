@@ -93,6 +93,10 @@ extern const size_t kAttrSize[SFP_NATTR];
 struct Store {
   int32_t flags = 0;
   int64_t cap = 0;
+  // bumped by every host-side change of the contents (upload, prune, a new
+  // count, a writable pointer handed out): a simulation keeps photon sleep
+  // state across run calls only while it stays the same
+  uint64_t version = 0;
   void* p[SFP_NATTR] = {};
   Counts* cnt = nullptr;  // device
 };

@@ -801,6 +801,7 @@ extern "C" int sfp_store_reserve(void* store, int64_t capacity) {
   Store* s = (Store*)store;
   if (!s) return set_err(SFP_E_ARG, "null store");
   if (capacity <= s->cap) return SFP_OK;
+  s->version++;
   HIPCHK(hipDeviceSynchronize());
   for (int a = 0; a < SFP_NATTR; ++a) {
     if (!s->p[a]) continue;
@@ -829,6 +830,7 @@ extern "C" int sfp_store_ptr(void* store, int32_t attr, void** dev) {
   Store* s = (Store*)store;
   if (!s || attr < 0 || attr >= SFP_NATTR) return set_err(SFP_E_ARG, "bad store/attribute");
   CHK(store_alloc_attr(s, attr));
+  s->version++;
   *dev = s->p[attr];
   return SFP_OK;
 }
@@ -841,6 +843,7 @@ extern "C" int sfp_store_set_count(void* store, int64_t n, uint64_t next_id) {
   Counts c{};
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(&c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
+  if (c.n != n || c.next_id != (int64_t)next_id) s->version++;
   c.n = n;
   c.next_id = (int64_t)next_id;
   c.n_old = n;
@@ -867,6 +870,7 @@ extern "C" int sfp_store_upload(void* store, int32_t attr, const void* host, int
   if (off < 0 || n < 0 || off + n > s->cap) return set_err(SFP_E_ARG, "upload out of range");
   CHK(store_alloc_attr(s, attr));
   if (n == 0) return SFP_OK;
+  s->version++;
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy((char*)s->p[attr] + kAttrSize[attr] * off, host, kAttrSize[attr] * n,
                    hipMemcpyHostToDevice));
@@ -894,6 +898,7 @@ __global__ void k_gather(const T* src, T* dst, const int64_t* idx, int64_t n) {
 extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
   Store* s = (Store*)store;
   if (!s) return set_err(SFP_E_ARG, "null store");
+  s->version++;
   int64_t n = 0;
   uint64_t next_id = 0;
   CHK(sfp_store_get_count(s, &n, &next_id));

@@ -32,6 +32,10 @@
 #include "sfp_mlp.h"
 
 static constexpr int TILE = 256;
+// Photons fall asleep and wake on steps that are multiples of WIN (counted
+// from the start of a run); the photon visit list is rebuilt on those steps
+// and kept (and extended) in between.
+static constexpr uint32_t WIN = 8;
 static constexpr int WAVES = TILE / 64;
 // kernels of a species and step: 0 sweep, 1 QED, 2 emission (leptons with the
 // learned photon-energy sampler)
@@ -58,7 +62,8 @@ struct SpeciesDev {
   double* tau;
   double *sx, *sy, *sz;
   uint32_t* hot;   // this step's QED list (ascending particle indices)
-  uint32_t* wake;  // photons: run-relative step of the next visit (multi-step push), or null
+  uint32_t* wake;  // photons: step of the next visit (multi-step push), or null
+  uint32_t* since;  // QED photons asleep: the step whose position the store holds
   uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
   EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
   // photons: state of the first `ccap` entries of the visit list, by list
@@ -135,6 +140,9 @@ struct StepArgs {
   double t, t_half, dt, cdt_half, cdt, ll_factor;
   uint64_t step;
   uint32_t rel, rel_end;  // step index within this run call, and the run's length
+  // step on the photon sleep clock (steps since the simulation's wake origin):
+  // QED photons sleep on it, across run calls that continue each other
+  uint32_t wrel, pad;
 };
 
 // --------------------------------------------------------- look-back scan --
@@ -471,28 +479,52 @@ __device__ inline uint32_t next_epoch(const ScanState* scan) {
 }
 
 // --------------------------------------------------- multi-step photon push --
+// The photon visit list is rebuilt (and photons fall asleep and wake) on the
+// steps of the sleep clock that are multiples of WIN, and on the first step of
+// every run call.
+__device__ inline bool list_rebuild(const StepArgs& a) {
+  return (a.wrel % WIN) == 0 || a.rel == 0;
+}
+
+// A QED photon asleep until this step holds the position of the step it fell
+// asleep at (since): the two position increments of every step it slept are
+// added now -- the same sequential FP64 additions, in closed form, that the
+// steps one at a time would have made (cpu.py:5-19).
+__device__ inline void photon_wake_position(const SpeciesDev& sp, const StepArgs& a, int64_t i,
+                                            double ig, double& x, double& y, double& z,
+                                            double ux, double uy, double uz) {
+  const uint32_t s0 = sp.since[i];
+  if (s0 >= a.wrel) return;
+  const uint32_t n = 2u * (a.wrel - s0);
+  x = sfp_repeat_add(x, a.cdt_half * ig * ux, n);
+  y = sfp_repeat_add(y, a.cdt_half * ig * uy, n);
+  z = sfp_repeat_add(z, a.cdt_half * ig * uz, n);
+}
+
 // A photon moves on a straight line at c and its momentum never changes, so
 // while no QED event is possible its push is the same two position
 // increments every step.  Number of steps, from this one on, for which the
 // envelope bound along the photon's own path proves that no event is possible
-// (1 if only this step is proven).  Windows end at the end of the run or at
-// the next multiple of 512, 64 or 8 steps, so that neighbouring photons --
-// one wave's lanes -- wake on the same steps.  Photons of a species without
-// QED sleep to the end of the run.
+// (1 if only this step is proven).  Windows end at the next multiple of 512,
+// 64 or WIN steps of the sleep clock, so that neighbouring photons -- one
+// wave's lanes -- wake on the same steps, and may last past the end of the
+// run.  Photons of a species without QED sleep to the end of the run.
 template <int FM>
 __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, const StepArgs& a,
                                        bool qed, double x,
                                        double y, double z, double ux, double uy, double uz,
                                        double ig) {
-  const uint32_t rem = a.rel_end - a.rel;
-  if (rem <= 1) return 1;
-  if (!qed) return rem;
+  if (!qed) {  // it only moves: to the end of the run
+    const uint32_t rem = a.rel_end - a.rel;
+    return rem <= 1 ? 1 : rem;
+  }
+  // QED photons, on the sleep clock: a sleep may last past the end of the run
   const double vx = SFP_C * ig * ux, vy = SFP_C * ig * uy, vz = SFP_C * ig * uz;
-  const uint32_t q[3] = {512u, 64u, 8u};
-  for (int k = 0; k < 4; ++k) {
-    uint32_t end = (k == 0) ? a.rel_end : (a.rel / q[k - 1] + 1u) * q[k - 1];
-    if (end > a.rel_end || end < a.rel + 2u || (k > 0 && end == a.rel_end)) continue;
-    uint32_t d = end - a.rel;
+  const uint32_t q[3] = {512u, 64u, WIN};
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t end = (a.wrel / q[k] + 1u) * q[k];
+    if (end < a.wrel + 2u) continue;
+    const uint32_t d = end - a.wrel;
     float S = sfp_field_bound_f32_path<FM>(FS, x, y, z, a.t, vx, vy, vz,
                                            a.dt * (double)d * 1.000001);
     float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
@@ -626,7 +658,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restr
           // a wave take them together
           if (wake)
             nst = !sp.push ? a.rel_end - a.rel
-                           : ((a.rel & 7u) == 0 ? sleep_steps<FM>(P, FS, a, QED, x, y, z, ux, uy, uz, ig)
+                           : ((a.rel % WIN) == 0 ? sleep_steps<FM>(P, FS, a, QED, x, y, z, ux, uy, uz, ig)
                                                 : 1u);
           if (sp.push) {
             // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
@@ -1163,15 +1195,18 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
     // row pass: the QED decision of every listed lepton; the ones without an
     // event (emission, or with the learned sampler an emission candidate)
     // are pushed and stored, the others are left untouched for the event pass
+    // the list index of the next row is loaded while this row computes
+    const int64_t k0 = tile * SPAN + threadIdx.x;
+    uint32_t i_next = k0 < n_hot ? sp.hot[k0] : 0u;
 #pragma unroll 1
     for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
       const bool valid = k < n_hot;
-      int64_t i = 0;
+      const int64_t i = valid ? (int64_t)i_next : 0;
+      i_next = (row + 1 < rows && k + TILE < n_hot) ? sp.hot[k + TILE] : 0u;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
       uint64_t pid = 0;
       if (valid) {
-        i = (int64_t)sp.hot[k];
         x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
         ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
         if (KIND == KIND_OD) tau = sp.tau[i];
@@ -1619,7 +1654,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   __shared__ uint64_t sh_pref;
   const int64_t n_old = sp.cnt->n_old;
   ScanState* scan = P.scan[s][0];
-  if ((a.rel & 7u) != 0) {
+  if (!list_rebuild(a)) {
     // Photons fall asleep and wake only on every 8th step: in between, the
     // list of the previous step stays valid (holes it holds are skipped) and
     // is extended by the photons appended since, whose indices are larger.
@@ -1672,8 +1707,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 #pragma unroll
     for (int j = 0; j < AW; ++j) {
       const bool in = i0 + j < n_old;
-      up |= (uint32_t)(in && w[j] <= a.rel) << j;
-      cnt[0] += (uint32_t)(in && w[j] > a.rel && w[j] != WAKE_NEVER);
+      up |= (uint32_t)(in && w[j] <= a.wrel) << j;
+      cnt[0] += (uint32_t)(in && w[j] > a.wrel && w[j] != WAKE_NEVER);
     }
     const uint32_t c = (uint32_t)__popc(up);
     cnt[1] += c;
@@ -1757,8 +1792,8 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
   // rebuild (or by the list kernel for appended photons) and written back
   // at the window's last step; photons falling asleep or converting go
   // straight to the store.
-  const bool rebuild = (a.rel & 7u) == 0;
-  const bool last = (a.rel & 7u) == 7u || a.rel + 1 == a.rel_end;
+  const bool rebuild = list_rebuild(a);
+  const bool last = (a.wrel + 1) % WIN == 0 || a.rel + 1 == a.rel_end;
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
   bool staged = false;
@@ -1773,37 +1808,46 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
       staged = true;
     }
 
+    // the list index of the next row is loaded while this row computes
+    const int64_t k0 = tile * SPAN + threadIdx.x;
+    uint32_t i_next = k0 < n_list ? sp.hot[k0] : 0u;
 #pragma unroll 1
     for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
       bool live = false;
-      int64_t i = 0;
+      const int64_t i = k < n_list ? (int64_t)i_next : 0;
+      i_next = (row + 1 < rows && k + TILE < n_list) ? sp.hot[k + TILE] : 0u;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
       const bool cached = k < sp.ccap;
+      bool woke = false;  // asleep until this step
       if (k < n_list) {
-        i = (int64_t)sp.hot[k];
         // a list kept from an earlier step holds photons that have fallen
         // asleep since (counted by the list kernel) and holes: skipped
         if (cached && !rebuild) {
-          if (sp.cwake[k] <= a.rel) {
+          if (sp.cwake[k] <= a.wrel) {
             x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
             ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
             live = true;
           }
-        } else if (sp.wake[i] <= a.rel) {
-          const uint8_t pr = sp.pruned[i];
-          x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-          ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-          live = pr == 0;
-          if (!live) {
-            sp.wake[i] = WAKE_NEVER;  // a hole: never visited again in this run
-            if (cached) sp.cwake[k] = WAKE_NEVER;
+        } else {
+          const uint32_t wk = sp.wake[i];
+          if (wk <= a.wrel) {
+            const uint8_t pr = sp.pruned[i];
+            x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+            ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+            live = pr == 0;
+            woke = live && rebuild && wk == a.wrel;
+            if (!live) {
+              sp.wake[i] = WAKE_NEVER;  // a hole: never visited again
+              if (cached) sp.cwake[k] = WAKE_NEVER;
+            }
+          } else if (cached) {
+            sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
           }
-        } else if (cached) {
-          sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
         }
       }
       const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
+      if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
       uint32_t shot = 0;
       const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
       bool hot = false;
@@ -1837,38 +1881,38 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
         uint32_t nst = 1;
         if (!sp.push) {
           nst = a.rel_end - a.rel;
-        } else if (!hot && (a.rel & 7u) == 0) {
+        } else if (!hot && rebuild) {
           nst = sleep_steps<FM>(P, FS, a, true, x, y, z, ux, uy, uz, ig);
         }
         if (sp.push) {
-          // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
-          // increment each time
-          const double hx = a.cdt_half * ig * ux, hy = a.cdt_half * ig * uy,
-                       hz = a.cdt_half * ig * uz;
           if (nst == 1) {
+            // cpu.py:5-19 twice per step: x += (c dt/2 / |u|) ux, the same
+            // increment each time
+            const double hx = a.cdt_half * ig * ux, hy = a.cdt_half * ig * uy,
+                         hz = a.cdt_half * ig * uz;
             x += hx; x += hx;
             y += hy; y += hy;
             z += hz; z += hz;
-          } else {  // the same 2 nst additions, in closed form
-            x = sfp_repeat_add(x, hx, 2 * nst);
-            y = sfp_repeat_add(y, hy, 2 * nst);
-            z = sfp_repeat_add(z, hz, 2 * nst);
-          }
-          if (!cached || nst > 1 || last) {
-            sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-          }
-          if (cached && nst == 1 && !last) {
-            sp.cx[k] = x; sp.cy[k] = y; sp.cz[k] = z;
-            if (rebuild) {
-              sp.cux[k] = ux; sp.cuy[k] = uy; sp.cuz[k] = uz;
-              sp.cwake[k] = 0u;
+            if (!cached || last) {
+              sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+            } else {
+              sp.cx[k] = x; sp.cy[k] = y; sp.cz[k] = z;
+              if (rebuild) {
+                sp.cux[k] = ux; sp.cuy[k] = uy; sp.cuz[k] = uz;
+                sp.cwake[k] = 0u;
+              }
             }
+          } else if (woke) {
+            // falls asleep again: the store keeps this step's position (the
+            // increments are added when it wakes, or at the end of the run)
+            sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
           }
         }
         if (hot && KIND == KIND_OD) sp.tau[i] = tau;
         if (nst > 1) {
-          sp.wake[i] = a.rel + nst;
-          if (cached) sp.cwake[k] = a.rel + nst;
+          sp.wake[i] = a.wrel + nst;
+          sp.since[i] = a.wrel;
+          if (cached) sp.cwake[k] = a.wrel + nst;
           cnt[3] += 1;
         }
         cnt[0] += 1;
@@ -1886,14 +1930,17 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
       const int64_t i = (int64_t)sp.hot[k];
       const bool cached = k < sp.ccap;
       double x, y, z, ux, uy, uz;
+      bool woke = false;
       if (cached && !rebuild) {
         x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
         ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
       } else {
         x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
         ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+        woke = rebuild && sp.wake[i] == a.wrel;
       }
       const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+      if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
       uint32_t shot = 0;
       const sfp_fieldset* FS = fields_of(P, sp, i, shot);
       double tau = KIND == KIND_OD ? sp.tau[i] : 0.0;
@@ -1944,7 +1991,7 @@ __global__ void k_step_prologue(const Plan* plan, const StepArgs* __restrict__ a
   if (s < plan->nspecies) {
     Counts* c = plan->sp[s].cnt;
     c->n_old = c->n;
-    if ((a.rel & 7u) == 0) c->asleep_n = 0;
+    if (list_rebuild(a)) c->asleep_n = 0;
   }
 }
 
@@ -1975,6 +2022,28 @@ __global__ void k_record(const Plan* __restrict__ plan, const StepArgs* __restri
     row[6 * i + 5] = in ? sp.uz[i] : nan;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) R.t[r] = a.t + a.dt;
+}
+
+// end of a run: QED photons asleep past its last step get the position of
+// that step (the store must be current when the run returns), and keep
+// sleeping from it
+__global__ void k_catchup(const Plan* __restrict__ plan, int s, uint32_t wend, double cdt_half) {
+  const SpeciesDev& sp = plan->sp[s];
+  const int64_t n = sp.cnt->n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = sp.wake[i];
+    if (w == WAKE_NEVER || w < wend) continue;
+    const uint32_t s0 = sp.since[i];
+    if (s0 >= wend || sp.pruned[i]) continue;
+    const double ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
+    const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+    const uint32_t m = 2u * (wend - s0);
+    sp.x[i] = sfp_repeat_add(sp.x[i], cdt_half * ig * ux, m);
+    sp.y[i] = sfp_repeat_add(sp.y[i], cdt_half * ig * uy, m);
+    sp.z[i] = sfp_repeat_add(sp.z[i], cdt_half * ig * uz, m);
+    sp.since[i] = wend;
+  }
 }
 
 // ------------------------------------------------------------- dispatch ----
@@ -2069,6 +2138,13 @@ struct Sim {
   std::vector<EmitRec*> emit_mem;   // leptons with the learned sampler
   std::vector<int64_t> emit_cap;
   std::vector<uint32_t*> wake_mem;  // photon species: multi-step push bookkeeping
+  std::vector<uint32_t*> since_mem;  // QED photon species: step of the stored position
+  // the sleep clock: QED photons keep sleeping across run calls while each
+  // call continues the previous one and no store was changed from the host
+  uint64_t wake_origin = 0, wake_end = 0;
+  bool wake_valid = false;
+  int64_t clock_restarts = 0;
+  std::vector<uint64_t> wake_version;
   std::vector<int64_t> wake_cap;
   std::vector<char*> cache_mem;     // photon species: window cache of the visit list
   std::vector<int64_t> cache_cap;
@@ -2176,10 +2252,20 @@ static int sim_refresh_plan(Sim* sim) {
                          hipMemcpyDeviceToDevice));
         HIPCHK(hipFree(sim->wake_mem[s]));
       }
+      uint32_t* sn = nullptr;
+      HIPCHK(hipMalloc(&sn, sizeof(uint32_t) * want));
+      HIPCHK(hipMemset(sn, 0, sizeof(uint32_t) * want));
+      if (sim->since_mem[s]) {
+        HIPCHK(hipMemcpy(sn, sim->since_mem[s], sizeof(uint32_t) * sim->wake_cap[s],
+                         hipMemcpyDeviceToDevice));
+        HIPCHK(hipFree(sim->since_mem[s]));
+      }
       sim->wake_mem[s] = w;
+      sim->since_mem[s] = sn;
       sim->wake_cap[s] = want;
     }
     d.wake = sim->wake_mem[s];
+    d.since = sim->since_mem[s];
     {
       const int64_t cc = sim->cache_cap[s];
       double* c = (double*)sim->cache_mem[s];
@@ -2364,6 +2450,8 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   sim->emit_cap.assign(nspecies, 0);
   sim->hot_cap.assign(nspecies, 0);
   sim->wake_mem.assign(nspecies, nullptr);
+  sim->since_mem.assign(nspecies, nullptr);
+  sim->wake_version.assign(nspecies, 0);
   sim->wake_cap.assign(nspecies, 0);
   sim->cache_mem.assign(nspecies, nullptr);
   sim->cache_cap.assign(nspecies, 0);
@@ -2436,6 +2524,7 @@ extern "C" int sfp_sim_set_shots(void* p, const sfp_field* terms, const int32_t*
   Sim* sim = (Sim*)p;
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   if (nshots < 1 || nshots > (1 << 20)) return set_err(SFP_E_ARG, "between 1 and 2^20 shots");
+  sim->wake_valid = false;  // sleeps were proven against the old fields
   std::vector<sfp_fieldset> fs(nshots);
   int mode = -1;
   int64_t off = 0;
@@ -2472,6 +2561,8 @@ extern "C" int sfp_sim_destroy(void* p) {
   for (auto q : sim->emit_mem)
     if (q) (void)hipFree(q);
   for (auto w : sim->wake_mem)
+    if (w) (void)hipFree(w);
+  for (auto w : sim->since_mem)
     if (w) (void)hipFree(w);
   for (auto c : sim->cache_mem)
     if (c) (void)hipFree(c);
@@ -2696,6 +2787,13 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   for (int s = 0; s < sim->nsp; ++s)
     if (sim->plan.sp[s].cap != sim->stores[s]->cap || sim->plan.sp[s].x != sim->stores[s]->p[SFP_X])
       dirty = true;
+  // the sleep clock goes on if this run continues the last one and no store
+  // changed in between (it restarts, waking every photon, otherwise)
+  bool keep = sim->wake_valid && !dirty && step0 == sim->wake_end &&
+              step0 + (uint64_t)std::max<int64_t>(nsteps, 0) - sim->wake_origin < (1ull << 31);
+  for (int s = 0; s < sim->nsp; ++s)
+    if (sim->stores[s]->version != sim->wake_version[s]) keep = false;
+  sim->wake_valid = false;
   if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
     return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
   // window cache of the photon visit lists: a quarter of the store (longer
@@ -2715,11 +2813,17 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
     }
   }
   if (dirty) CHK(sim_refresh_plan(sim));
-  // every photon is visited at the first step of a run (positions are current
-  // between runs; the store may have changed)
-  for (int s = 0; s < sim->nsp; ++s)
-    if (sim->wake_mem[s])
-      HIPCHK(hipMemsetAsync(sim->wake_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], sim->stream));
+  // Photons without QED sleep to the end of each run, so all of them are
+  // visited at its first step; QED photons too when the sleep clock restarts
+  if (!keep) {
+    sim->wake_origin = step0;
+    sim->clock_restarts++;
+  }
+  for (int s = 0; s < sim->nsp; ++s) {
+    if (!sim->wake_mem[s] || (keep && sim->qed[s])) continue;
+    HIPCHK(hipMemsetAsync(sim->wake_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], sim->stream));
+    HIPCHK(hipMemsetAsync(sim->since_mem[s], 0, sizeof(uint32_t) * sim->wake_cap[s], sim->stream));
+  }
   if (autotune) CHK(sim_read_counts(sim));
   const double dt = sim->params.dt;
   int64_t k = 0;
@@ -2738,6 +2842,8 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       a.step = step0 + (uint64_t)(k + j);
       a.rel = (uint32_t)(k + j);
       a.rel_end = (uint32_t)nsteps;
+      a.wrel = (uint32_t)(a.step - sim->wake_origin);
+      a.pad = 0;
       tt = tt + dt;
     }
     HIPCHK(hipMemcpyAsync(sim->args_dev, sim->args_host, sizeof(StepArgs) * batch,
@@ -2777,6 +2883,20 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   }
   *t = tt;
   sim->rec_end = std::max<uint64_t>(sim->rec_end, step0 + (uint64_t)nsteps);
+  // sleeping QED photons: positions of the run's last step
+  if (nsteps > 0) {
+    const uint32_t wend = (uint32_t)(step0 + (uint64_t)nsteps - sim->wake_origin);
+    for (int s = 0; s < sim->nsp; ++s) {
+      if (!sim->wake_mem[s] || !sim->qed[s] || !sim->plan.sp[s].push) continue;
+      hipLaunchKernelGGL(k_catchup, dim3(sim->cus * 8), dim3(256), 0, sim->stream,
+                         (const Plan*)sim->plan_dev, s, wend, SFP_C * (0.5 * dt));
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(sim->stream));
+  }
+  sim->wake_end = step0 + (uint64_t)nsteps;
+  for (int s = 0; s < sim->nsp; ++s) sim->wake_version[s] = sim->stores[s]->version;
+  sim->wake_valid = true;
   return SFP_OK;
 }
 
@@ -2892,6 +3012,7 @@ extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
   stats[11] = sim->captures;
   stats[12] = sim->tune_batches;
   stats[13] = c.emit_total;
+  stats[14] = sim->clock_restarts;
   return SFP_OK;
 }
 
@@ -2907,5 +3028,6 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   sim->syncs = 0;
   sim->replays = 0;
   sim->graph_batches = sim->captures = sim->tune_batches = 0;
+  sim->clock_restarts = 0;
   return SFP_OK;
 }

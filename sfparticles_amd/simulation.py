@@ -248,12 +248,13 @@ class Simulation(object):
         """Counters of one species since the last reset: particle-steps (live
         particles after each step), particles read by its step kernel,
         particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 14)()
+        v = (C.c_int64 * 15)()
         L.call("sfp_sim_stats", self._sim, int(species), v)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
                     hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
                     slept=int(v[8]), sleeps=int(v[9]), graph_batches=int(v[10]),
-                    captures=int(v[11]), tune_batches=int(v[12]), emit=int(v[13]))
+                    captures=int(v[11]), tune_batches=int(v[12]), emit=int(v[13]),
+                    clock_restarts=int(v[14]))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)

@@ -335,3 +335,59 @@ def test_envelope_bounds_are_upper_bounds(which):
             worst = max(worst, float(np.max(mm / b)))
         print(which, nsteps, "path: max field/bound", worst)
         assert worst > 0.05
+
+
+def test_photon_sleep_carries_across_run_calls():
+    """Run calls that continue each other keep QED photons asleep across the
+    call boundary (no restart of the sleep clock, no visit of every photon
+    at the first step of a call); positions read between calls are those of
+    that step, and the end state still equals the oracle's bit for bit."""
+    import sfparticles_amd as sfa
+    from sfparticles_amd import qed as Q
+    from sfparticles_amd.fields import Fields
+    case = head_on_case(n=3000, nsteps=240, seed=11)
+    Q.set_model("rs")
+
+    def make():
+        n = case["props"][0].shape[0]
+        e = sfa.Particles("electron", -1, 1, N=n, props=[a.copy() for a in case["props"]],
+                          RR=sfa.RadiationReactionType.PHOTON)
+        p = sfa.Particles("positron", 1, 1)
+        g = sfa.Particles("photon", 0, 0)
+        e.set_photon(g)
+        p.set_photon(g)
+        g.set_pair(e, p)
+        sim = sfa.Simulation(e, p, g, dt=case["dt"], fields=Fields(terms=case["field"]),
+                             print_every=None, photon_threshold=case["threshold"],
+                             seed=case["seed"])
+        return sim, (e, p, g)
+
+    # calls of uneven length, none a multiple of the 8-step list window; the
+    # last one through start(), which prunes the converted photons at its end
+    sim, sp = make()
+
+    def photons(g):
+        n = g.N_buffered
+        live = ~np.array(g._to_be_pruned)[:n]  # converted photons are holes until pruned
+        return {a: np.array(getattr(g, a)[:n])[live] for a in ("x", "y", "z", "id")}
+
+    for m in (37, 64, 83):
+        sim.run_steps(m)
+    mid = photons(sp[2])
+    sim.start(56)
+    st = sim.stats(2)
+    print("clock restarts", st["clock_restarts"], "slept", st["slept"], "sleeps", st["sleeps"])
+    assert st["clock_restarts"] == 1 and st["slept"] > 0
+    # positions between calls: a single call stopped at the same step
+    sim1, sp1 = make()
+    sim1.run_steps(184)
+    ref_mid = photons(sp1[2])
+    for a in ("x", "y", "z", "id"):
+        assert np.array_equal(mid[a], ref_mid[a]), a
+    got = {s.name: {"n": int(s.N_buffered),
+                    **{a: np.array(getattr(s, a)[: s.N_buffered])
+                       for a in ("x", "y", "z", "ux", "uy", "uz", "id")}} for s in sp}
+    compare_final(got, run_oracle_case(case))
+    # a change of a store from the host (here the prune) restarts the clock
+    sim.run_steps(8)
+    assert sim.stats(2)["clock_restarts"] == 2
