@@ -609,6 +609,16 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restr
       cnt[2] += ns;
     }
 
+    // leptons: software pipeline over the rows, the state of the next row
+    // loads while this row computes
+    uint8_t npr = 1;
+    double nx = 0, ny = 0, nz = 0, nux = 0, nuy = 0, nuz = 0;
+    if (!PHOTON && base + threadIdx.x < n_old) {
+      const int64_t i0 = base + threadIdx.x;
+      npr = sp.pruned[i0];
+      nx = sp.x[i0]; ny = sp.y[i0]; nz = sp.z[i0];
+      nux = sp.ux[i0]; nuy = sp.uy[i0]; nuz = sp.uz[i0];
+    }
 #pragma unroll 1
     for (int row = 0; row < rows; ++row) {
       const int64_t i = base + (int64_t)row * TILE + threadIdx.x;
@@ -619,7 +629,19 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restr
       }
       bool active = false;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, sx = 0, sy = 0, sz = 0;
-      if (i < n_old && !((asleep >> row) & 1u)) {
+      if (!PHOTON) {
+        if (i < n_old) {
+          x = nx; y = ny; z = nz;
+          ux = nux; uy = nuy; uz = nuz;
+          active = npr == 0;
+        }
+        if (row + 1 < rows && i + TILE < n_old) {
+          const int64_t in = i + TILE;
+          npr = sp.pruned[in];
+          nx = sp.x[in]; ny = sp.y[in]; nz = sp.z[in];
+          nux = sp.ux[in]; nuy = sp.uy[in]; nuz = sp.uz[in];
+        }
+      } else if (i < n_old && !((asleep >> row) & 1u)) {
         // the pruned flag and the state are loaded together
         const uint8_t pr = sp.pruned[i];
         x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
@@ -1195,23 +1217,35 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
     // row pass: the QED decision of every listed lepton; the ones without an
     // event (emission, or with the learned sampler an emission candidate)
     // are pushed and stored, the others are left untouched for the event pass
-    // the list index of the next row is loaded while this row computes
+    // software pipeline over the rows: the state of the next row and the
+    // list index of the one after are loaded while this row computes
     const int64_t k0 = tile * SPAN + threadIdx.x;
-    uint32_t i_next = k0 < n_hot ? sp.hot[k0] : 0u;
+    uint32_t i_cur = k0 < n_hot ? sp.hot[k0] : 0u;
+    uint32_t i_next = (rows > 1 && k0 + TILE < n_hot) ? sp.hot[k0 + TILE] : 0u;
+    double nx = 0, ny = 0, nz = 0, nux = 0, nuy = 0, nuz = 0;
+    uint64_t npid = 0;
+    if (k0 < n_hot) {
+      nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
+      nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
+      npid = sp.id[i_cur];
+    }
 #pragma unroll 1
     for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
       const bool valid = k < n_hot;
-      const int64_t i = valid ? (int64_t)i_next : 0;
-      i_next = (row + 1 < rows && k + TILE < n_hot) ? sp.hot[k + TILE] : 0u;
-      double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
-      uint64_t pid = 0;
+      const int64_t i = valid ? (int64_t)i_cur : 0;
+      double x = nx, y = ny, z = nz, ux = nux, uy = nuy, uz = nuz, tau = 0, sx = 0, sy = 0, sz = 0;
+      uint64_t pid = npid;
+      if (row + 1 < rows && k + TILE < n_hot) {
+        i_cur = i_next;
+        nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
+        nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
+        npid = sp.id[i_cur];
+        i_next = (row + 2 < rows && k + 2 * TILE < n_hot) ? sp.hot[k + 2 * TILE] : 0u;
+      }
       if (valid) {
-        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
         if (KIND == KIND_OD) tau = sp.tau[i];
         if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
-        pid = sp.id[i];
       }
       uint32_t shot = 0;
       const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
