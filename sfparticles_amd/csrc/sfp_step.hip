@@ -505,10 +505,10 @@ __device__ inline void photon_wake_position(const SpeciesDev& sp, const StepArgs
 // while no QED event is possible its push is the same two position
 // increments every step.  Number of steps, from this one on, for which the
 // envelope bound along the photon's own path proves that no event is possible
-// (1 if only this step is proven).  Windows end at the next multiple of 512,
-// 64 or WIN steps of the sleep clock, so that neighbouring photons -- one
-// wave's lanes -- wake on the same steps, and may last past the end of the
-// run.  Photons of a species without QED sleep to the end of the run.
+// (1 if only this step is proven).  Windows end at the next multiple of
+// 32768, 4096, 512, 64 or WIN steps of the sleep clock (the longest proven
+// one), so that neighbouring photons -- one wave's lanes -- wake on the same
+// steps, and may last past the end of the run.  Photons of a species without QED sleep to the end of the run.
 template <int FM>
 __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, const StepArgs& a,
                                        bool qed, double x,
@@ -520,8 +520,8 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
   }
   // QED photons, on the sleep clock: a sleep may last past the end of the run
   const double vx = SFP_C * ig * ux, vy = SFP_C * ig * uy, vz = SFP_C * ig * uz;
-  const uint32_t q[3] = {512u, 64u, WIN};
-  for (int k = 0; k < 3; ++k) {
+  const uint32_t q[5] = {32768u, 4096u, 512u, 64u, WIN};
+  for (int k = 0; k < 5; ++k) {
     const uint32_t end = (a.wrel / q[k] + 1u) * q[k];
     if (end < a.wrel + 2u) continue;
     const uint32_t d = end - a.wrel;
