@@ -1673,9 +1673,10 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
 //              and is never visited again.
 static constexpr uint32_t WAKE_NEVER = 0xffffffffu;
 
-// Wake steps per thread in k_awake: four 16-byte loads.  The wake array is
+// Wake steps per thread in k_awake: eight 16-byte loads (a tile of 8192
+// words per look-back).  The wake array is
 // allocated in whole tiles (WAKE_PAD words), so full-tile loads stay inside it.
-static constexpr int AW = 16;
+static constexpr int AW = 32;
 static constexpr int64_t WAKE_PAD = (int64_t)TILE * AW;
 
 __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
