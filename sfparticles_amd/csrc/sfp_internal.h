@@ -99,7 +99,21 @@ struct Store {
   uint64_t version = 0;
   void* p[SFP_NATTR] = {};
   Counts* cnt = nullptr;  // device
+  // A simulation whose sleeping photons hold the positions of the steps they
+  // fell asleep at: settle(sim) brings them to the last step run.  Every host
+  // access to the store calls it first (store_settle).
+  int (*settle)(void* sim) = nullptr;
+  void* settle_sim = nullptr;
 };
+
+static inline int store_settle(Store* s) {
+  if (!s->settle) return 0;
+  int (*f)(void*) = s->settle;
+  void* sim = s->settle_sim;
+  s->settle = nullptr;
+  s->settle_sim = nullptr;
+  return f(sim);
+}
 
 int sfp_compile_fields(const sfp_field* terms, int32_t nterms, sfp_fieldset* fs);
 int sfp_compact_indices(const uint8_t* f, int64_t n, int invert, int64_t* out, int64_t* count_host,

@@ -789,6 +789,7 @@ extern "C" int sfp_store_create(int32_t flags, int64_t capacity, void** out) {
 extern "C" int sfp_store_destroy(void* store) {
   Store* s = (Store*)store;
   if (!s) return SFP_OK;
+  (void)store_settle(s);
   (void)hipDeviceSynchronize();
   for (int a = 0; a < SFP_NATTR; ++a)
     if (s->p[a]) (void)hipFree(s->p[a]);
@@ -801,6 +802,7 @@ extern "C" int sfp_store_reserve(void* store, int64_t capacity) {
   Store* s = (Store*)store;
   if (!s) return set_err(SFP_E_ARG, "null store");
   if (capacity <= s->cap) return SFP_OK;
+  CHK(store_settle(s));
   s->version++;
   HIPCHK(hipDeviceSynchronize());
   for (int a = 0; a < SFP_NATTR; ++a) {
@@ -830,6 +832,7 @@ extern "C" int sfp_store_ptr(void* store, int32_t attr, void** dev) {
   Store* s = (Store*)store;
   if (!s || attr < 0 || attr >= SFP_NATTR) return set_err(SFP_E_ARG, "bad store/attribute");
   CHK(store_alloc_attr(s, attr));
+  CHK(store_settle(s));
   s->version++;
   *dev = s->p[attr];
   return SFP_OK;
@@ -843,7 +846,10 @@ extern "C" int sfp_store_set_count(void* store, int64_t n, uint64_t next_id) {
   Counts c{};
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(&c, s->cnt, sizeof(c), hipMemcpyDeviceToHost));
-  if (c.n != n || c.next_id != (int64_t)next_id) s->version++;
+  if (c.n != n || c.next_id != (int64_t)next_id) {
+    CHK(store_settle(s));
+    s->version++;
+  }
   c.n = n;
   c.next_id = (int64_t)next_id;
   c.n_old = n;
@@ -870,6 +876,7 @@ extern "C" int sfp_store_upload(void* store, int32_t attr, const void* host, int
   if (off < 0 || n < 0 || off + n > s->cap) return set_err(SFP_E_ARG, "upload out of range");
   CHK(store_alloc_attr(s, attr));
   if (n == 0) return SFP_OK;
+  CHK(store_settle(s));
   s->version++;
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy((char*)s->p[attr] + kAttrSize[attr] * off, host, kAttrSize[attr] * n,
@@ -884,6 +891,7 @@ extern "C" int sfp_store_download(void* store, int32_t attr, void* host, int64_t
   if (off < 0 || n < 0 || off + n > s->cap) return set_err(SFP_E_ARG, "download out of range");
   CHK(store_alloc_attr(s, attr));
   if (n == 0) return SFP_OK;
+  CHK(store_settle(s));
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(host, (char*)s->p[attr] + kAttrSize[attr] * off, kAttrSize[attr] * n,
                    hipMemcpyDeviceToHost));
@@ -898,6 +906,7 @@ __global__ void k_gather(const T* src, T* dst, const int64_t* idx, int64_t n) {
 extern "C" int sfp_store_prune(void* store, int64_t* n_kept) {
   Store* s = (Store*)store;
   if (!s) return set_err(SFP_E_ARG, "null store");
+  CHK(store_settle(s));
   s->version++;
   int64_t n = 0;
   uint64_t next_id = 0;

@@ -2179,6 +2179,10 @@ struct Sim {
   uint64_t wake_origin = 0, wake_end = 0;
   bool wake_valid = false;
   int64_t clock_restarts = 0;
+  // sleeping photons hold the positions of their sleep start until settled
+  // (by the next run that continues this one, or before any host access)
+  bool lag = false;
+  uint32_t lag_wend = 0;
   std::vector<uint64_t> wake_version;
   std::vector<int64_t> wake_cap;
   std::vector<char*> cache_mem;     // photon species: window cache of the visit list
@@ -2584,9 +2588,12 @@ extern "C" int sfp_sim_set_shots(void* p, const sfp_field* terms, const int32_t*
   return SFP_OK;
 }
 
+static int sim_settle(Sim* sim);
+
 extern "C" int sfp_sim_destroy(void* p) {
   Sim* sim = (Sim*)p;
   if (!sim) return SFP_OK;
+  (void)sim_settle(sim);  // the stores outlive the simulation: positions current
   (void)hipDeviceSynchronize();
   for (int ph = 0; ph < NPH; ++ph)
     for (auto f : sim->flags[ph])
@@ -2811,6 +2818,29 @@ static int sim_tune_pick(Sim* sim) {
   return SFP_OK;
 }
 
+// QED photons asleep past the last step run get that step's position
+// (k_catchup); their sleep goes on from it
+static int sim_settle(Sim* sim) {
+  if (!sim->lag) return SFP_OK;
+  sim->lag = false;
+  const double cdt_half = SFP_C * (0.5 * sim->params.dt);
+  for (int s = 0; s < sim->nsp; ++s) {
+    if (!sim->wake_mem[s] || !sim->qed[s] || !sim->plan.sp[s].push) continue;
+    hipLaunchKernelGGL(k_catchup, dim3(sim->cus * 8), dim3(256), 0, sim->stream,
+                       (const Plan*)sim->plan_dev, s, sim->lag_wend, cdt_half);
+    HIPCHK(hipGetLastError());
+    Store* st = sim->stores[s];
+    if (st->settle_sim == sim) {
+      st->settle = nullptr;
+      st->settle_sim = nullptr;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(sim->stream));
+  return SFP_OK;
+}
+
+static int sim_settle_cb(void* p) { return sim_settle((Sim*)p); }
+
 extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {
   Sim* sim = (Sim*)p;
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
@@ -2829,6 +2859,7 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   for (int s = 0; s < sim->nsp; ++s)
     if (sim->stores[s]->version != sim->wake_version[s]) keep = false;
   sim->wake_valid = false;
+  if (!keep) CHK(sim_settle(sim));  // the restarted clock needs current positions
   if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
     return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
   // window cache of the photon visit lists: a quarter of the store (longer
@@ -2918,16 +2949,18 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   }
   *t = tt;
   sim->rec_end = std::max<uint64_t>(sim->rec_end, step0 + (uint64_t)nsteps);
-  // sleeping QED photons: positions of the run's last step
+  // sleeping QED photons lag behind the run's last step until settled: the
+  // stores call back before the host touches them
   if (nsteps > 0) {
-    const uint32_t wend = (uint32_t)(step0 + (uint64_t)nsteps - sim->wake_origin);
+    sim->lag_wend = (uint32_t)(step0 + (uint64_t)nsteps - sim->wake_origin);
     for (int s = 0; s < sim->nsp; ++s) {
       if (!sim->wake_mem[s] || !sim->qed[s] || !sim->plan.sp[s].push) continue;
-      hipLaunchKernelGGL(k_catchup, dim3(sim->cus * 8), dim3(256), 0, sim->stream,
-                         (const Plan*)sim->plan_dev, s, wend, SFP_C * (0.5 * dt));
-      HIPCHK(hipGetLastError());
+      Store* st = sim->stores[s];
+      if (st->settle && st->settle_sim != sim) CHK(store_settle(st));
+      st->settle = sim_settle_cb;
+      st->settle_sim = sim;
+      sim->lag = true;
     }
-    HIPCHK(hipStreamSynchronize(sim->stream));
   }
   sim->wake_end = step0 + (uint64_t)nsteps;
   for (int s = 0; s < sim->nsp; ++s) sim->wake_version[s] = sim->stores[s]->version;

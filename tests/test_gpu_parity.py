@@ -378,9 +378,11 @@ def test_photon_sleep_carries_across_run_calls():
     st = sim.stats(2)
     print("clock restarts", st["clock_restarts"], "slept", st["slept"], "sleeps", st["sleeps"])
     assert st["clock_restarts"] == 1 and st["slept"] > 0
-    # positions between calls: a single call stopped at the same step
+    # positions between calls: a single call stopped at the same step, read
+    # after its simulation is gone (which brings sleeping photons up to date)
     sim1, sp1 = make()
     sim1.run_steps(184)
+    del sim1
     ref_mid = photons(sp1[2])
     for a in ("x", "y", "z", "id"):
         assert np.array_equal(mid[a], ref_mid[a]), a
