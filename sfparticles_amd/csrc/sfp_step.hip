@@ -232,7 +232,7 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
 // look-back (a round trip through the device-coherent flags, several
 // microseconds) is paid once per tile, not once per row.  Short lists take
 // fewer rows per tile, so that every workgroup of the grid has a tile.
-static constexpr int QROWS = 8;
+static constexpr int QROWS = 16;
 
 __device__ inline int qed_rows(int64_t n_list) {
   const int64_t per_round = (int64_t)gridDim.x * TILE;
@@ -2860,6 +2860,21 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
     if (sim->stores[s]->version != sim->wake_version[s]) keep = false;
   sim->wake_valid = false;
   if (!keep) CHK(sim_settle(sim));  // the restarted clock needs current positions
+  // From here the run tracks every photon itself: this simulation's store
+  // callbacks (installed at the end of the last run) come off, so that a
+  // store grown during the run does not apply the old run's catch-up to
+  // photons that have moved since; another simulation's pending catch-up on a
+  // shared store runs now.
+  for (int s = 0; s < sim->nsp; ++s) {
+    Store* st = sim->stores[s];
+    if (st->settle_sim == sim) {
+      st->settle = nullptr;
+      st->settle_sim = nullptr;
+    } else if (st->settle) {
+      CHK(store_settle(st));
+    }
+  }
+  sim->lag = false;
   if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
     return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
   // window cache of the photon visit lists: a quarter of the store (longer

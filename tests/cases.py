@@ -71,7 +71,7 @@ def _species_list(case, make):
 
 
 def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None,
-                 grids=None, **sim_kw):
+                 grids=None, reserve=None, **sim_kw):
     """Run a case through sfparticles_amd.Simulation; returns per-species arrays."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
@@ -104,6 +104,12 @@ def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sa
                          sampler=sampler, **sim_kw)
     for (si, ph), grid in (grids or {}).items():
         sim.set_grid(si, ph, grid)
+    # capacities reserved up front (stores otherwise start at the seed count
+    # and grow through capacity halts)
+    for s, cap in zip(sp, reserve or ()):
+        s._ensure_device()
+        if cap > s.buffer_size:
+            s._extend(cap - s.N_buffered)
     steps = case["nsteps"] if nsteps is None else nsteps
     per = steps // chunks
     done = 0

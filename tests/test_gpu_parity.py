@@ -88,6 +88,20 @@ def test_head_on_rs_chunked_equals_single():
         assert np.array_equal(g1[name]["x"], g4[name]["x"])
 
 
+@pytest.mark.parametrize("qed", ["rs", "od"])
+def test_head_on_window_cache_bit_exact(qed):
+    """Stores reserved up front, as in the benchmark: the photon visit list
+    fits the window cache (state kept by list position between rebuilds and
+    written back at each window's last step) -- bit for bit the oracle."""
+    case = head_on_case(n=3000, nsteps=240, seed=13, qed=qed)
+    gpu, sim = run_gpu_case(case, reserve=(60000, 60000, 400000))
+    ref = run_oracle_case(case)
+    st = sim.stats(2)
+    print("counts", _names(gpu), "replays", st["replays"], "photon capacity", st["capacity"])
+    assert st["capacity"] == 400000 and ref["photon"]["n"] > 1000
+    compare_final(gpu, ref)
+
+
 def test_head_on_od_bit_exact():
     case = head_on_case(n=3000, nsteps=240, seed=12, qed="od")
     gpu, _ = run_gpu_case(case)
@@ -337,16 +351,18 @@ def test_envelope_bounds_are_upper_bounds(which):
         assert worst > 0.05
 
 
-def test_photon_sleep_carries_across_run_calls():
+@pytest.mark.parametrize("qed,graph", [("rs", False), ("od", False), ("rs", True)])
+def test_photon_sleep_carries_across_run_calls(qed, graph):
     """Run calls that continue each other keep QED photons asleep across the
     call boundary (no restart of the sleep clock, no visit of every photon
     at the first step of a call); positions read between calls are those of
-    that step, and the end state still equals the oracle's bit for bit."""
+    that step, and the end state still equals the oracle's bit for bit (the
+    optical depth of sleeping photons included; also with graph replays)."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
     from sfparticles_amd.fields import Fields
-    case = head_on_case(n=3000, nsteps=240, seed=11)
-    Q.set_model("rs")
+    case = head_on_case(n=3000, nsteps=240, seed=11, qed=qed)
+    Q.set_model(qed)
 
     def make():
         n = case["props"][0].shape[0]
@@ -359,7 +375,7 @@ def test_photon_sleep_carries_across_run_calls():
         g.set_pair(e, p)
         sim = sfa.Simulation(e, p, g, dt=case["dt"], fields=Fields(terms=case["field"]),
                              print_every=None, photon_threshold=case["threshold"],
-                             seed=case["seed"])
+                             seed=case["seed"], graph=graph)
         return sim, (e, p, g)
 
     # calls of uneven length, none a multiple of the 8-step list window; the
@@ -371,7 +387,7 @@ def test_photon_sleep_carries_across_run_calls():
         live = ~np.array(g._to_be_pruned)[:n]  # converted photons are holes until pruned
         return {a: np.array(getattr(g, a)[:n])[live] for a in ("x", "y", "z", "id")}
 
-    for m in (37, 64, 83):
+    for m in (37, 64, 83) if not graph else (37, 128, 19):  # graph replays need 64-step batches
         sim.run_steps(m)
     mid = photons(sp[2])
     sim.start(56)
@@ -386,9 +402,14 @@ def test_photon_sleep_carries_across_run_calls():
     ref_mid = photons(sp1[2])
     for a in ("x", "y", "z", "id"):
         assert np.array_equal(mid[a], ref_mid[a]), a
-    got = {s.name: {"n": int(s.N_buffered),
-                    **{a: np.array(getattr(s, a)[: s.N_buffered])
-                       for a in ("x", "y", "z", "ux", "uy", "uz", "id")}} for s in sp}
+    got = {}
+    for s_ in sp:
+        d = {"n": int(s_.N_buffered)}
+        for a in ("x", "y", "z", "ux", "uy", "uz", "id") + (("tau",) if "tau" in s_.attrs else ()):
+            d[a] = np.array(getattr(s_, a)[: s_.N_buffered])
+        got[s_.name] = d
+    if graph:
+        assert sim.stats(0)["graph_batches"] >= 2
     compare_final(got, run_oracle_case(case))
     # a change of a store from the host (here the prune) restarts the clock
     sim.run_steps(8)
