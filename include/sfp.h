@@ -320,7 +320,14 @@ int sfp_sim_destroy(void* sim);
  * simulation.py:143) and step index step0 (the Philox counter).  Counts stay
  * on the device; the host synchronises every 64 steps.  A step whose appends
  * would overflow a store halts on the device before changing anything; the
- * store grows and the step is replayed from the halted kernel. */
+ * store grows and the step is replayed from the halted kernel.
+ * QED photons proven cold sleep, with the position of the step they fell
+ * asleep at, and the sleep goes on into the next call when it starts at the
+ * step this one ended and no store was changed from the host in between
+ * (otherwise every photon is visited at its first step).  Every sfp_store_*
+ * call on a photon store (and sfp_sim_destroy) first brings sleeping photons
+ * to the last step run, so what the store API returns is current; a device
+ * pointer taken before the run may show older positions for them. */
 #define SFP_RUN_TIMING 1 /* record HIP events around each species kernel */
 /* Run each full batch of 64 steps as one replay of a captured HIP graph
  * (captured at the first such batch, again after the kernels or grids in use
