@@ -324,10 +324,11 @@ int sfp_sim_destroy(void* sim);
  * QED photons proven cold sleep, with the position of the step they fell
  * asleep at, and the sleep goes on into the next call when it starts at the
  * step this one ended and no store was changed from the host in between
- * (otherwise every photon is visited at its first step).  Every sfp_store_*
- * call on a photon store (and sfp_sim_destroy) first brings sleeping photons
- * to the last step run, so what the store API returns is current; a device
- * pointer taken before the run may show older positions for them. */
+ * (otherwise every photon is visited at its first step).  The sfp_store_*
+ * calls that read or change particle data (download, upload, ptr, prune,
+ * reserve, a new count) and sfp_sim_destroy first bring sleeping photons to
+ * the last step run, so what they return is current; a device pointer taken
+ * before the run may show older positions for them. */
 #define SFP_RUN_TIMING 1 /* record HIP events around each species kernel */
 /* Run each full batch of 64 steps as one replay of a captured HIP graph
  * (captured at the first such batch, again after the kernels or grids in use
