@@ -1236,16 +1236,18 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restric
       const int64_t i = valid ? (int64_t)i_cur : 0;
       double x = nx, y = ny, z = nz, ux = nux, uy = nuy, uz = nuz, tau = 0, sx = 0, sy = 0, sz = 0;
       uint64_t pid = npid;
+      // this row's own loads go first: loads complete in order, so one issued
+      // after the next row's would wait for those
+      if (valid) {
+        if (KIND == KIND_OD) tau = sp.tau[i];
+        if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
+      }
       if (row + 1 < rows && k + TILE < n_hot) {
         i_cur = i_next;
         nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
         nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
         npid = sp.id[i_cur];
         i_next = (row + 2 < rows && k + 2 * TILE < n_hot) ? sp.hot[k + 2 * TILE] : 0u;
-      }
-      if (valid) {
-        if (KIND == KIND_OD) tau = sp.tau[i];
-        if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       }
       uint32_t shot = 0;
       const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
