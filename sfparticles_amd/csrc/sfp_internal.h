@@ -85,7 +85,7 @@ struct Counts {
   int64_t asleep_n;   // photons: live photons inside a multi-step push
   int64_t n_emit;     // emission candidates handed to the emission kernel this step
   int64_t emit_total; // sum of n_emit over steps
-  int64_t pad[1];
+  int64_t cpar;       // photons: which of the two window caches holds this window
 };
 
 extern const size_t kAttrSize[SFP_NATTR];

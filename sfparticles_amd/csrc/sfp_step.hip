@@ -67,16 +67,35 @@ struct SpeciesDev {
   uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
   EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
   // photons: state of the first `ccap` entries of the visit list, by list
-  // position, for the 8-step window the list is kept (see k_photon)
-  double *cx, *cy, *cz, *cux, *cuy, *cuz;
-  uint32_t* cwake;
+  // position, for the window the list is kept; two sets (win_cache), this
+  // window's and the last one's (see k_photon)
+  double* cache;
   int64_t ccap;
+  uint32_t* slot;  // photons: position in the visit list it was last put on
+  uint32_t* oldk;  // photons, per list entry at a rebuild: position in the last window's cache, or none
   Counts* cnt;
   int64_t cap;
   double q, ae;
   int32_t rr, push, spin, photon, tag, qed;
   int32_t photon_dst, pair_e_dst, pair_p_dst, pad;
 };
+
+// One set of the photon window cache: x, y, z, ux, uy, uz and the wake step of
+// the first n list entries, as 7 rows of n words (set `par` of the two)
+struct WinCache {
+  double* b;
+  int64_t n;
+  __device__ double& x(int64_t k) const { return b[k]; }
+  __device__ double& y(int64_t k) const { return b[n + k]; }
+  __device__ double& z(int64_t k) const { return b[2 * n + k]; }
+  __device__ double& ux(int64_t k) const { return b[3 * n + k]; }
+  __device__ double& uy(int64_t k) const { return b[4 * n + k]; }
+  __device__ double& uz(int64_t k) const { return b[5 * n + k]; }
+  __device__ uint32_t& wake(int64_t k) const { return *(uint32_t*)(b + 6 * n + k); }
+};
+__device__ inline WinCache win_cache(const SpeciesDev& sp, uint32_t par) {
+  return WinCache{sp.cache + (int64_t)(par & 1u) * 7 * sp.ccap, sp.ccap};
+}
 
 struct ScanState {
   uint32_t tile_ctr;
@@ -1698,14 +1717,16 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     const int64_t from = sp.cnt->list_end, n_list = sp.cnt->n_hot;
     const uint32_t nwg = participants((n_old - from + TILE - 1) / TILE);
     if (blockIdx.x >= nwg) return;
+    const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
     for (int64_t j = from + (int64_t)blockIdx.x * TILE + threadIdx.x; j < n_old;
          j += (int64_t)nwg * TILE) {
       const int64_t k = n_list + (j - from);
       sp.hot[k] = (uint32_t)j;
+      sp.slot[j] = (uint32_t)k;
       if (k < sp.ccap) {  // into the window cache (the new photons are contiguous)
-        sp.cx[k] = sp.x[j]; sp.cy[k] = sp.y[j]; sp.cz[k] = sp.z[j];
-        sp.cux[k] = sp.ux[j]; sp.cuy[k] = sp.uy[j]; sp.cuz[k] = sp.uz[j];
-        sp.cwake[k] = sp.pruned[j] ? WAKE_NEVER : sp.wake[j];
+        C.x(k) = sp.x[j]; C.y(k) = sp.y[j]; C.z(k) = sp.z[j];
+        C.ux(k) = sp.ux[j]; C.uy(k) = sp.uy[j]; C.uz(k) = sp.uz[j];
+        C.wake(k) = sp.pruned[j] ? WAKE_NEVER : sp.wake[j];
       }
     }
     kernel_exit(scan, nwg, [&]() {
@@ -1721,6 +1742,11 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   const uint32_t nwg = participants(ntiles);
   if (blockIdx.x >= nwg) return;
   const uint32_t epoch = next_epoch(scan);
+  const int64_t prev_end = sp.cnt->list_end;  // the last window's list covered [0, prev_end)
+  // the last window's cache set and this one's (k_photon sees the new one
+  // after this launch flips the parity)
+  const uint32_t par0 = (uint32_t)sp.cnt->cpar;
+  const WinCache Co = win_cache(sp, par0), Cn = win_cache(sp, par0 ^ 1u);
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
   uint32_t cnt[2] = {0, 0};  // asleep (live), listed
@@ -1740,11 +1766,12 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
       const uint4 v = src[q];
       w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
     }
-    uint32_t up = 0;
+    uint32_t up = 0, cont = 0;  // listed; listed and awake through the last window
 #pragma unroll
     for (int j = 0; j < AW; ++j) {
       const bool in = i0 + j < n_old;
       up |= (uint32_t)(in && w[j] <= a.wrel) << j;
+      cont |= (uint32_t)(in && w[j] < a.wrel) << j;
       cnt[0] += (uint32_t)(in && w[j] > a.wrel && w[j] != WAKE_NEVER);
     }
     const uint32_t c = (uint32_t)__popc(up);
@@ -1769,9 +1796,38 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     }
     __syncthreads();
     uint64_t r = sh_pref + woff + incl - c;
-#pragma unroll
-    for (int j = 0; j < AW; ++j)
-      if ((up >> j) & 1u) sp.hot[r++] = (uint32_t)(i0 + j);
+    for (uint32_t m = up; m; m &= m - 1u) {
+      const int j = __builtin_ctz(m);
+      const int64_t i = i0 + j;
+      // 0: awake through the last window (on its list, not just woken)
+      sp.oldk[r] = (a.rel != 0 && ((cont >> j) & 1u) && i < prev_end) ? 0u : WAKE_NEVER;
+      sp.hot[r++] = (uint32_t)i;
+    }
+    // The tile's entries, one per thread: a photon awake through the last
+    // window has its state in that window's cache, at the position it had
+    // there; it moves to this window's cache at its new position (both lists
+    // are in index order, so the reads are nearly sequential), or to the
+    // store if the new position lies past the cache.
+    __threadfence_block();
+    __syncthreads();
+    for (int64_t q = (int64_t)sh_pref + threadIdx.x; q < (int64_t)sh_pref + agg; q += TILE) {
+      const int64_t i = (int64_t)sp.hot[q];
+      uint32_t ok = WAKE_NEVER;
+      if (sp.oldk[q] == 0u) {
+        const uint32_t sl = sp.slot[i];
+        if ((int64_t)sl < sp.ccap) {
+          if (q < sp.ccap) {
+            Cn.x(q) = Co.x(sl); Cn.y(q) = Co.y(sl); Cn.z(q) = Co.z(sl);
+            Cn.ux(q) = Co.ux(sl); Cn.uy(q) = Co.uy(sl); Cn.uz(q) = Co.uz(sl);
+            ok = sl;
+          } else {
+            sp.x[i] = Co.x(sl); sp.y[i] = Co.y(sl); sp.z[i] = Co.z(sl);
+          }
+        }
+      }
+      sp.oldk[q] = ok;
+      sp.slot[i] = (uint32_t)q;
+    }
     __syncthreads();  // sh_w and sh_pref are rewritten by the next tile
   }
 
@@ -1786,6 +1842,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
     uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
     sp.cnt->n_hot = (int64_t)total;
     sp.cnt->list_end = n_old;
+    sp.cnt->cpar ^= 1;  // the new window's cache is the other set
     scan->epoch = epoch;
   });
 }
@@ -1825,12 +1882,15 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
   const int wv = threadIdx.x / 64;
   // Window cache: the list is rebuilt every 8th step and kept in between,
   // so the state of its first ccap entries lives in arrays indexed by list
-  // position for the window (coalesced), gathered from the store at the
-  // rebuild (or by the list kernel for appended photons) and written back
-  // at the window's last step; photons falling asleep or converting go
-  // straight to the store.
+  // position for the window (coalesced).  A rebuild takes a photon that was
+  // awake through the last window from that window's cache (the other set,
+  // at the position the list kernel found, in the same order: nearly
+  // sequential) and gathers the others from the store; appended photons are
+  // copied in by the list kernel.  The store gets a photon's position when it
+  // falls asleep and, for every cached photon, at the end of the run.
   const bool rebuild = list_rebuild(a);
-  const bool last = (a.wrel + 1) % WIN == 0 || a.rel + 1 == a.rel_end;
+  const bool last = a.rel + 1 == a.rel_end;
+  const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
   bool staged = false;
@@ -1857,14 +1917,19 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
       const bool cached = k < sp.ccap;
       bool woke = false;  // asleep until this step
+      bool stale = false;  // the store holds an older position
       if (k < n_list) {
         // a list kept from an earlier step holds photons that have fallen
         // asleep since (counted by the list kernel) and holes: skipped
-        if (cached && !rebuild) {
-          if (sp.cwake[k] <= a.wrel) {
-            x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
-            ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
+        // at a rebuild, a photon awake through the last window was moved
+        // into this window's cache by the list kernel (oldk set)
+        const bool moved = rebuild && cached && sp.oldk[k] != WAKE_NEVER;
+        if (cached && (!rebuild || moved)) {
+          if (moved || C.wake(k) <= a.wrel) {
+            x = C.x(k); y = C.y(k); z = C.z(k);
+            ux = C.ux(k); uy = C.uy(k); uz = C.uz(k);
             live = true;
+            stale = moved;
           }
         } else {
           const uint32_t wk = sp.wake[i];
@@ -1876,10 +1941,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
             woke = live && rebuild && wk == a.wrel;
             if (!live) {
               sp.wake[i] = WAKE_NEVER;  // a hole: never visited again
-              if (cached) sp.cwake[k] = WAKE_NEVER;
+              if (cached) C.wake(k) = WAKE_NEVER;
             }
           } else if (cached) {
-            sp.cwake[k] = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
+            C.wake(k) = WAKE_NEVER;  // asleep at the rebuild: not visited in this window
           }
         }
       }
@@ -1932,15 +1997,16 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
             z += hz; z += hz;
             if (!cached || last) {
               sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-            } else {
-              sp.cx[k] = x; sp.cy[k] = y; sp.cz[k] = z;
+            }
+            if (cached) {
+              C.x(k) = x; C.y(k) = y; C.z(k) = z;
               if (rebuild) {
-                sp.cux[k] = ux; sp.cuy[k] = uy; sp.cuz[k] = uz;
-                sp.cwake[k] = 0u;
+                C.ux(k) = ux; C.uy(k) = uy; C.uz(k) = uz;
+                C.wake(k) = 0u;
               }
             }
-          } else if (woke) {
-            // falls asleep again: the store keeps this step's position (the
+          } else if (woke || stale) {
+            // falls asleep: the store keeps this step's position (the
             // increments are added when it wakes, or at the end of the run)
             sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
           }
@@ -1949,7 +2015,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
         if (nst > 1) {
           sp.wake[i] = a.wrel + nst;
           sp.since[i] = a.wrel;
-          if (cached) sp.cwake[k] = a.wrel + nst;
+          if (cached) C.wake(k) = a.wrel + nst;
           cnt[3] += 1;
         }
         cnt[0] += 1;
@@ -1968,9 +2034,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
       const bool cached = k < sp.ccap;
       double x, y, z, ux, uy, uz;
       bool woke = false;
-      if (cached && !rebuild) {
-        x = sp.cx[k]; y = sp.cy[k]; z = sp.cz[k];
-        ux = sp.cux[k]; uy = sp.cuy[k]; uz = sp.cuz[k];
+      const bool moved = rebuild && cached && sp.oldk[k] != WAKE_NEVER;
+      if (cached && (!rebuild || moved)) {
+        x = C.x(k); y = C.y(k); z = C.z(k);
+        ux = C.ux(k); uy = C.uy(k); uz = C.uz(k);
       } else {
         x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
         ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
@@ -1999,7 +2066,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
                               dp * ux, dp * uy, dp * uz);
       sp.pruned[i] = 1;
       sp.wake[i] = WAKE_NEVER;
-      if (cached) sp.cwake[k] = WAKE_NEVER;
+      if (cached) C.wake(k) = WAKE_NEVER;
     }
   }
 
@@ -2259,12 +2326,15 @@ static int sim_refresh_plan(Sim* sim) {
     }
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
       // photons keep their visit list from step to step: grown inside a run,
-      // the list keeps its entries
+      // the list keeps its entries; the photons' old cache positions (oldk)
+      // follow the list in the same block
       int64_t want = std::max<int64_t>(st->cap, 1);
       uint32_t* h = nullptr;
-      HIPCHK(hipMalloc(&h, sizeof(uint32_t) * want));
+      HIPCHK(hipMalloc(&h, sizeof(uint32_t) * 2 * want));
       if (sim->hot_mem[s]) {
-        HIPCHK(hipMemcpy(h, sim->hot_mem[s], sizeof(uint32_t) * sim->hot_cap[s],
+        const int64_t hc = sim->hot_cap[s];
+        HIPCHK(hipMemcpy(h, sim->hot_mem[s], sizeof(uint32_t) * hc, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h + want, sim->hot_mem[s] + hc, sizeof(uint32_t) * hc,
                          hipMemcpyDeviceToDevice));
         HIPCHK(hipFree(sim->hot_mem[s]));
       }
@@ -2272,6 +2342,7 @@ static int sim_refresh_plan(Sim* sim) {
       sim->hot_cap[s] = want;
     }
     d.hot = sim->hot_mem[s];
+    d.oldk = sim->hot_mem[s] ? sim->hot_mem[s] + sim->hot_cap[s] : nullptr;
     if (sim->emit[s] && st->cap > sim->emit_cap[s]) {
       // one step's emission candidates: written by k_hot, read by k_emit
       if (sim->emit_mem[s]) HIPCHK(hipFree(sim->emit_mem[s]));
@@ -2293,11 +2364,14 @@ static int sim_refresh_plan(Sim* sim) {
                          hipMemcpyDeviceToDevice));
         HIPCHK(hipFree(sim->wake_mem[s]));
       }
+      // since and slot share one block: [since | slot]
       uint32_t* sn = nullptr;
-      HIPCHK(hipMalloc(&sn, sizeof(uint32_t) * want));
-      HIPCHK(hipMemset(sn, 0, sizeof(uint32_t) * want));
+      HIPCHK(hipMalloc(&sn, sizeof(uint32_t) * 2 * want));
+      HIPCHK(hipMemset(sn, 0, sizeof(uint32_t) * 2 * want));
       if (sim->since_mem[s]) {
-        HIPCHK(hipMemcpy(sn, sim->since_mem[s], sizeof(uint32_t) * sim->wake_cap[s],
+        const int64_t wc = sim->wake_cap[s];
+        HIPCHK(hipMemcpy(sn, sim->since_mem[s], sizeof(uint32_t) * wc, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(sn + want, sim->since_mem[s] + wc, sizeof(uint32_t) * wc,
                          hipMemcpyDeviceToDevice));
         HIPCHK(hipFree(sim->since_mem[s]));
       }
@@ -2307,13 +2381,11 @@ static int sim_refresh_plan(Sim* sim) {
     }
     d.wake = sim->wake_mem[s];
     d.since = sim->since_mem[s];
+    d.slot = sim->since_mem[s] ? sim->since_mem[s] + sim->wake_cap[s] : nullptr;
     {
       const int64_t cc = sim->cache_cap[s];
       double* c = (double*)sim->cache_mem[s];
-      d.cx = c; d.cy = c ? c + cc : nullptr; d.cz = c ? c + 2 * cc : nullptr;
-      d.cux = c ? c + 3 * cc : nullptr; d.cuy = c ? c + 4 * cc : nullptr;
-      d.cuz = c ? c + 5 * cc : nullptr;
-      d.cwake = c ? (uint32_t*)(c + 6 * cc) : nullptr;
+      d.cache = c;
       d.ccap = c ? cc : 0;
     }
   }
@@ -2890,7 +2962,8 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       sim->cache_mem[s] = nullptr;
       sim->cache_cap[s] = 0;
       const int64_t cc = want + want / 2;
-      HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * (6 * sizeof(double) + sizeof(uint32_t))));
+      // two sets (this window's and the last one's) of 7 words per entry
+      HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * 2 * 7 * sizeof(double)));
       sim->cache_cap[s] = cc;
       dirty = true;
     }
