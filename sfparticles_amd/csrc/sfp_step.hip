@@ -1708,6 +1708,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[2 * WAVES];
   __shared__ uint64_t sh_pref;
+  __shared__ uint32_t sh_list[AW * TILE];
+  __shared__ uint8_t sh_cont[AW * TILE];
   const int64_t n_old = sp.cnt->n_old;
   ScanState* scan = P.scan[s][0];
   if (!list_rebuild(a)) {
@@ -1795,25 +1797,27 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
       if (lane == 0) sh_pref = pre;
     }
     __syncthreads();
-    uint64_t r = sh_pref + woff + incl - c;
+    // the tile's listed photons go to LDS in list order (with a flag: awake
+    // through the last window, i.e. on its list and not just woken) ...
+    uint32_t lq = woff + incl - c;
     for (uint32_t m = up; m; m &= m - 1u) {
       const int j = __builtin_ctz(m);
       const int64_t i = i0 + j;
-      // 0: awake through the last window (on its list, not just woken)
-      sp.oldk[r] = (a.rel != 0 && ((cont >> j) & 1u) && i < prev_end) ? 0u : WAKE_NEVER;
-      sp.hot[r++] = (uint32_t)i;
+      sh_list[lq] = (uint32_t)i;
+      sh_cont[lq++] = (uint8_t)(a.rel != 0 && ((cont >> j) & 1u) && i < prev_end);
     }
-    // The tile's entries, one per thread: a photon awake through the last
-    // window has its state in that window's cache, at the position it had
-    // there; it moves to this window's cache at its new position (both lists
-    // are in index order, so the reads are nearly sequential), or to the
-    // store if the new position lies past the cache.
-    __threadfence_block();
     __syncthreads();
-    for (int64_t q = (int64_t)sh_pref + threadIdx.x; q < (int64_t)sh_pref + agg; q += TILE) {
-      const int64_t i = (int64_t)sp.hot[q];
+    // ... and are written out one per thread (coalesced).  A photon awake
+    // through the last window has its state in that window's cache, at the
+    // position it had there; it moves to this window's cache at its new
+    // position (both lists are in index order, so the reads are nearly
+    // sequential), or to the store if the new position lies past the cache.
+    for (uint32_t e = threadIdx.x; e < agg; e += TILE) {
+      const int64_t q = (int64_t)sh_pref + e;
+      const int64_t i = (int64_t)sh_list[e];
+      sp.hot[q] = (uint32_t)i;
       uint32_t ok = WAKE_NEVER;
-      if (sp.oldk[q] == 0u) {
+      if (sh_cont[e]) {
         const uint32_t sl = sp.slot[i];
         if ((int64_t)sl < sp.ccap) {
           if (q < sp.ccap) {
