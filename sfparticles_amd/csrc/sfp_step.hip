@@ -527,7 +527,8 @@ __device__ inline void photon_wake_position(const SpeciesDev& sp, const StepArgs
 // (1 if only this step is proven).  Windows end at the next multiple of
 // 32768, 4096, 512, 64 or WIN steps of the sleep clock (the longest proven
 // one), so that neighbouring photons -- one wave's lanes -- wake on the same
-// steps, and may last past the end of the run.  Photons of a species without QED sleep to the end of the run.
+// steps, and may last past the end of the run.  Photons of a species without
+// QED sleep to the end of the run.
 template <int FM>
 __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, const StepArgs& a,
                                        bool qed, double x,
@@ -557,12 +558,13 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
 // t of every row, so loads stay coalesced); the QED list keeps ascending
 // index order, i.e. (row, wave, lane) order, with one look-back per tile.
 //
-// Photons with a wake array: a photon whose wake step lies ahead is skipped
-// (it was pushed through that step already); a photon proven cold for n > 1
-// steps is pushed n steps at once -- the exact result of the 2n sequential
-// FP64 additions of the same increment that n single steps make
-// (sfp_repeat_add), so positions are bit-identical -- and sleeps until then.  Windows end at the end of the run, so every
-// position is current when the run returns.
+// Photons of a species without QED (this kernel; photons with QED have their
+// own kernels, below): a photon whose wake step lies ahead is skipped (it was
+// pushed through that step already); a photon is pushed to the end of the
+// run at once -- the exact result of the 2n sequential FP64 additions of the
+// same increment that n single steps make (sfp_repeat_add), so positions are
+// bit-identical -- and sleeps until then, so every position is current when
+// the run returns.
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM, bool MIX>
@@ -695,8 +697,8 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restr
       if (active && !hot) {
         if (PHOTON) {
           uint32_t nst = 1;
-          // multi-step decisions are taken every 8th step, so that the lanes of
-          // a wave take them together
+          // multi-step decisions are taken on every WIN-th step of the run,
+          // so that the lanes of a wave take them together
           if (wake)
             nst = !sp.push ? a.rel_end - a.rel
                            : ((a.rel % WIN) == 0 ? sleep_steps<FM>(P, FS, a, QED, x, y, z, ux, uy, uz, ig)
@@ -1686,17 +1688,18 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
 //              as live and skipped.
 //  k_photon -- one listed photon per thread: the envelope bound decides
 //              whether a pair is possible; if not, the photon is pushed one
-//              step, or -- every 8th step -- as many steps as the bound along
-//              its straight path allows (closed form, sfp_repeat_add) and put
-//              to sleep; if so, the exact Breit-Wheeler front runs, events
-//              are ranked in list order (= index order) and the pairs are
-//              appended already pushed; the converted photon becomes a hole
-//              and is never visited again.
+//              step or -- on rebuild steps -- put to sleep for as many steps
+//              as the bound along its straight path allows (its position
+//              advanced in closed form, sfp_repeat_add, when it wakes); if
+//              so, the exact Breit-Wheeler front runs, events are ranked in
+//              list order (= index order) and the pairs are appended already
+//              pushed; the converted photon becomes a hole and is never
+//              visited again.
 static constexpr uint32_t WAKE_NEVER = 0xffffffffu;
 
 // Wake steps per thread in k_awake: eight 16-byte loads (a tile of 8192
-// words per look-back).  The wake array is
-// allocated in whole tiles (WAKE_PAD words), so full-tile loads stay inside it.
+// words per look-back).  The wake array is allocated in whole tiles
+// (WAKE_PAD words), so full-tile loads stay inside it.
 static constexpr int AW = 32;
 static constexpr int64_t WAKE_PAD = (int64_t)TILE * AW;
 
@@ -1713,7 +1716,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   const int64_t n_old = sp.cnt->n_old;
   ScanState* scan = P.scan[s][0];
   if (!list_rebuild(a)) {
-    // Photons fall asleep and wake only on every 8th step: in between, the
+    // Photons fall asleep and wake only on rebuild steps: in between, the
     // list of the previous step stays valid (holes it holds are skipped) and
     // is extended by the photons appended since, whose indices are larger.
     const int64_t from = sp.cnt->list_end, n_list = sp.cnt->n_hot;
