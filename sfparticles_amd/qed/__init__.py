@@ -39,6 +39,12 @@ def set_seed(value: int) -> None:
 _tables_handle = None
 
 
+def _forget_device_tables():
+    """The next device_tables() uploads the tables anew (tables.use_tables)."""
+    global _tables_handle
+    _tables_handle = None
+
+
 def device_tables():
     """Library handle of the RS and OD tables (uploaded once per process)."""
     global _tables_handle

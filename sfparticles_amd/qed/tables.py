@@ -244,6 +244,58 @@ def load_od_tables():
     return _od_cache
 
 
+def _h5py():
+    try:
+        import h5py
+    except ImportError as exc:  # not a dependency of this package
+        raise ImportError("reading the reference's HDF5 table files needs h5py") from exc
+    return h5py
+
+
+def read_rs_tables_h5(path):
+    """The reference's rejection-sampling table file (written by
+    qed/rejection_sampling_tables.py:96-112, read there at :16-24): datasets
+    int_Ai and Aip, each carrying the attributes z_N, z_range, z_delta.  Ai
+    (the spin term) is not in that file and is generated on the same grid."""
+    h5py = _h5py()
+    with h5py.File(path, "r") as f:
+        d = f["int_Ai"]
+        z_range = d.attrs["z_range"]
+        return RSTables(d[()], f["Aip"][()], d.attrs["z_N"], z_range[0], z_range[1],
+                        d.attrs["z_delta"])
+
+
+def read_od_tables_h5(path):
+    """The reference's optical-depth table file (written by
+    qed/optical_depth_tables.py:206-230, read there at :17-30): datasets
+    photon_prob_rate_total, pair_prob_rate_total,
+    integral_photon_prob_along_delta, integral_pair_prob_along_delta; the
+    attributes chi_N, log_chi_range, log_chi_delta, delta_N, log_delta_range,
+    log_delta_delta on the file."""
+    h5py = _h5py()
+    with h5py.File(path, "r") as f:
+        a = f.attrs
+        return ODTables(f["photon_prob_rate_total"][()], f["pair_prob_rate_total"][()],
+                        f["integral_photon_prob_along_delta"][()],
+                        f["integral_pair_prob_along_delta"][()], a["chi_N"],
+                        a["log_chi_range"][0], a["log_chi_range"][1], a["log_chi_delta"],
+                        a["delta_N"], a["log_delta_range"][0], a["log_delta_range"][1],
+                        a["log_delta_delta"])
+
+
+def use_tables(rs=None, od=None):
+    """Tables for the simulations created from now on (RSTables / ODTables,
+    e.g. from read_rs_tables_h5); the default is the shipped pair.  Tables
+    already on the device stay there for the simulations that use them."""
+    global _rs_cache, _od_cache
+    if rs is not None:
+        _rs_cache = rs
+    if od is not None:
+        _od_cache = od
+    from . import _forget_device_tables
+    _forget_device_tables()
+
+
 def main():
     warnings.filterwarnings("ignore")  # quad's IntegrationWarnings near z -> inf, as in the reference
     os.makedirs(TABLE_DIR, exist_ok=True)

@@ -260,8 +260,14 @@ class Simulation(object):
         L.call("sfp_sim_reset_stats", self._sim)
 
     # ------------------------------------------------------------ driver --
-    def start(self, nstep: int, total: int = None, call_back: Callable[[int], None] = None):
-        """simulation.py:74-161"""
+    def start(self, nstep: int, total: int = None, call_back: Callable[[int], None] = None,
+              callback_every: int = 1):
+        """simulation.py:74-161.  ``call_back(istep)`` runs after the steps
+        istep with (istep + 1) a multiple of ``callback_every`` (1: after
+        every step, as in the reference); the steps in between run as one
+        call, without a host round trip (and as graph replays with
+        ``graph=True``)."""
+        callback_every = max(1, int(callback_every))
         if total is None:
             total = nstep
         if self.progress_bar is None and self.print_every and tqdm is not None:
@@ -271,15 +277,14 @@ class Simulation(object):
         tic = perf_counter_ns()
         while self.step < end:
             if call_back is not None:
-                chunk = 1
+                chunk = min(end - self.step, callback_every - self.step % callback_every)
             elif self.print_every:
                 chunk = min(end - self.step, self.print_every - self.step % self.print_every)
             else:
                 chunk = end - self.step
-            istep = self.step
             self.run_steps(chunk)
-            if call_back is not None:
-                call_back(istep)
+            if call_back is not None and self.step % callback_every == 0:
+                call_back(self.step - 1)
             if self.print_every and self.step % self.print_every == 0:
                 elapsed = perf_counter_ns() - tic
                 tic = perf_counter_ns()

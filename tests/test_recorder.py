@@ -109,3 +109,25 @@ def test_recorder_rejects_photons_and_can_be_removed():
     sim.record(e, 10, n=0)
     sim.start(4)
     assert sim.records()["t"].size == 0
+
+
+@pytest.mark.gpu
+def test_callback_every_k_steps():
+    """start(call_back=..., callback_every=k) calls back after the steps whose
+    index + 1 is a multiple of k, with the state a per-step callback sees."""
+    case = head_on_case(2000, 64, seed=14, a0=150.0)
+    every1, every5 = {}, {}
+
+    def grab(store, sim, e):
+        def cb(istep):
+            store[istep] = (np.array(e.x)[:100].copy(), sim.step)
+        return cb
+
+    sim, e, _ = _make(case)
+    sim.start(40, call_back=grab(every1, sim, e))
+    sim5, e5, _ = _make(case)
+    sim5.start(40, call_back=grab(every5, sim5, e5), callback_every=5)
+    assert sorted(every5) == [4, 9, 14, 19, 24, 29, 34, 39]
+    for istep, (x, step) in every5.items():
+        assert step == istep + 1
+        assert np.array_equal(x, every1[istep][0])
