@@ -418,28 +418,35 @@ __device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const S
 // cut, cpu.py:227-232, and leave no trace).  If the bound on p, doubled, is
 // below 2^-53 no event is possible.  Optical depth: skipped below chi_min
 // (optical_depth.py:43), so only that test is exact.
-__device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
-                                    double dt, bool spin = false) {
-  if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
+// Rejection sampling: the bound on p itself (< 0: no bound, evaluate; 0: no
+// event possible).
+__device__ inline double rs_pbound(const Plan& P, bool photon, float chi_b, double ig, double dt,
+                                   bool spin) {
   float num;
   if (photon) {
     num = 4.0f;
   } else {
     num = (float)(P.threshold * ig) * 0.999f;
-    if (!(num > 0.0f)) return true;
+    if (!(num > 0.0f)) return -1.0;
   }
-  if (!(chi_b >= 0.0f)) return true;                    // NaN bound: evaluate exactly
-  if ((float)P.zmax15 * chi_b < num) return false;      // z > z_max for every reachable d: w = 0
+  if (!(chi_b >= 0.0f)) return -1.0;                    // NaN bound: evaluate exactly
+  if ((float)P.zmax15 * chi_b < num) return 0.0;        // z > z_max for every reachable d: w = 0
   // lower bound of the smallest reachable z, rounded down
   float q = num / chi_b;
   float zlo = exp2f(log2f(q) * (2.0f / 3.0f)) * 0.999f;
   float fj = (zlo - (float)P.z_min) * (float)(1.0 / P.z_delta);
-  if (!(fj >= 1.0f)) return true;                       // first bracket: 2/z unbounded
+  if (!(fj >= 1.0f)) return -1.0;                       // first bracket: 2/z unbounded
   int64_t j = (int64_t)fj;
-  if (j > P.z_N - 2) return false;
+  if (j > P.z_N - 2) return 0.0;
   double2 e = spin ? P.rs_env_spin[j] : P.rs_env[j];
-  double pbound = P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
-  return !(2.0 * pbound < 0x1p-53);
+  return P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
+}
+
+__device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
+                                    double dt, bool spin = false) {
+  if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
+  const double pb = rs_pbound(P, photon, chi_b, ig, dt, spin);
+  return pb < 0.0 || !(2.0 * pb < 0x1p-53);
 }
 
 // ------------------------------------------------------------ exit protocol --
@@ -1960,14 +1967,29 @@ __global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __rest
       uint32_t shot = 0;
       const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
       bool hot = false;
+      double pb = -1.0;
       if (live && sp.push) {
         float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
         float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
-        hot = qed_possible(P, true, chi_b, ig, a.dt);
+        if (KIND == KIND_RS) {
+          pb = rs_pbound(P, true, chi_b, ig, a.dt, false);
+          hot = pb < 0.0 || !(2.0 * pb < 0x1p-53);
+        } else {
+          hot = qed_possible(P, true, chi_b, ig, a.dt);
+        }
       }
       bool event = false;
       double delta = 0.0;
-      if (hot) {
+      // Rejection sampling: a pair needs r2 < p <= pb.  The photon's own r2
+      // (the same Philox words as the exact front) at or above 2 pb rules
+      // it out without the exact field, chi and rate.
+      bool exact = hot;
+      if (KIND == KIND_RS && hot && pb >= 0.0) {
+        const sfp_u32x4 d = sfp_draw4(P.seed, sp.id[i], a.step, (uint32_t)sp.tag, 0u);
+        const double r2 = MIX ? (double)sfp32_u01(d.v[2]) : sfp_u01(d.v[2], d.v[3]);
+        exact = r2 < 2.0 * pb;
+      }
+      if (exact) {
         if (KIND == KIND_OD) tau = sp.tau[i];
         if (MIX) {
           event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x,
