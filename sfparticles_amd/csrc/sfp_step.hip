@@ -572,10 +572,14 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
 // same increment that n single steps make (sfp_repeat_add), so positions are
 // bit-identical -- and sleeps until then, so every position is current when
 // the run returns.
+//
+// Occupancy floor: 5 waves per SIMD in mixed precision (96 VGPRs), 3 in FP64
+// (168 VGPRs, a few spilled); each ran faster than the compiler's own choice
+// (104 and 203 VGPRs), and 6 or 4 waves ran slower.
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, MIX ? 5 : 1) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
