@@ -1189,8 +1189,14 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // creates them), then recoil (cpu.py:78-88) and push (simulation.py:128-137).
 // NETK: 0 tables, 1 the learned photon-energy sampler, 2 the learned
 // spin-resolved emission rate (mixed precision, rejection sampling)
+//
+// Occupancy: the compiler's own register count (about 160 VGPRs, 3 waves
+// per SIMD in mixed precision) ran faster than a cap at 128 VGPRs (4 waves)
+// or an explicit 3-wave floor, which schedules differently; the NETK == 2
+// instances keep the 4-wave cap, which ran faster for them.  The LDS of a
+// workgroup (37-50 KB) allows at most 4 workgroups per CU.
 template <int KIND, int FM, bool MIX, int NETK>
-__global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : 1) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1865,8 +1871,11 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   });
 }
 
+// Occupancy: the compiler's own register count (168 VGPRs, 3 waves per SIMD,
+// for the mixed-precision Gaussian-field instance) ran faster than a cap at
+// 128 VGPRs (4 waves), which spilled.
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, MIX ? 4 : 1) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
