@@ -13,8 +13,8 @@ steps starting at step 0 of the collision.  --warmup steps run first on a
 separate, smaller copy of the workload (kernel load, allocator warm-up).
 
 Prints one JSON line.  Also reported: the dominant kernel's achieved
-algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around each of
-its launches) and the CPU oracle timed on this machine's cores on a bounded
+algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around its
+launches on every 7th step of the timed region, --timing-every) and the CPU oracle timed on this machine's cores on a bounded
 sample of the same workload.
 
 --workload cascade runs the cascade configuration instead: 1e4 electrons at
@@ -148,7 +148,7 @@ def photon_energy_sample(g, chunks=100, size=2000, seed=1):
     return np.concatenate(out)
 
 
-def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250):
+def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=7):
     from sfparticles_amd import _lib as L
     if warmup > 0:
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision)
@@ -161,7 +161,7 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250):
     done = 0
     while done < steps:  # chunks, so that progress is visible during long runs
         m = min(chunk, steps - done)
-        sim.run_steps(m, timing=True)
+        sim.run_steps(m, timing=True, timing_every=timing_every)
         done += m
         print(f"[bench] {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
               f"{[s.N_buffered for s in species]}", file=sys.stderr, flush=True)
@@ -375,6 +375,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--timing-every", type=int, default=7,
+                    help="time the step kernels of every k-th step with HIP events (k coprime "
+                         "to the photon list's 8-step rebuild period samples all its phases)")
     ap.add_argument("--chunk", type=int, default=250,
                     help="head-on: steps per Simulation.run_steps call (a progress line each)")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -444,14 +447,19 @@ def main():
 
     def measure(precision):
         elapsed, stats, ktime, counts, spectrum = gpu_run(n, args.steps, args.warmup,
-                                                          nsteps_config, precision, args.chunk)
+                                                          nsteps_config, precision, args.chunk,
+                                                          args.timing_every)
+        # the kernels of steps 0, e, 2e, ... are timed (HIP events on the
+        # simulation's stream); sums scale to all steps by steps / timed
+        timed = len(range(0, args.steps, args.timing_every))
+        scale = args.steps / max(timed, 1)
         psteps = sum(s["psteps"] for s in stats)
         # dominant kernel: largest accumulated time over (species, phase); one
         # launch of each kernel per timed step does the work (launches replayed
         # after a capacity halt return at entry), so averages are per step
         dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)),
                       key=lambda ip: ktime[ip[0]][ip[1]][0])
-        ms, _ = ktime[dom][ph]
+        ms = ktime[dom][ph][0] * scale
         per_launch = kernel_bytes(dom == 2, ph, stats[dom]) / args.steps
         avg_s = ms / 1e3 / args.steps
         achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -465,13 +473,14 @@ def main():
                   (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_"
                  + precision.replace("+", "_"))
         traffic = load_traffic(kfile)
-        total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2))
+        total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2)) * scale
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
                     "kernel": kname, "avg_launch_ms": round(ms / args.steps, 4),
                     "alg_bytes_per_launch": int(per_launch),
-                    "kernel_time_share": round(ms / max(1e-9, total_ms), 3)}
+                    "kernel_time_share": round(ms / max(1e-9, total_ms), 3),
+                    "timed_steps": timed}
         detail = {"value": round(psteps / elapsed, 1),
                   "ms_per_step": round(elapsed / args.steps * 1e3, 4),
                   "final_counts": dict(zip(names, counts)),
@@ -479,7 +488,7 @@ def main():
                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
                   "photon_steps_in_multistep_push": stats[2]["slept"],
                   "photon_multistep_pushes": stats[2]["sleeps"],
-                  "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0], 2)
+                  "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0] * scale, 2)
                                 for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
         return detail, roofline, counts, spectrum

@@ -358,6 +358,12 @@ int sfp_sim_launch_shape(void* sim, int32_t species, int32_t phase, int32_t* gri
 int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
+/* With SFP_RUN_TIMING, time only the steps whose index `every` divides
+ * (default 1: every step); the two events around a launch cost about 8 us
+ * of step time, so a sample (every coprime to the photon list's 8-step
+ * rebuild period samples every phase of it) keeps the timed run close to
+ * an untimed one.  No counterpart in the reference (instrumentation). */
+int sfp_sim_set_timing_every(void* sim, int64_t every);
 /* Ensemble of independent shots (no counterpart in the reference): shot s has
  * its own field, the nterms[s] terms starting at terms + sum_{j<s} nterms[j].
  * Every particle of a species with a SFP_SHOT attribute moves in the field of

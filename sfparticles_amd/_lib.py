@@ -145,6 +145,7 @@ SIGNATURES = {
     "sfp_sim_run": [P, C.POINTER(D), U64, I64, I32],
     "sfp_sim_kernel_time": [P, I32, I32, C.POINTER(D), C.POINTER(I64)],
     "sfp_sim_reset_timing": [P],
+    "sfp_sim_set_timing_every": [P, I64],
     "sfp_store_histogram": [P, I32, I32, I32, D, D, P],
     "sfp_sim_set_grid": [P, I32, I32, I32],
     "sfp_sim_launch_shape": [P, I32, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],

@@ -2308,6 +2308,7 @@ struct Sim {
   sfp_sim_params params{};
   int fm = SFP_FM_ANY;
   bool timing = false;
+  int64_t timing_every = 1;         // with timing: the steps whose index it divides
   std::vector<EvRec> ev_pending[SFP_MAX_SPECIES][NPH];
   double ms_acc[SFP_MAX_SPECIES][NPH] = {};
   int64_t launches[SFP_MAX_SPECIES][NPH] = {};
@@ -2765,7 +2766,8 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
   kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
   const std::vector<int>& cand = sim->cand[s][ph];
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
-  const bool ev = sim->timing || tune >= 0;
+  const bool timed = sim->timing && sim->args_host[j].step % (uint64_t)sim->timing_every == 0;
+  const bool ev = timed || tune >= 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ev) {
     e0 = sim_event(sim);
@@ -2777,7 +2779,7 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
   HIPCHK(hipGetLastError());
   if (ev && e0 && e1) {
     HIPCHK(hipEventRecord(e1, sim->stream));
-    sim->ev_pending[s][ph].push_back({e0, e1, tune >= 0 ? c : -1, sim->timing});
+    sim->ev_pending[s][ph].push_back({e0, e1, tune >= 0 ? c : -1, timed});
   }
   return SFP_OK;
 }
@@ -3178,6 +3180,14 @@ extern "C" int sfp_sim_kernel_time(void* p, int32_t species, int32_t phase, doub
   CHK(sim_collect_timing(sim));
   *ms = sim->ms_acc[species][phase];
   *launches = sim->launches[species][phase];
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_timing_every(void* p, int64_t every) {
+  Sim* sim = (Sim*)p;
+  if (every < 1) return set_err(SFP_E_ARG, "timing_every must be >= 1, got %lld", (long long)every);
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  sim->timing_every = every;
   return SFP_OK;
 }
 
