@@ -1919,7 +1919,7 @@ __global__ void __launch_bounds__(TILE) k_photon(const Plan* __restrict__ plan, 
   const bool last = a.rel + 1 == a.rel_end;
   const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
 
-  uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), exact QED, multi-step pushes
+  uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), past the bound (id read), multi-step pushes
   bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
