@@ -254,8 +254,14 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
 static constexpr int QROWS = 16;
 
 __device__ inline int qed_rows(int64_t n_list) {
+  // the fewest rounds of tiles (one tile per workgroup per round) that
+  // QROWS-row tiles need, then the fewest rows that fill that many rounds: a
+  // list of 1.5 rounds of 16-row tiles would leave half the workgroups idle in
+  // the second; in 12-row tiles it fills two
   const int64_t per_round = (int64_t)gridDim.x * TILE;
-  const int64_t r = (n_list + per_round - 1) / per_round;
+  const int64_t m = (n_list + per_round * QROWS - 1) / (per_round * QROWS);
+  if (m < 1) return 1;
+  const int64_t r = (n_list + per_round * m - 1) / (per_round * m);
   return r < 1 ? 1 : (r > QROWS ? QROWS : (int)r);
 }
 
