@@ -392,6 +392,51 @@ SFP_HD float sfp_field_bound_f32_impl(const sfp_fieldset* fs, double x, double y
       S += 2.02f * (float)c[FG_E0] * exp2f(-a * LOG2E);
     } else if (kind == SFP_FIELD_STATIC) {
       S += 1.01f * (float)c[FST_S];
+    } else if (kind == SFP_FIELD_FOCUSED) {
+      // fields.py:173-265: the envelope E0 eta exp(-r^2/w(x)^2)
+      // exp(-phi^2/(k ctau)^2), eta = w0/w(x), times the epsilon corrections,
+      // each bounded term by term with |S_n|, |C_n| <= eta^n, |y|, |z| <= r,
+      // |y z| <= r^2/2 (rho = r/w0); the phase phi differs from
+      // k (x' - x0 - c t) by the Gouy phase (|.| < pi/2) and the curvature
+      // phase (|.| <= k r^2/(4 zR)).  In FP64, with 1e-3 rad and 1% margins.
+      const double zR = c[FF_ZR], k0 = c[FF_K0], w0 = c[FF_W0];
+      const double xa = c[FF_DIR] * (x - c[FF_XFOC]);
+      const double p0 = k0 * (xa - c[FF_DX0] - ct);
+      double pm = p0 < 0.0 ? -p0 : p0;
+      double amin = xa < 0.0 ? -xa : xa, amax = amin;
+      if (PATH) {
+        const double xb = c[FF_DIR] * (x1 - c[FF_XFOC]);
+        const double p1 = k0 * (xb - c[FF_DX0] - SFP_C * t1);
+        const double q1 = p1 < 0.0 ? -p1 : p1;
+        pm = ((p0 <= 0.0) != (p1 < 0.0)) ? 0.0 : (q1 < pm ? q1 : pm);
+        const double b1 = xb < 0.0 ? -xb : xb;
+        amin = ((xa <= 0.0) != (xb < 0.0)) ? 0.0 : (b1 < amin ? b1 : amin);
+        amax = b1 > amax ? b1 : amax;
+      }
+      const double q = (double)r2hi / (w0 * w0);  // rho^2, largest
+      double phl = pm - 1.5717963267948966 - k0 * (double)r2hi / (4.0 * zR) - pm * 1e-6;
+      if (phl < 0.0) phl = 0.0;
+      const double xz = amax / zR, u = amin / zR;
+      const double a = (double)r2 / (w0 * w0 * (1.0 + xz * xz)) + phl * phl / c[FF_KCT2];
+      const double h = 1.0 / sfp_sqrt(1.0 + u * u);  // eta, largest
+      const double r = sfp_sqrt(q);
+      const double eps = c[FF_EPS], e2 = c[FF_E2], e3 = c[FF_E3], e4 = c[FF_E4], e5 = c[FF_E5];
+      const double h2 = h * h, h3 = h2 * h, h4 = h3 * h, h5 = h4 * h, h6 = h5 * h, h7 = h6 * h;
+      const double q2 = q * q, q3 = q2 * q, q4 = q3 * q;
+      const double ex = r * (eps * h + e3 * (h2 / 2 + q * h3 + q2 * h4 / 4) +
+                             e5 * (3.0 / 8 * h3 + 3.0 / 8 * q * h4 + 17.0 / 16 * q2 * h5 +
+                                   3.0 / 8 * q3 * h6 + q4 * h7 / 32));
+      const double ey = 1.0 + e2 * (q * h2 + q2 * h3 / 4) +
+                        e4 * (h2 / 8 + q * h3 / 4 + 17.0 / 16 * q2 * h4 + 3.0 / 8 * q3 * h5 +
+                              q4 * h6 / 32);
+      const double ez = q / 2 * (e2 * h2 + e4 * (q * h4 + q2 * h5 / 4));
+      const double bx = r * (eps * h + e3 * (h2 / 2 + q * h3 / 2 + q2 * h4 / 4) +
+                             e5 * (3.0 / 8 * h3 + 3.0 / 8 * q * h4 + 3.0 / 16 * q2 * h5 +
+                                   q3 * h6 / 4 + q4 * h7 / 32));
+      const double bz = 1.0 + e2 * (q * h2 / 2 + q2 * h3 / 4) +
+                        e4 * (h2 / 8 + q * h3 / 4 + 5.0 / 16 * q2 * h4 + q3 * h5 / 4 + q4 * h6 / 32);
+      const double e0 = c[FF_E0] < 0.0 ? -c[FF_E0] : c[FF_E0];
+      S += (float)(1.01 * e0 * h * sfp_exp(-a) * (ex + ey + ez + bx + bz));
     } else {
       return __builtin_inff();
     }
@@ -419,7 +464,7 @@ SFP_HD float sfp_field_bound_f32_path(const sfp_fieldset* fs, double x, double y
  *            Gouy phase is within pi/2, the curvature phase k r^2/(2R) within
  *            k r^2/(4 zR), w0/w <= 1
  *  static:   |E| + c|B|
- *  focused:  no bound (+inf). */
+ *  focused:  no bound (+inf) here; the FP32 bound used by the kernels has one. */
 template <int FM = SFP_FM_ANY>
 SFP_HD double sfp_field_bound(const sfp_fieldset* fs, double x, double y, double z, double t) {
   double S = 0.0;

@@ -133,6 +133,20 @@ def test_qed_off_spin_ll_trajectories():
     compare_final(gpu, ref_libm, rtol=1e-10)
 
 
+@pytest.mark.parametrize("qed", ["rs", "od"])
+def test_head_on_focused_pulse_bit_exact(qed):
+    """QED with a focused pulse: the culling bound and the photons' sleep
+    along their paths use the focused envelope; every event still matches."""
+    case = head_on_case(n=3000, nsteps=240, seed=14, qed=qed)
+    c = case["field"][0][1]
+    case["field"] = [(3, (100.0, 2e-6, c[2], 1, 0.0, c[4], 0.8e-6, 0.3, 0.1))]
+    gpu, st = run_gpu_case(case)
+    ref = run_oracle_case(case)
+    print("counts", _names(gpu))
+    assert ref["photon"]["n"] > 1000
+    compare_final(gpu, ref)
+
+
 def test_focused_and_simple_fields_push():
     """Focused Gaussian and a sum of plane pulses through the fused push."""
     case = head_on_case(n=1000, nsteps=120, seed=21, species=("e",), rr="CLL")
@@ -291,7 +305,7 @@ def test_pair_bookkeeping_and_momentum():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["gaussian", "standing_wave"])
+@pytest.mark.parametrize("which", ["gaussian", "standing_wave", "focused"])
 def test_envelope_bounds_are_upper_bounds(which):
     """The FP32 envelope bounds the step culls with exceed |E| + c|B| of the
     exact FP64 field (oracle): the point bound at the point, the path bound at
@@ -302,14 +316,17 @@ def test_envelope_bounds_are_upper_bounds(which):
     from sfparticles_amd.fields import terms_to_c
     C_LIGHT = 299792458.0
     rng = np.random.Generator(np.random.PCG64(7))
-    if which == "gaussian":
+    if which in ("gaussian", "focused"):
         case = head_on_case(n=1, nsteps=3000)
+        if which == "focused":  # at its focus at t0, polarisation and CEP not 0
+            c = case["field"][0][1]
+            case["field"] = [(3, (100.0, 2e-6, c[2], 1, 0.0, c[4], 0.8e-6, 0.3, 0.1))]
     else:
         case = cascade_case(n=1, nsteps=100, a0=500.0, x0_ctau=1.0)
     terms = case["field"]
     tc = terms_to_c(terms)
     m = 4000
-    t0 = 1500 * case["dt"] if which == "gaussian" else 0.0
+    t0 = 1500 * case["dt"] if which != "standing_wave" else 0.0
     x = rng.uniform(-20e-6, 20e-6, m)
     y = rng.normal(0, 2e-6, m)
     z = rng.normal(0, 2e-6, m)
