@@ -1202,7 +1202,8 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // instances keep the 4-wave cap, which ran faster for them.  In FP64 with
 // general fields (focused pulses) the compiler's choice spills into AGPRs and
 // runs 1 wave per SIMD; a 2-wave floor (256 VGPRs, a few spills to scratch)
-// ran 15% faster.  The LDS of a workgroup (37-50 KB) allows at most 4
+// ran faster (a 1e6-electron focused-pulse collision: 3.2e10 -> 3.5e10
+// particle-steps/s).  The LDS of a workgroup (37-50 KB) allows at most 4
 // workgroups per CU.
 template <int KIND, int FM, bool MIX, int NETK>
 __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
@@ -1883,8 +1884,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 // Occupancy: the compiler's own register count (168 VGPRs, 3 waves per SIMD,
 // for the mixed-precision Gaussian-field instance) ran faster than a cap at
 // 128 VGPRs (4 waves), which spilled.  FP64 with general fields: a 2-wave
-// floor, as for k_hot (photon kernel 4.95 -> 3.93 s per 3000 steps of a
-// focused-pulse collision).
+// floor, as for k_hot (photon kernel 0.26 -> 0.22 s per 3000 steps of a
+// 1e6-electron focused-pulse collision).
 template <int KIND, int FM, bool MIX>
 __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : 1) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
