@@ -88,7 +88,7 @@ def kernel_bytes(photon, phase, st, od=False):
     return 60 * hot + 48 * hot + tau + 57 * st["spawned"]
 
 
-def make_case(n, nsteps, seed=0):
+def make_case(n, nsteps, seed=0, field="gaussian"):
     rng = np.random.Generator(np.random.PCG64(seed))
     gamma = rng.normal(1957.0, 0.05 * 1957.0, n)
     d = C_LIGHT * DT * nsteps / 2  # 12 um for 3000 steps: the pulse meets the bunch half-way
@@ -96,15 +96,19 @@ def make_case(n, nsteps, seed=0):
     y = rng.normal(0.0, 1e-6, n)
     z = rng.normal(0.0, 1e-6, n)
     ux = -np.sqrt(gamma ** 2 - 1)
-    return (x, y, z, ux, np.zeros(n), np.zeros(n)), [(2, (100.0, 2e-6, CTAU, 1, -d, LAMBDA, 0.0, 0.0))]
+    if field == "focused":  # the same pulse in the focused expansion, focus at x = 0
+        term = (3, (100.0, 2e-6, CTAU, 1, 0.0, -d, LAMBDA, 0.0, 0.0))
+    else:
+        term = (2, (100.0, 2e-6, CTAU, 1, -d, LAMBDA, 0.0, 0.0))
+    return (x, y, z, ux, np.zeros(n), np.zeros(n)), [term]
 
 
-def build_gpu(n, nsteps, reserve, path="fp64"):
+def build_gpu(n, nsteps, reserve, path="fp64", field="gaussian"):
     import sfparticles_amd as sfa
     from sfparticles_amd import qed
     from sfparticles_amd.fields import Fields
     qed.set_model("rs")
-    props, field = make_case(n, nsteps)
+    props, field = make_case(n, nsteps, field=field)
     e = sfa.Particles("electron", -1, 1, N=n, props=props)
     p = sfa.Particles("positron", 1, 1)
     g = sfa.Particles("photon", 0, 0)
@@ -148,14 +152,15 @@ def photon_energy_sample(g, chunks=100, size=2000, seed=1):
     return np.concatenate(out)
 
 
-def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=7):
+def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=7,
+            field="gaussian"):
     from sfparticles_amd import _lib as L
     if warmup > 0:
-        wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision)
+        wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision, field)
         wsim.run_steps(warmup)
         del wsim
     reserve = (2 * n + 4096, n + 4096, 28 * n + 4096)
-    sim, species = build_gpu(n, nsteps_config, reserve, precision)
+    sim, species = build_gpu(n, nsteps_config, reserve, precision, field)
     L.call("sfp_synchronize")
     t0 = time.perf_counter()
     done = 0
@@ -375,6 +380,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=float, default=1e6)
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--field", choices=["gaussian", "focused"], default="gaussian",
+                    help="head-on workload: the paraxial Gaussian pulse (the benchmark) or the "
+                         "same pulse in the focused expansion")
     ap.add_argument("--timing-every", type=int, default=7,
                     help="time the step kernels of every k-th step with HIP events (k coprime "
                          "to the photon list's 8-step rebuild period samples all its phases)")
@@ -448,7 +456,7 @@ def main():
     def measure(precision):
         elapsed, stats, ktime, counts, spectrum = gpu_run(n, args.steps, args.warmup,
                                                           nsteps_config, precision, args.chunk,
-                                                          args.timing_every)
+                                                          args.timing_every, args.field)
         # the kernels of steps 0, e, 2e, ... are timed (HIP events on the
         # simulation's stream); sums scale to all steps by steps / timed
         timed = len(range(0, args.steps, args.timing_every))
@@ -463,15 +471,17 @@ def main():
         per_launch = kernel_bytes(dom == 2, ph, stats[dom]) / args.steps
         avg_s = ms / 1e3 / args.steps
         achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-        kname = ({(0, False): "k_sweep<lepton, RS, gaussian field>",
-                  (1, False): "k_hot<RS, gaussian field>", (0, True): "k_awake",
-                  (1, True): "k_photon<RS, gaussian field>",
-                  (2, False): "k_emit<RS, gaussian field>"}[(ph, dom == 2)]
+        fk = "gaussian" if args.field == "gaussian" else "general"
+        kname = ({(0, False): f"k_sweep<lepton, RS, {fk} field>",
+                  (1, False): f"k_hot<RS, {fk} field>", (0, True): "k_awake",
+                  (1, True): f"k_photon<RS, {fk} field>",
+                  (2, False): f"k_emit<RS, {fk} field>"}[(ph, dom == 2)]
                  + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
         kfile = ({(0, False): "k_sweep", (1, False): "k_hot", (0, True): "k_awake",
                   (2, False): "k_emit",
                   (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_"
-                 + precision.replace("+", "_"))
+                 + precision.replace("+", "_")
+                 + ("" if args.field == "gaussian" else "_" + args.field))
         traffic = load_traffic(kfile)
         total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2)) * scale
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -497,7 +507,7 @@ def main():
     res = {p: measure(p) for p in paths}
     head = paths[-1]
     detail, roofline, counts, spectrum = res[head]
-    config = {"workload": f"head-on collision, {n} e-, a0 100, gaussian pulse, RS tables, "
+    config = {"workload": f"head-on collision, {n} e-, a0 100, {args.field} pulse, RS tables, "
                           f"{nsteps_config}-step geometry",
               "path": {"fp64": "fp64-tables", "mixed": "mixed-fp32/fp64-tables",
                        "mixed+mlp": "mixed-fp32/fp64 + bf16-mfma photon-energy network"}[head],
