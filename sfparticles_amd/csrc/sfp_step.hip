@@ -1885,9 +1885,11 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 // for the mixed-precision Gaussian-field instance) ran faster than a cap at
 // 128 VGPRs (4 waves), which spilled.  FP64 with general fields: a 2-wave
 // floor, as for k_hot (photon kernel 0.26 -> 0.22 s per 3000 steps of a
-// 1e6-electron focused-pulse collision).
+// 1e6-electron focused-pulse collision).  Mixed precision with optical
+// depth: a 3-wave floor (168 VGPRs; the compiler chose 202, 2 waves) ran the
+// photon kernel of a 4e6-electron collision in 3.65 s instead of 3.90.
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : 1) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX && KIND == KIND_OD) ? 3 : 1)) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
