@@ -162,10 +162,26 @@ static void o_field_gaussian(const double* p, double x, double y, double z, doub
   F[5] = Ey / O_C * direction;
 }
 
-/* fields.py:173-265 */
+/* fields.py:173-265.
+ * p[9] (not a reference argument; 0 in every product and parity use) selects
+ * the sign convention.  0: the reference's, +psiG in the phase and the
+ * longitudinal components as written.  1: the signs under which the
+ * expansion (Salamin & Keitel, PRL 88, 095005 (2002), written there for the
+ * phase omega t - k z) is a solution of Maxwell's equations in the phase
+ * k x - omega t used here: the Gouy phase enters with -psiG, Ex and Bx change
+ * sign, Ex is mirrored with the direction as By and Bz are, and the
+ * transverse coordinates are rotated with the polarisation (the reference
+ * rotates the field vectors by pol_angle but evaluates them at the unrotated
+ * y, z, fields.py:257-263).  The
+ * coefficients are the same code in both; mode 1 exists so that tests can pin
+ * them against Maxwell's equations (the residuals fall as eps^6 only if every
+ * coefficient up to eps^5 is right), which the reference's own signs do not
+ * satisfy beyond O(eps). */
 static void o_field_focused(const double* p, double x, double y, double z, double t, double* F) {
   double a0 = p[0], w0 = p[1], ctau = p[2], direction = p[3], xfoc = p[4], x0 = p[5];
   double l0 = p[6], pol_angle = p[7], cep = p[8];
+  double gs = p[9] != 0.0 ? -1.0 : 1.0;
+  double ex_dir = p[9] != 0.0 ? direction : 1.0;
   double omega0 = 2 * O_PI * O_C / l0;
   double k0 = 2 * O_PI / l0;
   double zR = O_PI * (w0 * w0) / l0;
@@ -174,10 +190,16 @@ static void o_field_focused(const double* p, double x, double y, double z, doubl
   double E0 = a0 / a0_norm;
 
   x = direction * (x - xfoc);
+  if (p[9] != 0.0) {
+    double c_ = O_COS(pol_angle), s_ = O_SIN(pol_angle);
+    double yl = y * c_ + z * s_;
+    z = -y * s_ + z * c_;
+    y = yl;
+  }
   double r2 = y * y + z * z;
   double xz = x / zR;
   double wx = w0 * O_SQRT(1 + xz * xz);
-  double psiG = O_ATAN(x / zR);
+  double psiG = gs * O_ATAN(x / zR);
   double psiR = k0 * x * r2 / 2 / (x * x + zR * zR);
   double psiP = k0 * (x - direction * x0 - O_C * t);
   double psi = psiG + psiR + psiP + cep;
@@ -225,10 +247,10 @@ static void o_field_focused(const double* p, double x, double y, double z, doubl
                (S0 + e2 * (rho2 * S2 / 2 - rho4 * S3 / 4) +
                 e4 * (-S2 / 8 + rho2 * S3 / 4 + 5.0 / 16 * rho4 * S4 - rho6 * S5 / 4 + rho8 * S6 / 32));
   double cp = O_COS(pol_angle), sp = O_SIN(pol_angle);
-  F[0] = Ex_;
+  F[0] = Ex_ * gs * ex_dir;
   F[1] = Ey_ * cp - Ez_ * sp;
   F[2] = Ey_ * sp + Ez_ * cp;
-  F[3] = Bx_;
+  F[3] = Bx_ * gs;
   F[4] = (By_ * cp - Bz_ * sp) * direction;
   F[5] = (By_ * sp + Bz_ * cp) * direction;
 }
