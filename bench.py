@@ -130,30 +130,64 @@ def build_gpu(n, nsteps, reserve, path="fp64", field="gaussian"):
     return sim, (e, p, g)
 
 
-def photon_energy_sample(g, chunks=100, size=2000, seed=1):
-    """|u| of live photons in `chunks` slices of `size` at the same pseudo-random
-    offsets of the photon store (a sample of the spectrum without copying the
-    whole store)."""
+def _settle(g):
+    """Bring sleeping photons to the last step run (the store's catch-up
+    callback runs before any read): a one-element download."""
+    import ctypes as C
+    from sfparticles_amd import _lib as L
+    if g.N_buffered:
+        one = np.empty(1)
+        L.call("sfp_store_download", g._store, L.X, one.ctypes.data_as(C.c_void_p), 0, 1)
+
+
+def photon_energy_sample(g, size=200000, seed=1, chunk=1 << 24):
+    """|u| of `size` live photons drawn uniformly without replacement from the
+    whole photon store (every live photon when there are fewer), read in
+    chunks so that the store is never copied whole."""
     import ctypes as C
     from sfparticles_amd import _lib as L
     n = g.N_buffered
     rng = np.random.Generator(np.random.PCG64(seed))
-    offs = np.sort(rng.integers(0, max(n - size, 1), chunks))
     out = []
-    for o in offs:
-        m = int(min(size, n - o))
+    for o in range(0, n, chunk):
+        m = int(min(chunk, n - o))
         u = []
         for attr in (L.UX, L.UY, L.UZ, L.PRUNED):
             a = np.empty(m, np.uint8 if attr == L.PRUNED else np.float64)
             L.call("sfp_store_download", g._store, attr, a.ctypes.data_as(C.c_void_p), int(o), m)
             u.append(a)
-        live = u[3] == 0
-        out.append(np.sqrt(u[0] ** 2 + u[1] ** 2 + u[2] ** 2)[live])
-    return np.concatenate(out)
+        out.append(np.sqrt(u[0] ** 2 + u[1] ** 2 + u[2] ** 2)[u[3] == 0])
+    e = np.concatenate(out) if out else np.zeros(0)
+    if e.size > size:
+        e = e[np.sort(rng.choice(e.size, size, replace=False))]
+    return e
+
+
+def sample_state(species, every):
+    """Every `every`-th live particle of each species (all attributes the
+    oracle needs, with ids), for the CPU baseline on the same window."""
+    import ctypes as C
+    from sfparticles_amd import _lib as L
+    out = []
+    for sp in species:
+        n = sp.N_buffered
+        d = {}
+        for name, attr, dt in (("x", L.X, np.float64), ("y", L.Y, np.float64),
+                               ("z", L.Z, np.float64), ("ux", L.UX, np.float64),
+                               ("uy", L.UY, np.float64), ("uz", L.UZ, np.float64),
+                               ("id", L.ID, np.uint64), ("pruned", L.PRUNED, np.uint8)):
+            a = np.empty(n, dt)
+            if n:
+                L.call("sfp_store_download", sp._store, attr, a.ctypes.data_as(C.c_void_p), 0, n)
+            d[name] = a
+        keep = np.flatnonzero(d["pruned"] == 0)[::every]
+        out.append({k: np.ascontiguousarray(v[keep]) for k, v in d.items() if k != "pruned"}
+                   | {"next_id": int(sp.next_id) if hasattr(sp, "next_id") else 0})
+    return out
 
 
 def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=7,
-            field="gaussian"):
+            field="gaussian", cpu_every=0):
     from sfparticles_amd import _lib as L
     if warmup > 0:
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision, field)
@@ -161,6 +195,23 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
         del wsim
     reserve = (2 * n + 4096, n + 4096, 28 * n + 4096)
     sim, species = build_gpu(n, nsteps_config, reserve, precision, field)
+    # a run shorter than the geometry times the window of `steps` steps
+    # centred on the collision (step nsteps_config / 2: the pulse centre
+    # meets the bunch centre), after an untimed fast-forward
+    start = 0 if steps >= nsteps_config else nsteps_config // 2 - steps // 2
+    t0 = time.perf_counter()
+    done = 0
+    while done < start:
+        m = min(chunk, start - done)
+        sim.run_steps(m)
+        done += m
+        print(f"[bench] {precision}: fast-forward {done}/{start} steps, "
+              f"{time.perf_counter() - t0:.1f} s, {[s.N_buffered for s in species]}",
+              file=sys.stderr, flush=True)
+    _settle(species[2])  # sleeping photons at the window's first step (untimed)
+    base = sample_state(species, cpu_every) if cpu_every else None
+    sim.reset_stats()
+    sim.reset_timing()
     L.call("sfp_synchronize")
     t0 = time.perf_counter()
     done = 0
@@ -168,8 +219,9 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
         m = min(chunk, steps - done)
         sim.run_steps(m, timing=True, timing_every=timing_every)
         done += m
-        print(f"[bench] {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
+        print(f"[bench] {precision}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
               f"{[s.N_buffered for s in species]}", file=sys.stderr, flush=True)
+    _settle(species[2])  # photons asleep in the window brought to its last step (timed)
     L.call("sfp_synchronize")
     t1 = time.perf_counter()
     stats = [sim.stats(i) for i in range(3)]
@@ -177,34 +229,48 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
     spectrum = photon_energy_sample(species[2])
     counts = [s.N_buffered for s in species]
     del sim, species
-    return t1 - t0, stats, ktime, counts, spectrum
+    return t1 - t0, stats, ktime, counts, spectrum, start, base
 
 
-def cpu_baseline(n, nsteps):
+def cpu_baseline(base, start, steps, every):
     """The CPU oracle in reference-structured mode (one OpenMP pass per
-    reference function, serial counts/compaction) on a bounded sample."""
+    reference function, serial counts/compaction) over the same window of
+    steps as the GPU line, on a bounded sample of the same state: every
+    `every`-th live particle of each species as the FP64 GPU run held them at
+    the window's first step (ids kept, so the draws are those of the same
+    particles)."""
     import oracle
     from oracle.sim import OParticles, OSimulation
     from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
-    props, field = make_case(n, nsteps)
-    e = OParticles("electron", -1, 1, N=n, props=[a.copy() for a in props])
-    p = OParticles("positron", 1, 1)
-    g = OParticles("photon", 0, 0)
+    _, field = make_case(1, 3000)
+    made = []
+    for d, (name, q, m) in zip(base, (("electron", -1, 1), ("positron", 1, 1), ("photon", 0, 0))):
+        k = d["x"].size
+        sp = OParticles(name, q, m, N=k, props=[d[a].copy() for a in
+                                                ("x", "y", "z", "ux", "uy", "uz")])
+        sp.id[:] = d["id"]
+        sp.next_id = d["next_id"]
+        made.append(sp)
+    e, p, g = made
     e.set_photon(g)
     p.set_photon(g)
     g.set_pair(e, p)
     sim = OSimulation(e, p, g, dt=DT, fields=field, photon_threshold=2.0, qed="rs", seed=0,
                       rs_tables=load_rs_tables(), od_tables=load_od_tables())
+    for _ in range(start):  # the time as the GPU run accumulates it
+        sim.t += DT
+    sim.step = start
     psteps = 0
 
     def cb(i):
         nonlocal psteps
         psteps += e.Npart + p.Npart + g.Npart
 
+    n0 = [s_.Npart for s_ in made]
     t0 = time.perf_counter()
-    sim.start(nsteps, call_back=cb, prune=False)
+    sim.start(steps, call_back=cb, prune=False)
     t1 = time.perf_counter()
-    return psteps / (t1 - t0), t1 - t0, oracle.num_threads()
+    return psteps / (t1 - t0), t1 - t0, oracle.num_threads(), n0
 
 
 def load_traffic(kernel):
@@ -378,8 +444,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=float, default=1e7)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=float, default=1e6)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-every", type=int, default=20,
+                    help="CPU baseline: every k-th particle of the GPU state at the window start")
     ap.add_argument("--field", choices=["gaussian", "focused"], default="gaussian",
                     help="head-on workload: the paraxial Gaussian pulse (the benchmark) or the "
                          "same pulse in the focused expansion")
@@ -453,18 +519,19 @@ def main():
     nsteps_config = 3000
     names = ["electron", "positron", "photon"]
 
+    cpu_every = 0 if args.no_cpu else max(1, int(args.cpu_every))
+
     def measure(precision):
-        elapsed, stats, ktime, counts, spectrum = gpu_run(n, args.steps, args.warmup,
-                                                          nsteps_config, precision, args.chunk,
-                                                          args.timing_every, args.field)
-        # the kernels of steps 0, e, 2e, ... are timed (HIP events on the
-        # simulation's stream); sums scale to all steps by steps / timed
-        timed = len(range(0, args.steps, args.timing_every))
+        elapsed, stats, ktime, counts, spectrum, start, base = gpu_run(
+            n, args.steps, args.warmup, nsteps_config, precision, args.chunk, args.timing_every,
+            args.field, cpu_every if precision == "fp64" else 0)
+        # the kernels of every timing_every-th step are timed (HIP events on
+        # the simulation's stream); sums scale to all steps by steps / timed
+        timed = sum(1 for k in range(start, start + args.steps) if k % args.timing_every == 0)
         scale = args.steps / max(timed, 1)
         psteps = sum(s["psteps"] for s in stats)
-        # dominant kernel: largest accumulated time over (species, phase); one
-        # launch of each kernel per timed step does the work (launches replayed
-        # after a capacity halt return at entry), so averages are per step
+        asleep = stats[2]["slept"]
+        # dominant kernel: largest accumulated time over (species, phase)
         dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)),
                       key=lambda ip: ktime[ip[0]][ip[1]][0])
         ms = ktime[dom][ph][0] * scale
@@ -477,66 +544,97 @@ def main():
                   (1, True): f"k_photon<RS, {fk} field>",
                   (2, False): f"k_emit<RS, {fk} field>"}[(ph, dom == 2)]
                  + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
-        kfile = ({(0, False): "k_sweep", (1, False): "k_hot", (0, True): "k_awake",
-                  (2, False): "k_emit",
-                  (1, True): "k_photon"}[(ph, dom == 2)] + "_" + names[dom] + "_rs_"
-                 + precision.replace("+", "_")
-                 + ("" if args.field == "gaussian" else "_" + args.field))
-        traffic = load_traffic(kfile)
         total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2)) * scale
+        # whole step at the survey's algorithmic bytes per particle-step: e+-
+        # 105 B (FP64) or 81 B (mixed: FP64 x, FP32 u), photons 81 B per
+        # executed (awake) photon-step; sleeping photons move no bytes
+        b_lep = 105 if precision == "fp64" else 81
+        step_bytes = b_lep * (stats[0]["psteps"] + stats[1]["psteps"]) + 81 * (
+            stats[2]["psteps"] - asleep)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                    "traffic": None,
                     "kernel": kname, "avg_launch_ms": round(ms / args.steps, 4),
                     "alg_bytes_per_launch": int(per_launch),
                     "kernel_time_share": round(ms / max(1e-9, total_ms), 3),
-                    "timed_steps": timed}
+                    "timed_steps": timed,
+                    "whole_step": {"alg_bytes_per_step": int(step_bytes / args.steps),
+                                   "GBps": round(step_bytes / elapsed / 1e9, 1),
+                                   "frac": round(step_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4)}}
         detail = {"value": round(psteps / elapsed, 1),
+                  "executed_value": round((psteps - asleep) / elapsed, 1),
                   "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+                  "window": [start, start + args.steps],
                   "final_counts": dict(zip(names, counts)),
                   "particle_steps": {names[i]: stats[i]["psteps"] for i in range(3)},
+                  "photon_steps_asleep": asleep,
                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
-                  "photon_steps_in_multistep_push": stats[2]["slept"],
                   "photon_multistep_pushes": stats[2]["sleeps"],
                   "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0] * scale, 2)
                                 for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
-        return detail, roofline, counts, spectrum
+        return detail, roofline, counts, spectrum, base, start
 
     paths = ["fp64", "mixed", "mixed+mlp"] if args.path == "all" else [args.path]
     res = {p: measure(p) for p in paths}
+
+    def errors(path):
+        """Yield and spectrum differences of a path to the FP64 table path on
+        the same workload, with the statistical resolution of each (3 sigma
+        of the FP64 count); the gate: every yield within max(1 %, 3 sigma)
+        and the photon spectra's two-sample KS p > 0.01."""
+        from scipy import stats as st
+        _, _, c64, s64, _, _ = res["fp64"]
+        _, _, c, sp, _, _ = res[path]
+        ks = st.ks_2samp(s64, sp)
+        e = {"photon_yield": round((c[2] - c64[2]) / max(c64[2], 1), 6),
+             "photon_yield_3sigma": round(3 / np.sqrt(max(c64[2], 1)), 6),
+             "pair_yield": round((c[1] - c64[1]) / max(c64[1], 1), 6),
+             "pair_yield_3sigma": round(3 / np.sqrt(max(c64[1], 1)), 6),
+             "photons_fp64": int(c64[2]), "pairs_fp64": int(c64[1]),
+             "photon_spectrum_ks_distance": round(float(ks.statistic), 6),
+             "photon_spectrum_ks_p": float(ks.pvalue),
+             "photon_spectrum_sample": [int(len(s64)), int(len(sp))]}
+        e["gate"] = bool(abs(e["photon_yield"]) <= max(0.01, e["photon_yield_3sigma"])
+                         and abs(e["pair_yield"]) <= max(0.01, e["pair_yield_3sigma"])
+                         and e["photon_spectrum_ks_p"] > 0.01)
+        return e
+
+    # the headline is the last path named (mixed+MLP by default) if it passes
+    # its error gate against FP64, else the FP64 table path
+    errs = {p: errors(p) for p in paths if p != "fp64" and "fp64" in res}
     head = paths[-1]
-    detail, roofline, counts, spectrum = res[head]
+    if head in errs and not errs[head]["gate"]:
+        head = "fp64"
+    detail, roofline, counts, spectrum, _, start = res[head]
+    window = (f"steps {start}-{start + args.steps} (the collision centre)" if start
+              else f"steps 0-{args.steps}")
     config = {"workload": f"head-on collision, {n} e-, a0 100, {args.field} pulse, RS tables, "
-                          f"{nsteps_config}-step geometry",
+                          f"{nsteps_config}-step geometry, timed window {window}",
               "path": {"fp64": "fp64-tables", "mixed": "mixed-fp32/fp64-tables",
                        "mixed+mlp": "mixed-fp32/fp64 + bf16-mfma photon-energy network"}[head],
               "electrons": n, "steps_timed": args.steps}
     config.update(detail)
-    if len(paths) > 1:
-        d64, _, c64, s64 = res["fp64"]
-        from scipy import stats as st
-        config["fp64"] = d64
-        if "mixed" in res and head != "mixed":
-            config["mixed"] = res["mixed"][0]
-        config["speedup_vs_fp64"] = round(detail["value"] / d64["value"], 3)
-        config["error_vs_fp64"] = {
-            "photon_yield": round((counts[2] - c64[2]) / c64[2], 6),
-            "pair_yield": round((counts[1] - c64[1]) / max(c64[1], 1), 6),
-            "pair_yield_3sigma": round(3 * np.sqrt(max(c64[1], 1)) / max(c64[1], 1), 6),
-            "photon_spectrum_ks_distance": float(st.ks_2samp(s64, spectrum).statistic),
-            "photon_spectrum_ks_p": float(st.ks_2samp(s64, spectrum).pvalue),
-            "photon_spectrum_sample": int(len(spectrum))}
+    for p_ in paths:
+        if p_ != head:
+            config[p_] = res[p_][0]
+    if "fp64" in res and head != "fp64":
+        config["speedup_vs_fp64"] = round(detail["value"] / res["fp64"][0]["value"], 3)
+    if errs:
+        config["error_vs_fp64"] = errs
+    if "mixed" in res and "mixed+mlp" in res:
+        config["sampler_gain"] = round(res["mixed+mlp"][0]["value"] / res["mixed"][0]["value"], 3)
 
     cpu = None
     if not args.no_cpu:
-        cn = int(args.cpu_n)
-        rate, secs, threads = cpu_baseline(cn, args.cpu_steps)
+        base, start64 = res["fp64"][4], res["fp64"][5]
+        rate, secs, threads, n0 = cpu_baseline(base, start64, args.steps, cpu_every)
         cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
-               "sample": f"CPU oracle, reference-structured passes (OpenMP), FP64; same head-on "
-                         f"workload scaled to {cn} electrons over the {args.cpu_steps} steps "
-                         f"around the collision ({secs:.1f} s)"}
+               "sample": f"CPU oracle, reference-structured passes (OpenMP), FP64, over the same "
+                         f"window ({window}) on every {cpu_every}th live particle of the FP64 "
+                         f"GPU state at its first step ({n0[0]} e-, {n0[1]} e+, {n0[2]} photons; "
+                         f"{secs:.1f} s)"}
 
     line = {
         "metric": "particle-steps/s (1e7 e-, QED on): FP64 tables vs mixed+MLP at <=1% yield err",

@@ -170,3 +170,64 @@ def bool_sum(event, N):
     cnt = C.c_int64(0)
     L.call("sfp_bool_sum", _b(event).ptr, int(N), C.byref(cnt), None)
     return int(cnt.value)
+
+
+# --------------------------------------------------------------------------
+# The QED model seam (particles.py:8-12): the reference's callables with its
+# argument order, for numpy arguments.  The reference draws from numba's
+# unseeded numpy.random; these draw Philox4x32-10 words keyed by ``seed``
+# with counter (id, step, tag, slot), so the extra keyword arguments name the
+# stream (defaults: ids 0..N-1, the package seed, step 0, tag 0 and the
+# per-operation slot).
+def _qed_stream(N, ids, seed, step, tag, slot):
+    from . import qed
+    ids = np.arange(int(N), dtype=np.uint64) if ids is None else np.asarray(ids, np.uint64)
+    return (_dev(ids, np.uint64), qed.seed if seed is None else int(seed), int(step), int(tag),
+            qed.PER_OP_SLOT if slot is None else int(slot), qed.device_tables())
+
+
+def _rs(name, inv_gamma, chi, dt, N, to_be_pruned, event, delta, ids, seed, step, tag, slot):
+    did, sd, st, tg, sl, tables = _qed_stream(N, ids, seed, step, tag, slot)
+    de, dd = _b(event), _f(delta)
+    L.call(name, _f(inv_gamma).ptr, _f(chi).ptr, float(dt), int(N), _b(to_be_pruned).ptr, de.ptr,
+           dd.ptr, did.ptr, sd, st, tg, sl, tables, None)
+    _back_bool(de, event)
+    _back(dd, delta)
+
+
+def photon_from_rejection_sampling(inv_gamma, chi_e, dt, N, to_be_pruned, event, photon_delta,
+                                   ids=None, seed=None, step=0, tag=0, slot=None):
+    """qed/rejection_sampling.py:68-85"""
+    _rs("sfp_photon_from_rejection_sampling", inv_gamma, chi_e, dt, N, to_be_pruned, event,
+        photon_delta, ids, seed, step, tag, slot)
+
+
+def pair_from_rejection_sampling(inv_gamma, chi_gamma, dt, N, to_be_pruned, event, pair_delta,
+                                 ids=None, seed=None, step=0, tag=0, slot=None):
+    """qed/rejection_sampling.py:87-101"""
+    _rs("sfp_pair_from_rejection_sampling", inv_gamma, chi_gamma, dt, N, to_be_pruned, event,
+        pair_delta, ids, seed, step, tag, slot)
+
+
+def _od(name, tau, inv_gamma, chi, dt, N, to_be_pruned, event, delta, ids, seed, step, tag, slot):
+    did, sd, st, tg, sl, tables = _qed_stream(N, ids, seed, step, tag, slot)
+    dtau, de, dd = _f(tau), _b(event), _f(delta)
+    L.call(name, dtau.ptr, _f(inv_gamma).ptr, _f(chi).ptr, float(dt), int(N), _b(to_be_pruned).ptr,
+           de.ptr, dd.ptr, did.ptr, sd, st, tg, sl, tables, None)
+    _back(dtau, tau)
+    _back_bool(de, event)
+    _back(dd, delta)
+
+
+def update_tau_e(tau_e, inv_gamma, chi_e, dt, N, to_be_pruned, event, photon_delta, ids=None,
+                 seed=None, step=0, tag=0, slot=None):
+    """qed/optical_depth.py:15-62"""
+    _od("sfp_update_tau_e", tau_e, inv_gamma, chi_e, dt, N, to_be_pruned, event, photon_delta, ids,
+        seed, step, tag, slot)
+
+
+def update_tau_gamma(tau_gamma, inv_gamma, chi_gamma, dt, N, to_be_pruned, event, pair_delta,
+                     ids=None, seed=None, step=0, tag=0, slot=None):
+    """qed/optical_depth.py:64-110"""
+    _od("sfp_update_tau_gamma", tau_gamma, inv_gamma, chi_gamma, dt, N, to_be_pruned, event,
+        pair_delta, ids, seed, step, tag, slot)
