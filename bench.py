@@ -580,17 +580,22 @@ def main():
 
     def errors(path):
         """Yield and spectrum differences of a path to the FP64 table path on
-        the same workload, with the statistical resolution of each (3 sigma
-        of the FP64 count); the gate: every yield within max(1 %, 3 sigma)
-        and the photon spectra's two-sample KS p > 0.01."""
+        the same workload, with the statistical resolution of each: 3 sigma
+        of the difference of two independent Poisson counts,
+        3 sqrt(n64 + n) / n64 (photon ids, and with them the draws of every
+        later photon, part once the two paths' decisions differ, so the pair
+        counts are all but independent samples; for photon counts, which
+        share most draws, this overstates the spread).  The gate: every
+        yield within max(1 %, 3 sigma) and the photon spectra's two-sample
+        KS p > 0.01."""
         from scipy import stats as st
         _, _, c64, s64, _, _ = res["fp64"]
         _, _, c, sp, _, _ = res[path]
         ks = st.ks_2samp(s64, sp)
         e = {"photon_yield": round((c[2] - c64[2]) / max(c64[2], 1), 6),
-             "photon_yield_3sigma": round(3 / np.sqrt(max(c64[2], 1)), 6),
+             "photon_yield_3sigma": round(3 * np.sqrt(c64[2] + c[2]) / max(c64[2], 1), 6),
              "pair_yield": round((c[1] - c64[1]) / max(c64[1], 1), 6),
-             "pair_yield_3sigma": round(3 / np.sqrt(max(c64[1], 1)), 6),
+             "pair_yield_3sigma": round(3 * np.sqrt(c64[1] + c[1]) / max(c64[1], 1), 6),
              "photons_fp64": int(c64[2]), "pairs_fp64": int(c64[1]),
              "photon_spectrum_ks_distance": round(float(ks.statistic), 6),
              "photon_spectrum_ks_p": float(ks.pvalue),

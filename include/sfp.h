@@ -91,7 +91,8 @@ int sfp_device_info(int32_t* num_cus, char* name, int32_t name_len);
 
 /* Device memory helpers for callers without their own HIP allocator (the
  * Python host layer uses these; no counterpart in the reference, whose arrays
- * are numpy/cupy).  Copies are synchronous. */
+ * are numpy/cupy).  Copies are synchronous; sfp_free waits for the device's
+ * queued work before it releases the buffer. */
 int sfp_malloc(void** dev, int64_t bytes);
 int sfp_free(void* dev);
 int sfp_memcpy_htod(void* dst_dev, const void* src_host, int64_t bytes);

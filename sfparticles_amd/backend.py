@@ -41,43 +41,79 @@ def _back_bool(d, a):
         a[...] = tmp.astype(bool)
 
 
+class _Dev:
+    """Device copies of one call's arguments.  Each copy lives as long as this
+    object, i.e. to the end of the wrapper, after the synchronous copies back:
+    a temporary whose pointer alone were passed would be freed before the
+    asynchronous kernel reads it."""
+
+    def __init__(self):
+        self.keep = []
+
+    def f(self, a):
+        d = _f(a)
+        self.keep.append(d)
+        return d.ptr
+
+    def b(self, a):
+        d = _b(a)
+        self.keep.append(d)
+        return d.ptr
+
+    def i64(self, a):
+        d = _dev(a, np.int64)
+        self.keep.append(d)
+        return d.ptr
+
+    def u64(self, a):
+        d = _dev(a, np.uint64)
+        self.keep.append(d)
+        return d.ptr
+
+    def out(self, a, kind="f"):
+        """A device copy of an in/out array, copied back by back()."""
+        d = _f(a) if kind == "f" else _b(a)
+        self.keep.append(d)
+        self._outs = getattr(self, "_outs", []) + [(d, a, kind)]
+        return d.ptr
+
+    def back(self):
+        for d, a, kind in getattr(self, "_outs", []):
+            (_back if kind == "f" else _back_bool)(d, a)
+
+
 def push_position(x, y, z, ux, uy, uz, inv_gamma, N, to_be_pruned, dt):
     """cpu.py:5-19"""
-    dx, dy, dz = _f(x), _f(y), _f(z)
-    L.call("sfp_push_position", dx.ptr, dy.ptr, dz.ptr, _f(ux).ptr, _f(uy).ptr, _f(uz).ptr,
-           _f(inv_gamma).ptr, int(N), _b(to_be_pruned).ptr, float(dt), None)
-    for d, a in ((dx, x), (dy, y), (dz, z)):
-        _back(d, a)
+    D = _Dev()
+    L.call("sfp_push_position", D.out(x), D.out(y), D.out(z), D.f(ux), D.f(uy), D.f(uz),
+           D.f(inv_gamma), int(N), D.b(to_be_pruned), float(dt), None)
+    D.back()
 
 
 def boris(ux, uy, uz, inv_gamma, Ex, Ey, Ez, Bx, By, Bz, q, N, to_be_pruned, dt):
     """cpu.py:22-31"""
-    d = [_f(a) for a in (ux, uy, uz, inv_gamma)]
-    F = [_f(a) for a in (Ex, Ey, Ez, Bx, By, Bz)]
-    L.call("sfp_boris", *[v.ptr for v in d], *[v.ptr for v in F], float(q), int(N),
-           _b(to_be_pruned).ptr, float(dt), None)
-    for v, a in zip(d, (ux, uy, uz, inv_gamma)):
-        _back(v, a)
+    D = _Dev()
+    L.call("sfp_boris", *[D.out(a) for a in (ux, uy, uz, inv_gamma)],
+           *[D.f(a) for a in (Ex, Ey, Ez, Bx, By, Bz)], float(q), int(N), D.b(to_be_pruned),
+           float(dt), None)
+    D.back()
 
 
 def boris_tbmt(ux, uy, uz, inv_gamma, sx, sy, sz, Ex, Ey, Ez, Bx, By, Bz, q, ae, N, to_be_pruned,
                dt):
     """cpu.py:33-42"""
-    arrs = (ux, uy, uz, inv_gamma, sx, sy, sz)
-    d = [_f(a) for a in arrs]
-    F = [_f(a) for a in (Ex, Ey, Ez, Bx, By, Bz)]
-    L.call("sfp_boris_tbmt", *[v.ptr for v in d], *[v.ptr for v in F], float(q), float(ae),
-           int(N), _b(to_be_pruned).ptr, float(dt), None)
-    for v, a in zip(d, arrs):
-        _back(v, a)
+    D = _Dev()
+    L.call("sfp_boris_tbmt", *[D.out(a) for a in (ux, uy, uz, inv_gamma, sx, sy, sz)],
+           *[D.f(a) for a in (Ex, Ey, Ez, Bx, By, Bz)], float(q), float(ae), int(N),
+           D.b(to_be_pruned), float(dt), None)
+    D.back()
 
 
 def _ll(name, ux, uy, uz, inv_gamma, chi_e, N, to_be_pruned, dt):
-    arrs = (ux, uy, uz, inv_gamma)
-    d = [_f(a) for a in arrs]
-    L.call(name, *[v.ptr for v in d], _f(chi_e).ptr, int(N), _b(to_be_pruned).ptr, float(dt), None)
-    for v, a in zip(d, arrs):
-        _back(v, a)
+    D = _Dev()
+    L.call(name, *[D.out(a) for a in (ux, uy, uz, inv_gamma)], D.f(chi_e), int(N),
+           D.b(to_be_pruned), float(dt), None)
+    D.back()
 
 
 def LL_push(ux, uy, uz, inv_gamma, chi_e, N, to_be_pruned, dt):
@@ -92,70 +128,59 @@ def CLL_push(ux, uy, uz, inv_gamma, chi_e, N, to_be_pruned, dt):
 
 def update_chi(Ex, Ey, Ez, Bx, By, Bz, ux, uy, uz, inv_gamma, chi_e, N, to_be_pruned):
     """cpu.py:62-75"""
-    dchi = _f(chi_e)
-    L.call("sfp_update_chi", *[_f(a).ptr for a in (Ex, Ey, Ez, Bx, By, Bz, ux, uy, uz, inv_gamma)],
-           dchi.ptr, int(N), _b(to_be_pruned).ptr, None)
-    _back(dchi, chi_e)
+    D = _Dev()
+    L.call("sfp_update_chi", *[D.f(a) for a in (Ex, Ey, Ez, Bx, By, Bz, ux, uy, uz, inv_gamma)],
+           D.out(chi_e), int(N), D.b(to_be_pruned), None)
+    D.back()
 
 
 def photon_recoil(ux, uy, uz, inv_gamma, event, photon_delta, N, to_be_pruned):
     """cpu.py:78-88"""
-    arrs = (ux, uy, uz, inv_gamma)
-    d = [_f(a) for a in arrs]
-    L.call("sfp_photon_recoil", *[v.ptr for v in d], _b(event).ptr, _f(photon_delta).ptr, int(N),
-           _b(to_be_pruned).ptr, None)
-    for v, a in zip(d, arrs):
-        _back(v, a)
+    D = _Dev()
+    L.call("sfp_photon_recoil", *[D.out(a) for a in (ux, uy, uz, inv_gamma)], D.b(event),
+           D.f(photon_delta), int(N), D.b(to_be_pruned), None)
+    D.back()
 
 
 def pick_hard_photon(event, delta, inv_gamma, threshold, N):
     """cpu.py:227-232"""
-    de = _b(event)
-    L.call("sfp_pick_hard_photon", de.ptr, _f(delta).ptr, _f(inv_gamma).ptr, float(threshold),
-           int(N), None)
-    _back_bool(de, event)
+    D = _Dev()
+    L.call("sfp_pick_hard_photon", D.out(event, "b"), D.f(delta), D.f(inv_gamma),
+           float(threshold), int(N), None)
+    D.back()
 
 
 def create_photon(x_src, y_src, z_src, ux_src, uy_src, uz_src, x_dst, y_dst, z_dst, ux_dst, uy_dst,
                   uz_dst, inv_gamma_dst, photon_to_be_pruned, event_index, photon_delta, N_buffered,
                   N_photon):
     """cpu.py:90-118"""
-    src = [_f(a) for a in (x_src, y_src, z_src, ux_src, uy_src, uz_src)]
-    dst_arrs = (x_dst, y_dst, z_dst, ux_dst, uy_dst, uz_dst, inv_gamma_dst)
-    dst = [_f(a) for a in dst_arrs]
-    dp = _b(photon_to_be_pruned)
-    L.call("sfp_create_photon", *[v.ptr for v in src], *[v.ptr for v in dst], dp.ptr,
-           _dev(event_index, np.int64).ptr, _f(photon_delta).ptr, int(N_buffered), int(N_photon),
-           None, 0, None, None, None)
-    for v, a in zip(dst, dst_arrs):
-        _back(v, a)
-    _back_bool(dp, photon_to_be_pruned)
+    D = _Dev()
+    L.call("sfp_create_photon", *[D.f(a) for a in (x_src, y_src, z_src, ux_src, uy_src, uz_src)],
+           *[D.out(a) for a in (x_dst, y_dst, z_dst, ux_dst, uy_dst, uz_dst, inv_gamma_dst)],
+           D.out(photon_to_be_pruned, "b"), D.i64(event_index), D.f(photon_delta),
+           int(N_buffered), int(N_photon), None, 0, None, None, None)
+    D.back()
 
 
 def create_pair(x_src, y_src, z_src, ux_src, uy_src, uz_src, photon_to_be_pruned, x_dst, y_dst,
                 z_dst, ux_dst, uy_dst, uz_dst, inv_gamma_dst, pair_to_be_pruned, event_index,
                 pair_delta, N_buffered, N_pair, inverse_delta):
     """cpu.py:120-191"""
-    src = [_f(a) for a in (x_src, y_src, z_src, ux_src, uy_src, uz_src)]
-    sp = _b(photon_to_be_pruned)
-    dst_arrs = (x_dst, y_dst, z_dst, ux_dst, uy_dst, uz_dst, inv_gamma_dst)
-    dst = [_f(a) for a in dst_arrs]
-    dp = _b(pair_to_be_pruned)
-    L.call("sfp_create_pair", *[v.ptr for v in src], sp.ptr, *[v.ptr for v in dst], dp.ptr,
-           _dev(event_index, np.int64).ptr, _f(pair_delta).ptr, int(N_buffered), int(N_pair),
-           int(bool(inverse_delta)), None, 0, None, None, None, None, None, None)
-    for v, a in zip(dst, dst_arrs):
-        _back(v, a)
-    _back_bool(dp, pair_to_be_pruned)
-    _back_bool(sp, photon_to_be_pruned)
+    D = _Dev()
+    L.call("sfp_create_pair", *[D.f(a) for a in (x_src, y_src, z_src, ux_src, uy_src, uz_src)],
+           D.out(photon_to_be_pruned, "b"),
+           *[D.out(a) for a in (x_dst, y_dst, z_dst, ux_dst, uy_dst, uz_dst, inv_gamma_dst)],
+           D.out(pair_to_be_pruned, "b"), D.i64(event_index), D.f(pair_delta), int(N_buffered),
+           int(N_pair), int(bool(inverse_delta)), None, 0, None, None, None, None, None, None)
+    D.back()
 
 
 def find_event_index(event, N_event, N):
     """cpu.py:193-216 (ascending indices of the events)"""
-    de = _b(event)
+    D = _Dev()
     out = L.DeviceArray(max(8 * int(N), 8))
     cnt = C.c_int64(0)
-    L.call("sfp_find_event_index", de.ptr, int(N), out.ptr, C.byref(cnt), None)
+    L.call("sfp_find_event_index", D.b(event), int(N), out.ptr, C.byref(cnt), None)
     res = np.zeros(int(N_event), dtype=np.int64)
     n = min(int(cnt.value), int(N_event))
     if n:
@@ -167,8 +192,9 @@ def find_event_index(event, N_event, N):
 
 def bool_sum(event, N):
     """cpu.py:218-225"""
+    D = _Dev()
     cnt = C.c_int64(0)
-    L.call("sfp_bool_sum", _b(event).ptr, int(N), C.byref(cnt), None)
+    L.call("sfp_bool_sum", D.b(event), int(N), C.byref(cnt), None)
     return int(cnt.value)
 
 
@@ -179,20 +205,19 @@ def bool_sum(event, N):
 # with counter (id, step, tag, slot), so the extra keyword arguments name the
 # stream (defaults: ids 0..N-1, the package seed, step 0, tag 0 and the
 # per-operation slot).
-def _qed_stream(N, ids, seed, step, tag, slot):
+def _qed_stream(D, N, ids, seed, step, tag, slot):
     from . import qed
     ids = np.arange(int(N), dtype=np.uint64) if ids is None else np.asarray(ids, np.uint64)
-    return (_dev(ids, np.uint64), qed.seed if seed is None else int(seed), int(step), int(tag),
+    return (D.u64(ids), qed.seed if seed is None else int(seed), int(step), int(tag),
             qed.PER_OP_SLOT if slot is None else int(slot), qed.device_tables())
 
 
 def _rs(name, inv_gamma, chi, dt, N, to_be_pruned, event, delta, ids, seed, step, tag, slot):
-    did, sd, st, tg, sl, tables = _qed_stream(N, ids, seed, step, tag, slot)
-    de, dd = _b(event), _f(delta)
-    L.call(name, _f(inv_gamma).ptr, _f(chi).ptr, float(dt), int(N), _b(to_be_pruned).ptr, de.ptr,
-           dd.ptr, did.ptr, sd, st, tg, sl, tables, None)
-    _back_bool(de, event)
-    _back(dd, delta)
+    D = _Dev()
+    stream = _qed_stream(D, N, ids, seed, step, tag, slot)
+    L.call(name, D.f(inv_gamma), D.f(chi), float(dt), int(N), D.b(to_be_pruned),
+           D.out(event, "b"), D.out(delta), *stream, None)
+    D.back()
 
 
 def photon_from_rejection_sampling(inv_gamma, chi_e, dt, N, to_be_pruned, event, photon_delta,
@@ -210,13 +235,11 @@ def pair_from_rejection_sampling(inv_gamma, chi_gamma, dt, N, to_be_pruned, even
 
 
 def _od(name, tau, inv_gamma, chi, dt, N, to_be_pruned, event, delta, ids, seed, step, tag, slot):
-    did, sd, st, tg, sl, tables = _qed_stream(N, ids, seed, step, tag, slot)
-    dtau, de, dd = _f(tau), _b(event), _f(delta)
-    L.call(name, dtau.ptr, _f(inv_gamma).ptr, _f(chi).ptr, float(dt), int(N), _b(to_be_pruned).ptr,
-           de.ptr, dd.ptr, did.ptr, sd, st, tg, sl, tables, None)
-    _back(dtau, tau)
-    _back_bool(de, event)
-    _back(dd, delta)
+    D = _Dev()
+    stream = _qed_stream(D, N, ids, seed, step, tag, slot)
+    L.call(name, D.out(tau), D.f(inv_gamma), D.f(chi), float(dt), int(N), D.b(to_be_pruned),
+           D.out(event, "b"), D.out(delta), *stream, None)
+    D.back()
 
 
 def update_tau_e(tau_e, inv_gamma, chi_e, dt, N, to_be_pruned, event, photon_delta, ids=None,

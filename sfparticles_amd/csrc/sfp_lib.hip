@@ -1040,6 +1040,9 @@ extern "C" int sfp_malloc(void** dev, int64_t bytes) {
 }
 
 extern "C" int sfp_free(void* dev) {
+  // work already queued may still read the buffer (a per-operation kernel
+  // launched on it): wait for the device before the memory goes back
+  if (dev) HIPCHK(hipDeviceSynchronize());
   if (dev) HIPCHK(hipFree(dev));
   return SFP_OK;
 }

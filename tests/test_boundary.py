@@ -116,16 +116,17 @@ def test_events_recoil_and_counts():
 
     B.photon_recoil(g["ux"], g["uy"], g["uz"], g["inv_gamma"], g["event"], g["delta"], N,
                     g["pruned"])
+    o_pruned = _u8(d0["pruned"])  # held: the oracle reads it through a raw pointer
     L.orc_photon_recoil(ptr(o["ux"]), ptr(o["uy"]), ptr(o["uz"]), ptr(o["inv_gamma"]),
-                        ptr(o_event), ptr(o["delta"]), N, ptr(_u8(d0["pruned"])))
+                        ptr(o_event), ptr(o["delta"]), N, ptr(o_pruned))
     _same(g, o, ("ux", "uy", "uz", "inv_gamma"))
 
+    ev8 = _u8(g["event"])
     for n in (0, 1, 63, 64, 65, 4097, N):
-        ev = g["event"][:n] if n else np.zeros(0, bool)
-        want = int(L.orc_bool_sum(ptr(_u8(ev)), n)) if n else 0
+        want = int(L.orc_bool_sum(ptr(ev8), n)) if n else 0
         assert B.bool_sum(g["event"], n) == want
         idx = np.zeros(max(want, 1), np.int64)
-        cnt = int(L.orc_find_event_index(ptr(_u8(g["event"])), want, n, ptr(idx))) if n else 0
+        cnt = int(L.orc_find_event_index(ptr(ev8), want, n, ptr(idx))) if n else 0
         got = B.find_event_index(g["event"], want, n)
         assert cnt == want and np.array_equal(got, idx[:want])
 
@@ -153,9 +154,9 @@ def test_create_photon_and_pair():
     B.create_photon(*[d0[a] for a in src], *[gd[a] for a in outs], gd["pruned"], idx, d0["delta"],
                     nbuf, n_ev)
     opr = _u8(od["pruned"])
+    ids = np.zeros(cap, np.uint64)
     L.orc_create_photon(*[ptr(d0[a]) for a in src], *[ptr(od[a]) for a in outs], ptr(opr),
-                        ptr(idx), ptr(d0["delta"]), nbuf, n_ev, ptr(np.zeros(cap, np.uint64)), 0,
-                        None, None)
+                        ptr(idx), ptr(d0["delta"]), nbuf, n_ev, ptr(ids), 0, None, None)
     od["pruned"] = opr
     _same(gd, od, outs + ("pruned",))
 
@@ -167,7 +168,7 @@ def test_create_photon_and_pair():
         opr = _u8(od["pruned"])
         L.orc_create_pair(*[ptr(d0[a]) for a in src], ptr(os_), *[ptr(od[a]) for a in outs],
                           ptr(opr), ptr(idx), ptr(d0["delta"]), nbuf, n_ev, inverse,
-                          ptr(np.zeros(cap, np.uint64)), 0, None, None, None, None, None)
+                          ptr(ids), 0, None, None, None, None, None)
         od["pruned"] = opr
         _same(gd, od, outs + ("pruned",))
         assert np.array_equal(gs.astype(np.uint8), os_) and gs[idx].all()

@@ -284,9 +284,9 @@ def test_boris_conserves_u_in_magnetic_field():
     ig = 1 / np.sqrt(1 + ux ** 2 + uy ** 2 + uz ** 2)
     z = _one(0)
     pr = np.array([False])
+    args = (ux, uy, uz, ig, z, z, z, z, z, _one(bz))  # held for the raw pointers
     for _ in range(20):
-        L.orc_boris(*[oracle.ptr(a) for a in (ux, uy, uz, ig, z, z, z, z, z, _one(bz))], -E, 1,
-                    oracle.ptr(pr), T / 20)
+        L.orc_boris(*[oracle.ptr(a) for a in args], -E, 1, oracle.ptr(pr), T / 20)
     assert abs(np.sqrt(ux ** 2 + uy ** 2 + uz ** 2)[0] - u0) / u0 < 1e-10
     assert abs(uz[0] - uz0) < 1e-12  # momentum along B untouched
 
@@ -300,9 +300,9 @@ def test_boris_pure_electric_field_is_linear():
     ig = 1 / np.sqrt(1 + ux ** 2)
     z = _one(0)
     pr = np.array([False])
+    args = (ux, uy, uz, ig, _one(Ex), z, z, z, z, z)  # held for the raw pointers
     for _ in range(50):
-        L.orc_boris(*[oracle.ptr(a) for a in (ux, uy, uz, ig, _one(Ex), z, z, z, z, z)], -E, 1,
-                    oracle.ptr(pr), dt)
+        L.orc_boris(*[oracle.ptr(a) for a in args], -E, 1, oracle.ptr(pr), dt)
     want = 3.0 - E * Ex * 50 * dt / (ME * C)
     assert abs(ux[0] - want) / abs(want) < 1e-13
     assert abs(ig[0] - 1 / np.sqrt(1 + want ** 2)) < 1e-15
@@ -316,7 +316,8 @@ def test_ll_known_answer_and_closed_form():
     ig = 1 / np.sqrt(1 + ux ** 2)
     ig0 = ig[0]
     pr = np.array([False])
-    L.orc_LL_push(*[oracle.ptr(a) for a in (ux, uy, uz, ig, _one(1.0))], 1, oracle.ptr(pr), 2.67e-17)
+    args = (ux, uy, uz, ig, _one(1.0))  # held for the raw pointers
+    L.orc_LL_push(*[oracle.ptr(a) for a in args], 1, oracle.ptr(pr), 2.67e-17)
     assert round((1000.0 - ux[0]) / 1000.0 - 0.1, 2) == 0
     F = -2 / 3 * ALPHA * ME * C ** 2 / HBAR * 2.67e-17
     assert abs(ux[0] - 1000.0 * (1 + F * ig0)) < 1e-9
@@ -335,9 +336,10 @@ def test_cll_quantum_correction():
         igb = iga.copy()
         # dt such that one step changes u by ~1e-3 (well above rounding)
         dt = 1e-3 * 500.0 / (2 / 3 * RATE_UNIT * chi ** 2)
-        L.orc_LL_push(*[oracle.ptr(v) for v in a], oracle.ptr(iga), oracle.ptr(_one(chi)), 1,
+        chi_a = _one(chi)  # held for the raw pointers
+        L.orc_LL_push(*[oracle.ptr(v) for v in a], oracle.ptr(iga), oracle.ptr(chi_a), 1,
                       oracle.ptr(pr), dt)
-        L.orc_CLL_push(*[oracle.ptr(v) for v in b], oracle.ptr(igb), oracle.ptr(_one(chi)), 1,
+        L.orc_CLL_push(*[oracle.ptr(v) for v in b], oracle.ptr(igb), oracle.ptr(chi_a), 1,
                        oracle.ptr(pr), dt)
         assert abs((500.0 - b[0][0]) / (500.0 - a[0][0]) - g) < 1e-9
 
@@ -347,25 +349,26 @@ def test_chi_static_field_construction():
     L = oracle.lib()
     gamma = 1e4
     ux = np.sqrt(gamma ** 2 - 1)
+    # every array the oracle reads is held in a name for the call (a raw
+    # pointer to a temporary would dangle)
+    no = np.array([False])
     for chi_target in (0.1, 1.0, 5.0):
         Bz = chi_target * ME ** 2 * C ** 2 / E / HBAR / ux
         out = _one(0)
         z = _one(0)
-        L.orc_update_chi(*[oracle.ptr(a) for a in (z, z, z, z, z, _one(Bz), _one(ux), z, z,
-                                                     _one(1 / np.sqrt(1 + ux ** 2)), out)],
-                         1, oracle.ptr(np.array([False])))
+        args = (z, z, z, z, z, _one(Bz), _one(ux), z, z, _one(1 / np.sqrt(1 + ux ** 2)), out)
+        L.orc_update_chi(*[oracle.ptr(a) for a in args], 1, oracle.ptr(no))
         assert abs(out[0] / chi_target - 1) < 1e-12
     # a plane wave co-moving with the particle: E + v x B cancels
     Ey = 1e14
     out = _one(0)
     u = 1e3
-    L.orc_update_chi(*[oracle.ptr(a) for a in (z, _one(Ey), z, z, z, _one(Ey / C), _one(u), z, z,
-                                               _one(1 / np.sqrt(1 + u * u)), out)],
-                     1, oracle.ptr(np.array([False])))
+    args = (z, _one(Ey), z, z, z, _one(Ey / C), _one(u), z, z, _one(1 / np.sqrt(1 + u * u)), out)
+    L.orc_update_chi(*[oracle.ptr(a) for a in args], 1, oracle.ptr(no))
     head_on = _one(0)
-    L.orc_update_chi(*[oracle.ptr(a) for a in (z, _one(Ey), z, z, z, _one(Ey / C), _one(-u), z, z,
-                                               _one(1 / np.sqrt(1 + u * u)), head_on)],
-                     1, oracle.ptr(np.array([False])))
+    args = (z, _one(Ey), z, z, z, _one(Ey / C), _one(-u), z, z, _one(1 / np.sqrt(1 + u * u)),
+            head_on)
+    L.orc_update_chi(*[oracle.ptr(a) for a in args], 1, oracle.ptr(no))
     assert out[0] < 1e-3 * head_on[0]
 
 
@@ -386,9 +389,9 @@ def test_tbmt_precession_in_uniform_b():
     ig = 1 / np.sqrt(1 + ux ** 2)
     z = _one(0)
     pr = np.array([False])
+    args = (ux, uy, uz, ig, sx, sy, sz, z, z, z, z, z, _one(Bz))  # held for the raw pointers
     for _ in range(nsteps):
-        L.orc_boris_tbmt(*[oracle.ptr(a) for a in (ux, uy, uz, ig, sx, sy, sz, z, z, z, z, z,
-                                                     _one(Bz))], -E, ae, 1, oracle.ptr(pr), dt)
+        L.orc_boris_tbmt(*[oracle.ptr(a) for a in args], -E, ae, 1, oracle.ptr(pr), dt)
     assert abs(np.sqrt(sx ** 2 + sy ** 2 + sz ** 2)[0] - 1) < 1e-12
     ang_u = np.arctan2(uy[0], ux[0])
     ang_s = np.arctan2(sy[0], sx[0])
