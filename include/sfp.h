@@ -177,6 +177,20 @@ int sfp_find_event_index(const uint8_t* event_dev, int64_t n, int64_t* event_ind
 /* cpu.py:218-225: number of true flags in [0, n) (synchronous) */
 int sfp_bool_sum(const uint8_t* flags_dev, int64_t n, int64_t* count_host, sfp_stream_t stream);
 
+/* No counterpart in the reference: sfp_find_event_index for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same indices into
+ * event_index_dev (room for n entries); the count goes to the DEVICE pointer
+ * count_dev, and the call is asynchronous (no synchronisation, no host
+ * copy).  Its scratch (block counts) is a static device array of 2^16
+ * int64 (512 KiB) that each GPU holds until the process ends, so n is at
+ * most 2^26 (SFP_E_ARG beyond).  Calls that overlap on one GPU share that
+ * array and race; calls in order on one stream do not.  It allocates
+ * nothing: no hipMalloc, hipFree or allocating library routine. */
+#define SFP_NOALLOC_MAX_FLAGS (1ll << 26)
+int sfp_find_event_index_noalloc(const uint8_t* event_dev, int64_t n, int64_t* event_index_dev,
+                                 int64_t* count_dev, sfp_stream_t stream);
+
 /* 1/gamma from the momentum, [0, n) skipping pruned: 1/sqrt(1+u^2) for
  * leptons (photon == 0), 1/|u| for photons (particles.py:118-123).  No
  * counterpart in the reference, which stores inv_gamma; the fused step keeps

@@ -12,6 +12,8 @@
 // elementary functions (sfp_math.h), so every FP64 result is bit-identical
 // to the CPU restatement used for validation.
 
+#include <mutex>
+
 #include "sfp_internal.h"
 
 // ----------------------------------------------------------------- errors --
@@ -498,6 +500,60 @@ extern "C" int sfp_find_event_index(const uint8_t* event, int64_t n, int64_t* ev
 extern "C" int sfp_bool_sum(const uint8_t* flags, int64_t n, int64_t* count_host,
                             sfp_stream_t stream) {
   return sfp_compact_indices(flags, n, 0, nullptr, count_host, S(stream));
+}
+
+// ---- the same compaction without allocation --------------------------------
+// For callers that cannot allocate (code under graph capture, tight serving
+// loops): the block counts live in `scratch`, a static device array that each
+// GPU holds for the life of the process.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current.  Calls that overlap on one GPU share the array and race.
+static constexpr int64_t NOALLOC_MAX_BLOCKS = 1 << 16;  // flags up to 2^16 * CB = 2^26
+__device__ __align__(256) int64_t scratch[NOALLOC_MAX_BLOCKS];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static int64_t* g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_of_current_device(int64_t** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return set_err(SFP_E_ARG, "device %d beyond the %d supported", dev, NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (int64_t*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return SFP_OK;
+}
+
+extern "C" int sfp_find_event_index_noalloc(const uint8_t* event, int64_t n,
+                                            int64_t* event_index, int64_t* count_dev,
+                                            sfp_stream_t stream) {
+  if (n < 0) return set_err(SFP_E_ARG, "n must be >= 0");
+  if (n > NOALLOC_MAX_BLOCKS * CB)
+    return set_err(SFP_E_ARG, "%lld flags: at most %lld without allocation", (long long)n,
+                   (long long)(NOALLOC_MAX_BLOCKS * CB));
+  if (!count_dev) return set_err(SFP_E_ARG, "count_dev must be a device pointer");
+  int64_t* counts = nullptr;
+  CHK(scratch_of_current_device(&counts));
+  hipStream_t st = S(stream);
+  const int64_t nb = (n + CB - 1) / CB;
+  if (nb == 0) {  // no flags: the count is 0 (a kernel, not a memset, so that capture works)
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, (int64_t)0, count_dev);
+  } else {
+    hipLaunchKernelGGL(k_block_count, dim3((unsigned)nb), dim3(256), 0, st, event, n, 0, counts);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, nb, count_dev);
+    if (event_index)
+      hipLaunchKernelGGL(k_scatter_idx, dim3((unsigned)nb), dim3(CB), 0, st, event, n, 0, counts,
+                         event_index);
+  }
+  HIPCHK(hipGetLastError());
+  return SFP_OK;
 }
 
 // ------------------------------------------------------ QED per-op kernels --
