@@ -73,6 +73,20 @@ def _species_list(case, make):
 def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None,
                  grids=None, reserve=None, **sim_kw):
     """Run a case through sfparticles_amd.Simulation; returns per-species arrays."""
+    sim, sp = build_gpu_case(case, precision=precision, sampler=sampler, grids=grids,
+                             reserve=reserve, **sim_kw)
+    steps = case["nsteps"] if nsteps is None else nsteps
+    per = steps // chunks
+    done = 0
+    for k in range(chunks):
+        m = per if k < chunks - 1 else steps - done
+        sim.start(m)
+        done += m
+    return _collect(sp), sim
+
+
+def build_gpu_case(case, precision="fp64", sampler=None, grids=None, reserve=None, **sim_kw):
+    """The species and Simulation of a case, not yet run: (sim, [species])."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
     from sfparticles_amd.fields import Fields
@@ -110,14 +124,7 @@ def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sa
         s._ensure_device()
         if cap > s.buffer_size:
             s._extend(cap - s.N_buffered)
-    steps = case["nsteps"] if nsteps is None else nsteps
-    per = steps // chunks
-    done = 0
-    for k in range(chunks):
-        m = per if k < chunks - 1 else steps - done
-        sim.start(m)
-        done += m
-    return _collect(sp), sim
+    return sim, sp
 
 
 def _collect(sp):
@@ -135,17 +142,25 @@ def _collect(sp):
     return out
 
 
-def run_oracle_case(case, nsteps=None, chunks=1, libm=False, spin_emission=False):
+def run_oracle_case(case, nsteps=None, chunks=1, libm=False, spin_emission=False, subset=None):
+    """Run a case through the oracle.  subset: indices of the seed electrons
+    to run (with their ids; particles created get ids from the case's full
+    seed count on), e.g. to replay part of a large GPU run."""
     import oracle
     from oracle.sim import OParticles, OSimulation
     from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
 
     od = case["qed"] == "od"
-    n = case["props"][0].shape[0]
+    n_all = case["props"][0].shape[0]
+    pick = (lambda a: a.copy()) if subset is None else (lambda a: np.ascontiguousarray(a[subset]))
+    n = n_all if subset is None else len(subset)
     rr = case["rr"]
-    e = OParticles("electron", -1, 1, N=n, props=[a.copy() for a in case["props"]], RR=rr,
-                   spin=case["spin"], spin_props=[a.copy() for a in case["spin_props"]]
+    e = OParticles("electron", -1, 1, N=n, props=[pick(a) for a in case["props"]], RR=rr,
+                   spin=case["spin"], spin_props=[pick(a) for a in case["spin_props"]]
                    if case["spin"] else None)
+    if subset is not None:
+        e.id[:] = np.asarray(subset, dtype=np.uint64)
+        e.next_id = n_all
     sp = [e]
     p = g = None
     if "p" in case["species"]:
