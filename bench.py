@@ -88,6 +88,20 @@ def kernel_bytes(photon, phase, st, od=False):
     return 60 * hot + 48 * hot + tau + 57 * st["spawned"]
 
 
+# The survey's algorithmic bytes per particle-step (the persistent state read
+# once and written once, id and status byte; transients in registers): e+-
+# 105 B in FP64, 81 B in mixed precision (FP64 x, FP32 u); photons 81 B; and
+# 57 B per appended particle.  They price the kernel that does a species'
+# whole step: the lepton QED kernel on dense steps (it visits every lepton by
+# index, the sweep stands down) and the photon kernel over the awake photons.
+def survey_bytes(photon, phase, st, precision, steps):
+    if photon and phase == 1:
+        return 81 * st["processed"] + 57 * 2 * st["spawned"]
+    if not photon and phase == 1 and st.get("dense_steps", 0) == steps and not st.get("emit"):
+        return (105 if precision == "fp64" else 81) * st["hot"] + 57 * st["spawned"]
+    return None
+
+
 def make_case(n, nsteps, seed=0, field="gaussian"):
     rng = np.random.Generator(np.random.PCG64(seed))
     gamma = rng.normal(1957.0, 0.05 * 1957.0, n)
@@ -225,11 +239,12 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
     L.call("sfp_synchronize")
     t1 = time.perf_counter()
     stats = [sim.stats(i) for i in range(3)]
+    qd = [sim.qed_diag(i) for i in range(3)]
     ktime = [[sim.kernel_time(i, ph) for ph in (0, 1, 2)] for i in range(3)]
     spectrum = photon_energy_sample(species[2])
     counts = [s.N_buffered for s in species]
     del sim, species
-    return t1 - t0, stats, ktime, counts, spectrum, start, base
+    return t1 - t0, stats, ktime, counts, spectrum, start, base, qd
 
 
 def cpu_baseline(base, start, steps, every):
@@ -522,7 +537,7 @@ def main():
     cpu_every = 0 if args.no_cpu else max(1, int(args.cpu_every))
 
     def measure(precision):
-        elapsed, stats, ktime, counts, spectrum, start, base = gpu_run(
+        elapsed, stats, ktime, counts, spectrum, start, base, qd = gpu_run(
             n, args.steps, args.warmup, nsteps_config, precision, args.chunk, args.timing_every,
             args.field, cpu_every if precision == "fp64" else 0)
         # the kernels of every timing_every-th step are timed (HIP events on
@@ -535,7 +550,8 @@ def main():
         dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)),
                       key=lambda ip: ktime[ip[0]][ip[1]][0])
         ms = ktime[dom][ph][0] * scale
-        per_launch = kernel_bytes(dom == 2, ph, stats[dom]) / args.steps
+        sb = survey_bytes(dom == 2, ph, stats[dom], precision, args.steps)
+        per_launch = (sb if sb is not None else kernel_bytes(dom == 2, ph, stats[dom])) / args.steps
         avg_s = ms / 1e3 / args.steps
         achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
         fk = "gaussian" if args.field == "gaussian" else "general"
@@ -556,6 +572,8 @@ def main():
                     "traffic": None,
                     "kernel": kname, "avg_launch_ms": round(ms / args.steps, 4),
                     "alg_bytes_per_launch": int(per_launch),
+                    "alg_bytes": "survey per-particle-step bytes" if sb is not None
+                    else "kernel's own state bytes (list, re-reads)",
                     "kernel_time_share": round(ms / max(1e-9, total_ms), 3),
                     "timed_steps": timed,
                     "whole_step": {"alg_bytes_per_step": int(step_bytes / args.steps),
@@ -570,6 +588,8 @@ def main():
                   "photon_steps_asleep": asleep,
                   "qed_listed": {names[i]: stats[i]["hot"] for i in range(3)},
                   "photon_multistep_pushes": stats[2]["sleeps"],
+                  "electron_dense_steps": stats[0]["dense_steps"],
+                  "qed_diagnostics": {names[i]: qd[i] for i in range(3)},
                   "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0] * scale, 2)
                                 for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}

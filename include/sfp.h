@@ -428,6 +428,34 @@ int sfp_sim_set_rate_net(void* sim, void* mlp);
  *      at their first step): the first one, and those that do not continue
  *      the previous call or follow a change of a store from the host */
 int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
+/* The same counters and more, n of them (at most SFP_SIM_NSTATS):
+ * [0]-[14] as sfp_sim_stats, [1] counting particles read by the QED kernel
+ *      on dense steps
+ * [15] dense steps: steps on which the QED kernel visited every lepton by
+ *      index, the sweep standing down (taken when the last step's culling
+ *      bound listed at least 7/8 of the particles; results do not change) */
+#define SFP_SIM_NSTATS 16
+int sfp_sim_stats_n(void* sim, int32_t species, int64_t* stats, int32_t n);
+/* Diagnostics of the exact QED evaluations of one species since the last
+ * reset (synchronous; no counterpart in the reference, whose behaviour in
+ * these cases is undefined or unreported):
+ *  counts[0] rejection-sampling evaluations whose acceptance probability p
+ *            exceeded 1 (a step samples at most one event, as in
+ *            qed/rejection_sampling.py:78-79,96-97, so such steps
+ *            under-sample)
+ *  counts[1] photon-emission evaluations with z beyond the table's end (rate
+ *            taken as 0; the reference reads past the table,
+ *            qed/rejection_sampling_tables.py:40-51)
+ *  counts[2] optical-depth evaluations with chi above the table (clamped to
+ *            its top node; the reference reads past it,
+ *            qed/optical_depth_tables.py:39-40)
+ *  counts[3] evaluations with chi NaN (E + v x B cancelled below rounding;
+ *            no event)
+ *  *p_max    the largest p evaluated (learned sampler: the step's event
+ *            probability W dtau).
+ * Particles the culling bound proves event-free are not evaluated, and a
+ * photon ruled out by its own draw has p < 1/2. */
+int sfp_sim_qed_diag(void* sim, int32_t species, int64_t* counts, double* p_max);
 int sfp_sim_reset_stats(void* sim);
 
 /* ---------------------------------------------------------------------------

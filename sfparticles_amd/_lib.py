@@ -105,6 +105,8 @@ SIGNATURES = {
     "sfp_find_event_index": [P, I64, P, C.POINTER(I64), P],
     "sfp_bool_sum": [P, I64, C.POINTER(I64), P],
     "sfp_find_event_index_noalloc": [P, I64, P, P, P],
+    "sfp_sim_stats_n": [P, I32, P, I32],
+    "sfp_sim_qed_diag": [P, I32, P, C.POINTER(D)],
     "sfp_photon_from_rejection_sampling": [P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],
     "sfp_pair_from_rejection_sampling": [P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],
     "sfp_update_tau_e": [P, P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],

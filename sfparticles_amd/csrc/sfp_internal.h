@@ -86,6 +86,20 @@ struct Counts {
   int64_t n_emit;     // emission candidates handed to the emission kernel this step
   int64_t emit_total; // sum of n_emit over steps
   int64_t cpar;       // photons: which of the two window caches holds this window
+  // leptons, dense steps: 1 when this step's QED kernel visits every particle
+  // itself (no sweep list); per step parity, the particles a step's bound
+  // listed, which decides the next step's mode; dense steps run
+  int64_t dense;
+  int64_t dense_listed[2];
+  int64_t dense_steps;
+  // exact QED evaluations (diagnostics): rejection-sampling probabilities
+  // above 1, photon-emission z beyond the table, optical-depth chi above the
+  // table, NaN chi; the largest probability as the bits of a positive float
+  int64_t p_over1;
+  int64_t z_clamp;
+  int64_t chi_clamp;
+  int64_t chi_nan;
+  int64_t p_max_bits;
 };
 
 extern const size_t kAttrSize[SFP_NATTR];

@@ -28,6 +28,8 @@
 // step from the halted kernel.  No worst-case reservation and no per-step
 // host synchronisation are needed.
 
+#include <cstring>
+
 #include "sfp_internal.h"
 #include "sfp_mlp.h"
 
@@ -596,6 +598,29 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
   __shared__ uint64_t sh_pref;
 
   const int64_t n_old = sp.cnt->n_old;
+  if (!PHOTON && QED) {
+    // Dense step: when the last step's bound listed at least 7/8 of the
+    // particles, this kernel stands down and the QED kernel visits every
+    // particle by index -- bound, exact QED where the bound allows an event,
+    // push -- reading the store once instead of here and again through the
+    // list.  A particle the bound culls is pushed there exactly as here, so
+    // results do not change.  The list length reported is the store's
+    // count: the room the QED kernel asks for, and what the capacity
+    // recovery grows the destination by.  Every workgroup reads the same
+    // counts (written by earlier launches) and takes the same decision.
+    const uint32_t par = (uint32_t)(a.step & 1u);
+    const int64_t prev = sp.cnt->dense_listed[par ^ 1u];
+    if (sp.push && n_old > 0 && prev * 8 >= n_old * 7) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sp.cnt->n_hot = n_old;
+        sp.cnt->hot_total += n_old;
+        sp.cnt->dense = 1;
+        sp.cnt->dense_listed[par] = 0;  // the QED kernel adds this step's listed particles
+        sp.cnt->dense_steps += 1;
+      }
+      return;
+    }
+  }
   // Rows per tile: ROWS once the store fills a tile per workgroup of the
   // grid; fewer (a power of two) for shorter stores, which then still spread
   // over many workgroups.  The list order does not depend on the tiling.
@@ -802,6 +827,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
       uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][0][ntiles - 1]) & FLAG_VAL_MASK) : 0;
       sp.cnt->n_hot = (int64_t)total;
       sp.cnt->hot_total += (int64_t)total;
+      if (!PHOTON) {
+        sp.cnt->dense = 0;
+        sp.cnt->dense_listed[a.step & 1u] = (int64_t)total;
+      }
       scan->epoch = epoch;
     } else {
       sp.cnt->n_hot = 0;
@@ -827,6 +856,48 @@ __device__ inline bool room_or_halt(const Plan& P, int s, const StepArgs& a, con
   return true;
 }
 
+// Diagnostics of the exact QED evaluations one thread makes, summed over the
+// workgroup at the end of a launch (qdiag_flush): rejection-sampling
+// probabilities above 1 (one step samples at most one event, as the
+// reference's qed/rejection_sampling.py:78-79,96-97 do), photon-emission
+// z beyond the table (rate 0 here; the reference reads past the table's end,
+// qed/rejection_sampling_tables.py:40-51), optical-depth chi above the table
+// (clamped to its top node; the reference reads past it,
+// qed/optical_depth_tables.py:39-40), NaN chi (no event), the largest
+// probability.
+struct QDiag {
+  Counts* c;
+  float pmax = 0.0f;
+  __device__ explicit QDiag(Counts* cnt) : c(cnt) {}
+  // a counter of the lanes (active at the call) for which `cond` holds: one
+  // atomic per wave, and no register kept across the launch
+  __device__ static void tally(int64_t* ctr, bool cond) {
+    const uint64_t m = __ballot(cond);
+    if (m && wave_lane() == __builtin_ctzll(m))
+      atomicAdd((unsigned long long*)ctr, (unsigned long long)__popcll(m));
+  }
+  __device__ void prob(double p) {
+    tally(&c->p_over1, p > 1.0);
+    pmax = fmaxf(pmax, (float)p);
+  }
+  // photon emission: z = (delta / ((1 - delta) chi))^(2/3) > z_max
+  __device__ void zcheck(const Plan& P, double chi, double delta) {
+    tally(&c->z_clamp, delta > P.zmax15 * chi * (1.0 - delta));
+  }
+  __device__ void nan_chi(double chi) { tally(&c->chi_nan, chi != chi); }
+  __device__ void clamp_chi(bool over) { tally(&c->chi_clamp, over); }
+};
+
+// The largest probability of the workgroup into the species' counter (the
+// counts go straight to it as they happen).  Called by every thread.
+__device__ inline void qdiag_flush(const QDiag& d) {
+  float m = d.pmax;
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  // positive floats order as their bits
+  if (wave_lane() == 0 && m > 0.0f)
+    atomicMax((unsigned long long*)&d.c->p_max_bits, (unsigned long long)__float_as_uint(m));
+}
+
 // Exact QED decision of one particle at time t (simulation.py:103-114):
 // field, chi, then the rejection-sampling draw (qed/rejection_sampling.py:68-101)
 // or the optical-depth update (qed/optical_depth.py:15-110), then the
@@ -837,12 +908,13 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
                                  const sfp_rs_node* lds_rs, const double2* lds_od, double x,
                                  double y, double z, double ux, double uy, double uz, double ig,
                                  uint64_t pid, double& tau, double& delta, double sx = 0.0,
-                                 double sy = 0.0, double sz = 0.0) {
+                                 double sy = 0.0, double sz = 0.0, QDiag* dg = nullptr) {
   bool event = false;
   delta = 0.0;
   double F[6];
   sfp_field_eval<FM>(FS, x, y, z, a.t, F);
   double chi = sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], ux, uy, uz, ig);
+  if (dg) dg->nan_chi(chi);
   if (KIND == KIND_RS) {
     if (chi != 0.0) {
       sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
@@ -859,15 +931,21 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
           w = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1_3, 1);
         }
         double p = 3 * sfp_powi(r1, 2) * w * dtau;
+        if (dg) {
+          dg->prob(p);
+          dg->zcheck(P, chi, r1_3);
+        }
         if (r2 < p) { event = true; delta = r1_3; }
       } else {
         double p = sfp_rs_rate(lds_rs, P.z_N, P.z_min, P.z_max, P.z_delta, chi, r1, 0) * dtau;
+        if (dg) dg->prob(p);
         if (r2 < p) { event = true; delta = r1; }
       }
     }
   } else {
     // below chi_min (or NaN): skipped, tau unchanged
     if (chi >= P.od.chi_min) {
+      if (dg) dg->clamp_chi(chi > P.od.chi_max);
       event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
                         PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
     }
@@ -923,12 +1001,13 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
                                    const sfp_rs_node32* lds_rs, const double2* lds_od, double x,
                                    double y, double z, float ux, float uy, float uz, float ig,
                                    uint64_t pid, double& tau, double& delta, float sx = 0.0f,
-                                   float sy = 0.0f, float sz = 0.0f) {
+                                   float sy = 0.0f, float sz = 0.0f, QDiag* dg = nullptr) {
   bool event = false;
   delta = 0.0;
   float F[6];
   sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = PHOTON ? sfp32_chi_photon(F, ux, uy, uz) : sfp32_chi(F, ux, uy, uz);
+  if (dg) dg->nan_chi(chi);
   if (KIND == KIND_RS) {
     if (chi != 0.0) {
       sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
@@ -947,15 +1026,21 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
           w = sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1_3, true);
         }
         const float p = 3.0f * (r1 * r1) * w * dtau;
+        if (dg) {
+          dg->prob(p);
+          dg->zcheck(P, chi, r1_3);
+        }
         if (r2 < p) { event = true; delta = r1_3; }
       } else {
         const float p =
             sfp32_rs_rate(lds_rs, (int)P.z_N, zmin, zmax, zd, izd, chif, r1, false) * dtau;
+        if (dg) dg->prob(p);
         if (r2 < p) { event = true; delta = r1; }
       }
     }
   } else {
     if (chi >= P.od.chi_min) {
+      if (dg) dg->clamp_chi(chi > P.od.chi_max);
       event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag,
                           lds_od, PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
     }
@@ -987,7 +1072,7 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
                                        double x, double y, double z, float ux, float uy, float uz,
                                        float ig, uint64_t pid, double& tau, double& delta,
                                        bool& net, float& in0, float& in1, double& chi_out,
-                                       double& u_table) {
+                                       double& u_table, QDiag* dg = nullptr) {
   bool event = false;
   delta = 0.0;
   net = false;
@@ -995,6 +1080,7 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
   sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = sfp32_chi(F, ux, uy, uz);
   chi_out = chi;
+  if (dg) dg->nan_chi(chi);
   const sfp_mlp_bins& nb = *P.net_bins;
   float u = 0.0f;     // uniform that selects delta, on the network's from-zero scale
   double od_u = 0.0;  // optical depth: the uniform of its own inverse CDF
@@ -1010,23 +1096,30 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
                                     (float)P.z_delta, (float)(1.0 / P.z_delta), (float)chi, r1_3,
                                     true) *
                       dtau;
+      if (dg) {
+        dg->prob(p);
+        dg->zcheck(P, chi, r1_3);
+      }
       if (r2 < p) { event = true; delta = r1_3; }
       return event;
     }
     float lc = __log10f((float)chi);
     const float lmax = (float)P.od.log_chi_max, lmin = (float)P.od.log_chi_min;
+    if (dg) dg->clamp_chi(lc > lmax);
     if (lc > lmax) lc = lmax;
     const float ld = (float)P.od.log_chi_delta;
     int idx = (int)((lc - lmin) / ld);
     if (idx > (int)P.od.chi_N - 2) idx = (int)P.od.chi_N - 2;
     const double2 nd = lds_od[idx];
     const float W = (float)nd.x + (float)nd.y * (lc - (lmin + (float)idx * ld));
+    if (dg) dg->prob(W * dtau);  // composition: the event probability of the step
     if (!(r2 < W * dtau)) return false;
     event = true;
     u = r1;
     u_table = (double)r1;
   } else {
     if (!(chi >= P.od.chi_min)) return false;
+    if (dg) dg->clamp_chi(chi > P.od.chi_max);
     event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, lds_od,
                         P.od.photon_cdf, P.od, &delta);
     if (!event) return false;
@@ -1070,19 +1163,21 @@ __device__ inline void qed_rate32_prep(const Plan& P, const sfp_fieldset* FS, co
                                        double y, double z, float ux, float uy, float uz, float ig,
                                        uint64_t pid, float sx, float sy, float sz, bool& net,
                                        float& in0, float& in1, float& in2, float& r1_3, float& r2,
-                                       float& dtau, float& p_table) {
+                                       float& dtau, float& p_table, QDiag* dg = nullptr) {
   net = false;
   p_table = 0.0f;
   r2 = 1.0f;
   float F[6];
   sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = sfp32_chi(F, ux, uy, uz);
+  if (dg) dg->nan_chi(chi);
   if (chi == 0.0) return;
   sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
   const float r1 = sfp32_u01(d.v[0]);
   r2 = sfp32_u01(d.v[2]);
   r1_3 = r1 * r1 * r1;
   dtau = (float)a.dt * ig;
+  if (dg) dg->zcheck(P, chi, r1_3);
   const float sig = sfp32_spin_sigma(sp.q, ux, uy, uz, ig, F, sx, sy, sz);
   const sfp_mlp_bins& nb = *P.rate_bins;
   const float lc = __log10f((float)chi);
@@ -1217,7 +1312,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   constexpr bool NET = NETK == 1;
   __shared__ double2 lds_od[(KIND == KIND_OD || NET) ? 255 : 1];
   __shared__ int64_t sh_tile;
-  __shared__ uint32_t sh_w[2 * WAVES];
+  __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[NETK == 2 ? 1 : QROWS][WAVES];
   __shared__ uint32_t sh_q[NETK == 2 ? 1 : QROWS * TILE];
@@ -1237,8 +1332,13 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   const int64_t idA = dA ? dA->cnt->next_id : 0;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
+  // dense step (set by this step's sweep): the list is the identity over the
+  // store, holes included; the culling bound is taken here
+  const bool dense = sp.cnt->dense != 0;
+  const bool spin_rate = P.spin_emission && sp.spin;
 
-  uint32_t alive = 0;
+  uint32_t alive = 0, proc = 0, listed = 0;
+  QDiag dg(sp.cnt);
   bool staged = false;
   if constexpr (NETK != 2) {
   const int64_t SPAN = span;
@@ -1268,19 +1368,22 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
     // software pipeline over the rows: the state of the next row and the
     // list index of the one after are loaded while this row computes
     const int64_t k0 = tile * SPAN + threadIdx.x;
-    uint32_t i_cur = k0 < n_hot ? sp.hot[k0] : 0u;
-    uint32_t i_next = (rows > 1 && k0 + TILE < n_hot) ? sp.hot[k0 + TILE] : 0u;
+    uint32_t i_cur = k0 < n_hot ? (dense ? (uint32_t)k0 : sp.hot[k0]) : 0u;
+    uint32_t i_next = (rows > 1 && k0 + TILE < n_hot)
+                          ? (dense ? (uint32_t)(k0 + TILE) : sp.hot[k0 + TILE]) : 0u;
     double nx = 0, ny = 0, nz = 0, nux = 0, nuy = 0, nuz = 0;
     uint64_t npid = 0;
+    uint8_t npr = 0;
     if (k0 < n_hot) {
       nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
       nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
       npid = sp.id[i_cur];
+      if (dense) npr = sp.pruned[i_cur];
     }
 #pragma unroll 1
     for (int row = 0; row < rows; ++row) {
       const int64_t k = tile * SPAN + (int64_t)row * TILE + threadIdx.x;
-      const bool valid = k < n_hot;
+      const bool valid = k < n_hot && npr == 0;
       const int64_t i = valid ? (int64_t)i_cur : 0;
       double x = nx, y = ny, z = nz, ux = nux, uy = nuy, uz = nuz, tau = 0, sx = 0, sy = 0, sz = 0;
       uint64_t pid = npid;
@@ -1295,7 +1398,9 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
         nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
         npid = sp.id[i_cur];
-        i_next = (row + 2 < rows && k + 2 * TILE < n_hot) ? sp.hot[k + 2 * TILE] : 0u;
+        npr = dense ? sp.pruned[i_cur] : (uint8_t)0;
+        i_next = (row + 2 < rows && k + 2 * TILE < n_hot)
+                     ? (dense ? (uint32_t)(k + 2 * TILE) : sp.hot[k + 2 * TILE]) : 0u;
       }
       uint32_t shot = 0;
       const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
@@ -1309,23 +1414,34 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
           ig = sfp_inv_gamma_lepton(ux, uy, uz);
         }
       }
+      // dense step: the sweep's culling bound, here (a culled particle is
+      // pushed below exactly as the sweep pushes it)
+      bool need = valid;
+      if (dense && valid) {
+        const double igd = MIX ? (double)igf : ig;
+        const float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
+        const float chi_b = (float)P.chi_coef * S * (float)(1.0 / igd) * 1.001f;
+        need = qed_possible(P, false, chi_b, igd, a.dt, spin_rate);
+        proc += 1;
+        listed += need ? 1u : 0u;
+      }
       bool ev = false;
       double delta = 0.0;
-      if (valid) {
+      if (need) {
         if (NET) {
           bool net = false;
           float in0 = 0.0f, in1 = 0.0f;
           double chi_n = 0.0, u_tab = 0.0;
           ev = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
                                          fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
-                                         u_tab);
+                                         u_tab, &dg);
         } else if (MIX) {
           ev = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
                                             z, fx, fy, fz, igf, pid, tau, delta, (float)sx,
-                                            (float)sy, (float)sz);
+                                            (float)sy, (float)sz, &dg);
         } else {
           ev = qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
-                                          ux, uy, uz, ig, pid, tau, delta, sx, sy, sz);
+                                          ux, uy, uz, ig, pid, tau, delta, sx, sy, sz, &dg);
         }
         if (!NET && !dA) ev = false;
       }
@@ -1343,7 +1459,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
         sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
         if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
-        if (KIND == KIND_OD) sp.tau[i] = tau;
+        if (KIND == KIND_OD && need) sp.tau[i] = tau;
         alive += 1;
       }
     }
@@ -1360,7 +1476,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t k = tile * SPAN + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
-      const int64_t i = (int64_t)sp.hot[k];
+      const int64_t i = dense ? k : (int64_t)sp.hot[k];
       double x = sp.x[i], y = sp.y[i], z = sp.z[i];
       double ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
       double tau = KIND == KIND_OD ? sp.tau[i] : 0.0, sx = 0, sy = 0, sz = 0;
@@ -1454,12 +1570,15 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
     }
 
     const int64_t k = tile * TILE + threadIdx.x;
-    const bool valid = k < n_hot;
     int64_t i = 0;
+    bool valid = false;
+    if (k < n_hot) {
+      i = dense ? k : (int64_t)sp.hot[k];
+      valid = !dense || sp.pruned[i] == 0;
+    }
     double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0, sx = 0, sy = 0, sz = 0;
     uint64_t pid = 0;
     if (valid) {
-      i = (int64_t)sp.hot[k];
       x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
       ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
       if (KIND == KIND_OD) tau = sp.tau[i];
@@ -1478,6 +1597,15 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         ig = sfp_inv_gamma_lepton(ux, uy, uz);
       }
     }
+    bool need = valid;
+    if (dense && valid) {
+      const double igd = MIX ? (double)igf : ig;
+      const float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
+      const float chi_b = (float)P.chi_coef * S * (float)(1.0 / igd) * 1.001f;
+      need = qed_possible(P, false, chi_b, igd, a.dt, spin_rate);
+      proc += 1;
+      listed += need ? 1u : 0u;
+    }
     bool event = false;
     double delta = 0.0;
     if (NET) {
@@ -1488,10 +1616,10 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
       bool net = false, cand = false;
       float in0 = 0.0f, in1 = 0.0f;
       double chi_n = 0.0, u_tab = 0.0;
-      if (valid)
+      if (need)
         cand = qed_event32_net<KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
                                          fx, fy, fz, igf, pid, tau, delta, net, in0, in1, chi_n,
-                                         u_tab);
+                                         u_tab, &dg);
       const int64_t q = tile_rank(cand, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
       if (cand) {
         EmitRec r;
@@ -1510,33 +1638,34 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
         sp.ux[i] = fx; sp.uy[i] = fy; sp.uz[i] = fz;
         if (sp.spin) { sp.sx[i] = s0; sp.sy[i] = s1; sp.sz[i] = s2; }
-        if (KIND == KIND_OD) sp.tau[i] = tau;
+        if (KIND == KIND_OD && need) sp.tau[i] = tau;
         alive += 1;
       }
       continue;
     } else if (NETK == 2) {
       bool net = false;
       float in0 = 0.0f, in1 = 0.0f, in2 = 0.0f, r1_3 = 0.0f, r2 = 1.0f, dtau = 0.0f, p = 0.0f;
-      if (valid)
+      if (need)
         qed_rate32_prep<FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, x, y, z, fx, fy, fz, igf,
                             pid, (float)sx, (float)sy, (float)sz, net, in0, in1, in2, r1_3, r2,
-                            dtau, p);
+                            dtau, p, &dg);
       const float o = net_rate_wave(P, net, in0, in1, in2);
       if (net)
         p = sfp_rate_from_net(P.rate_net, o) * (float)P.rate_unit * dtau;
+      if (need) dg.prob(p);
       if (valid && r2 < p) {
         event = true;
         delta = r1_3;
         if ((float)delta / igf < (float)P.threshold) event = false;  // cpu.py:227-232
       }
-    } else if (valid) {
+    } else if (need) {
       if (MIX) {
         event = qed_event32<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
                                              z, fx, fy, fz, igf, pid, tau, delta, (float)sx,
-                                             (float)sy, (float)sz);
+                                             (float)sy, (float)sz, &dg);
       } else {
         event = qed_event<false, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
-                                           ux, uy, uz, ig, pid, tau, delta, sx, sy, sz);
+                                           ux, uy, uz, ig, pid, tau, delta, sx, sy, sz, &dg);
       }
     }
     if (!dA) event = false;
@@ -1578,16 +1707,22 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
       sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
       sp.ux[i] = ux; sp.uy[i] = uy; sp.uz[i] = uz;
       if (sp.spin) { sp.sx[i] = sx; sp.sy[i] = sy; sp.sz[i] = sz; }
-      if (KIND == KIND_OD) sp.tau[i] = tau;
+      if (KIND == KIND_OD && need) sp.tau[i] = tau;
       alive += 1;
     }
   }
   }
 
-  uint32_t cnt[1] = {alive};
-  uint64_t tot[1];
+  uint32_t cnt[3] = {alive, proc, listed};
+  uint64_t tot[3];
   block_sum(cnt, sh_w, tot);
-  if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
+  if (threadIdx.x == 0) {
+    count_add(&sp.cnt->psteps, tot[0]);
+    count_add(&sp.cnt->processed, tot[1]);
+    // the particles this step's bound lists decide the next step's mode
+    count_add(&sp.cnt->dense_listed[a.step & 1u], tot[2]);
+  }
+  qdiag_flush(dg);
   kernel_exit(scan, nwg, [&]() {
     if (NET) {
       const uint64_t c = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
@@ -1934,6 +2069,7 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
   const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), past the bound (id read), multi-step pushes
+  QDiag dg(sp.cnt);
   bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
@@ -2021,10 +2157,11 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
         if (MIX) {
           event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x,
                                               y, z, (float)ux, (float)uy, (float)uz, (float)ig,
-                                              sp.id[i], tau, delta);
+                                              sp.id[i], tau, delta, 0.0f, 0.0f, 0.0f, &dg);
         } else {
           event = qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y,
-                                            z, ux, uy, uz, ig, sp.id[i], tau, delta);
+                                            z, ux, uy, uz, ig, sp.id[i], tau, delta, 0.0, 0.0, 0.0,
+                                            &dg);
         }
       }
       if (!dA && !dB) event = false;
@@ -2135,6 +2272,7 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
     count_add(&sp.cnt->sleeps, tot[3]);
     count_add(&sp.cnt->asleep_n, tot[3]);
   }
+  qdiag_flush(dg);
   kernel_exit(scan, nwg, [&]() {
     finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
     scan->epoch = epoch;
@@ -3240,6 +3378,33 @@ extern "C" int sfp_sim_stats(void* p, int32_t species, int64_t* stats) {
   return SFP_OK;
 }
 
+extern "C" int sfp_sim_stats_n(void* p, int32_t species, int64_t* stats, int32_t n) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
+  if (n < 0) return set_err(SFP_E_ARG, "n must be >= 0");
+  int64_t v[SFP_SIM_NSTATS];
+  CHK(sfp_sim_stats(p, species, v));
+  v[15] = sim->counts_host[species].dense_steps;
+  for (int32_t k = 0; k < n && k < SFP_SIM_NSTATS; ++k) stats[k] = v[k];
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_qed_diag(void* p, int32_t species, int64_t* counts, double* p_max) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
+  CHK(sim_read_counts(sim));
+  const Counts& c = sim->counts_host[species];
+  counts[0] = c.p_over1;
+  counts[1] = c.z_clamp;
+  counts[2] = c.chi_clamp;
+  counts[3] = c.chi_nan;
+  uint32_t bits = (uint32_t)c.p_max_bits;
+  float f;
+  memcpy(&f, &bits, sizeof(f));
+  *p_max = (double)f;
+  return SFP_OK;
+}
+
 extern "C" int sfp_sim_reset_stats(void* p) {
   Sim* sim = (Sim*)p;
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
@@ -3247,6 +3412,7 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   for (int s = 0; s < sim->nsp; ++s) {
     Counts c = sim->counts_host[s];
     c.psteps = c.processed = c.spawned = c.hot_total = c.slept = c.sleeps = c.emit_total = 0;
+    c.dense_steps = c.p_over1 = c.z_clamp = c.chi_clamp = c.chi_nan = c.p_max_bits = 0;
     HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   }
   sim->syncs = 0;

@@ -250,15 +250,28 @@ class Simulation(object):
 
     def stats(self, species: int) -> dict:
         """Counters of one species since the last reset: particle-steps (live
-        particles after each step), particles read by its step kernel,
-        particles that kernel appended, current buffered count."""
-        v = (C.c_int64 * 15)()
-        L.call("sfp_sim_stats", self._sim, int(species), v)
+        particles after each step), particles read by its step kernels,
+        particles they appended, current buffered count, QED-listed
+        particles, and the run's host synchronisations, capacity replays,
+        photon sleep counts, graph and tuner batches, dense steps."""
+        v = (C.c_int64 * 16)()
+        L.call("sfp_sim_stats_n", self._sim, int(species), v, 16)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
                     hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
                     slept=int(v[8]), sleeps=int(v[9]), graph_batches=int(v[10]),
                     captures=int(v[11]), tune_batches=int(v[12]), emit=int(v[13]),
-                    clock_restarts=int(v[14]))
+                    clock_restarts=int(v[14]), dense_steps=int(v[15]))
+
+    def qed_diag(self, species: int) -> dict:
+        """Diagnostics of the exact QED evaluations of a species since the last
+        reset: rejection-sampling probabilities above 1 and the largest one,
+        photon-emission z beyond the table, optical-depth chi above the
+        table, NaN chi (no counterpart in the reference)."""
+        v = (C.c_int64 * 4)()
+        pm = C.c_double(0)
+        L.call("sfp_sim_qed_diag", self._sim, int(species), v, C.byref(pm))
+        return dict(p_over1=int(v[0]), z_clamp=int(v[1]), chi_clamp=int(v[2]), chi_nan=int(v[3]),
+                    p_max=float(pm.value))
 
     def reset_stats(self):
         L.call("sfp_sim_reset_stats", self._sim)
