@@ -379,6 +379,13 @@ int sfp_sim_reset_timing(void* sim);
  * rebuild period samples every phase of it) keeps the timed run close to
  * an untimed one.  No counterpart in the reference (instrumentation). */
 int sfp_sim_set_timing_every(void* sim, int64_t every);
+/* The stable ranks of appended particles come from a decoupled look-back
+ * over per-tile flag words tagged with a 29-bit launch epoch.  The library
+ * zeroes the flags and epochs between batches every `steps` steps (default
+ * 2^28, at most 2^28), so that an epoch never wraps onto a stale flag; a
+ * smaller interval only exercises that clearing (tests).  No counterpart in
+ * the reference. */
+int sfp_sim_set_flag_clear_every(void* sim, int64_t steps);
 /* Ensemble of independent shots (no counterpart in the reference): shot s has
  * its own field, the nterms[s] terms starting at terms + sum_{j<s} nterms[j].
  * Every particle of a species with a SFP_SHOT attribute moves in the field of

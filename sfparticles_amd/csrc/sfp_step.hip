@@ -178,12 +178,19 @@ __device__ inline void st_flag(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// flag word: [epoch:24][status:2][value:38]; status 1 = aggregate, 2 = inclusive prefix
-static constexpr int FLAG_VAL_BITS = 38;
+// flag word: [epoch:29][status:2][value:33]; status 1 = aggregate, 2 = inclusive
+// prefix.  Values are counts of at most a store's capacity (< 2^32).  The
+// epoch advances once per launch of a scanning kernel and is never 0; the
+// host clears the flags and epochs long before it could wrap (sim_clear_flags),
+// so a flag of an earlier launch can never carry the current epoch.
+static constexpr int FLAG_VAL_BITS = 33;
+static constexpr int FLAG_EPOCH_SHIFT = FLAG_VAL_BITS + 2;
+static constexpr uint32_t FLAG_EPOCH_MASK = (1u << (64 - FLAG_EPOCH_SHIFT)) - 1u;
 static constexpr uint64_t FLAG_VAL_MASK = (1ull << FLAG_VAL_BITS) - 1ull;
 
 __device__ inline uint64_t flag_word(uint32_t epoch, uint32_t status, uint64_t v) {
-  return ((uint64_t)epoch << 40) | ((uint64_t)status << FLAG_VAL_BITS) | (v & FLAG_VAL_MASK);
+  return ((uint64_t)epoch << FLAG_EPOCH_SHIFT) | ((uint64_t)status << FLAG_VAL_BITS) |
+         (v & FLAG_VAL_MASK);
 }
 
 // Decoupled look-back (one full wave): publish the tile aggregate, accumulate
@@ -204,7 +211,7 @@ __device__ uint64_t lookback(uint64_t* flags, int64_t tile, uint64_t agg, uint32
     uint32_t st;
     while (true) {
       w = (idx < 0) ? flag_word(epoch, 2, 0) : ld_flag(&flags[idx]);
-      st = ((uint32_t)(w >> 40) == epoch) ? (uint32_t)((w >> FLAG_VAL_BITS) & 3u) : 0u;
+      st = ((uint32_t)(w >> FLAG_EPOCH_SHIFT) == epoch) ? (uint32_t)((w >> FLAG_VAL_BITS) & 3u) : 0u;
       if (__all(st != 0)) break;
       __builtin_amdgcn_s_sleep(1);
     }
@@ -508,7 +515,7 @@ __device__ inline void count_add(int64_t* dst, uint64_t v) {
 }
 
 __device__ inline uint32_t next_epoch(const ScanState* scan) {
-  uint32_t e = (scan->epoch + 1u) & 0xffffffu;
+  uint32_t e = (scan->epoch + 1u) & FLAG_EPOCH_MASK;
   return e == 0 ? 1u : e;
 }
 
@@ -2469,6 +2476,10 @@ struct Sim {
   int cus = 0;                       // compute units of the device
   bool net = false;                  // leptons use the learned photon-energy sampler
   uint64_t rec_end = 0;              // recorder: one past the last step run
+  // steps run since the look-back flags and epochs were last cleared; each
+  // step advances an epoch by at most one
+  int64_t steps_since_clear = 0;
+  int64_t clear_every = (int64_t)1 << 28;  // well inside the 29-bit epoch
   bool rate_net = false;             // spin leptons use the learned emission rate
   sfp_fieldset* shot_fields = nullptr;  // device, one per shot
   int32_t nshots = 0;
@@ -3111,6 +3122,29 @@ static int sim_settle(Sim* sim) {
 
 static int sim_settle_cb(void* p) { return sim_settle((Sim*)p); }
 
+// Zero every look-back flag and scan state (epochs back to 0) between
+// batches, with nothing in flight (a synchronous fill, never inside a
+// captured graph), before any epoch could wrap onto a stale flag.
+static int sim_clear_flags(Sim* sim) {
+  HIPCHK(hipStreamSynchronize(sim->stream));
+  for (int s = 0; s < sim->nsp; ++s)
+    for (int ph = 0; ph < NPH; ++ph)
+      if (sim->flags[ph][s])
+        HIPCHK(hipMemset(sim->flags[ph][s], 0, sizeof(uint64_t) * sim->flags_cap[ph][s]));
+  HIPCHK(hipMemset(sim->scan_mem, 0, sizeof(ScanState) * NPH * sim->nsp));
+  sim->steps_since_clear = 0;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_flag_clear_every(void* p, int64_t steps) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  if (steps < 1 || steps > ((int64_t)1 << 28))
+    return set_err(SFP_E_ARG, "flag clear interval must be in [1, 2^28] steps");
+  sim->clear_every = steps;
+  return SFP_OK;
+}
+
 extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {
   Sim* sim = (Sim*)p;
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
@@ -3182,6 +3216,8 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   double tt = *t;
   while (k < nsteps) {
     const int64_t batch = std::min<int64_t>(SYNC_EVERY, nsteps - k);
+    if (sim->steps_since_clear + batch > sim->clear_every) CHK(sim_clear_flags(sim));
+    sim->steps_since_clear += batch;
     // the previous batch has been waited for: the staging slots are free
     for (int64_t j = 0; j < batch; ++j) {
       StepArgs& a = sim->args_host[j];

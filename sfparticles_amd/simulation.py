@@ -240,6 +240,13 @@ class Simulation(object):
         self._prepare()
         L.call("sfp_sim_set_grid", self._sim, int(species), int(phase), int(grid))
 
+    def set_flag_clear_every(self, steps: int):
+        """Clear the look-back flags between batches every `steps` steps (the
+        library does so every 2^28 steps, before the 29-bit launch epoch
+        could wrap; a small interval exercises the clearing)."""
+        self._prepare()
+        L.call("sfp_sim_set_flag_clear_every", self._sim, int(steps))
+
     def launch_shape(self, species: int, phase: int):
         """(grid, workgroup size, number of candidate grids) of a kernel."""
         self._prepare()

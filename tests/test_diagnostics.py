@@ -120,3 +120,20 @@ def test_dense_steps_in_the_collision():
     print("electron stats", st)
     assert st["dense_steps"] > 50
     compare_final(gpu, run_oracle_case(case))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_lookback_flags_cleared_between_batches(graph):
+    """The look-back flags and epochs cleared before every batch (the
+    library does it every 2^28 steps, before the 29-bit epoch could wrap):
+    ranks, ids and states stay the oracle's, eagerly and in graph replays."""
+    from tests.cases import head_on_case, build_gpu_case, run_oracle_case, compare_final, _collect
+    case = head_on_case(n=3000, nsteps=240, seed=11)
+    sim, sp = build_gpu_case(case, graph=graph)
+    sim.set_flag_clear_every(1)
+    for m in (7, 64, 64, 41):  # uneven run calls; full batches replay as graphs
+        sim.run_steps(m)
+    sim.start(64)  # the last call through start(), which prunes as the oracle's does
+    if graph:
+        assert sim.stats(0)["graph_batches"] >= 3
+    compare_final(_collect(sp), run_oracle_case(case))
