@@ -343,7 +343,9 @@ SFP_HD double sfp_repeat_add(double x, double h, uint32_t n) {
     }
     const double u = sfp_u2d((uint64_t)(be - 52) << 52);
     const double H = neg ? -h : h;
-    const double r = H / u;  // exact: division by a power of two
+    // r = H / u, as a product with the exact reciprocal 2^(1075 - be) of the
+    // power of two u (the same rounding, no division sequence)
+    const double r = H * sfp_u2d((uint64_t)(2098 - be) << 52);
     if (!(r < 4503599627370496.0 && r > -4503599627370496.0)) {
       x = x + h;
       --n;

@@ -436,7 +436,7 @@ __device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const S
 // Rejection sampling: the bound on p itself (< 0: no bound, evaluate; 0: no
 // event possible).
 __device__ inline double rs_pbound(const Plan& P, bool photon, float chi_b, double ig, double dt,
-                                   bool spin) {
+                                   bool spin, const double2* env_lds = nullptr) {
   float num;
   if (photon) {
     num = 4.0f;
@@ -453,14 +453,18 @@ __device__ inline double rs_pbound(const Plan& P, bool photon, float chi_b, doub
   if (!(fj >= 1.0f)) return -1.0;                       // first bracket: 2/z unbounded
   int64_t j = (int64_t)fj;
   if (j > P.z_N - 2) return 0.0;
-  double2 e = spin ? P.rs_env_spin[j] : P.rs_env[j];
+  // from the workgroup's LDS copy where it has one: a global load issued
+  // after the next row's state would wait for those loads (they complete in
+  // order) and undo the software pipeline
+  double2 e = env_lds ? env_lds[j] : (spin ? P.rs_env_spin[j] : P.rs_env[j]);
   return P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
 }
 
 __device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
-                                    double dt, bool spin = false) {
+                                    double dt, bool spin = false,
+                                    const double2* env_lds = nullptr) {
   if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
-  const double pb = rs_pbound(P, photon, chi_b, ig, dt, spin);
+  const double pb = rs_pbound(P, photon, chi_b, ig, dt, spin, env_lds);
   return pb < 0.0 || !(2.0 * pb < 0x1p-53);
 }
 
@@ -863,46 +867,63 @@ __device__ inline bool room_or_halt(const Plan& P, int s, const StepArgs& a, con
   return true;
 }
 
-// Diagnostics of the exact QED evaluations one thread makes, summed over the
-// workgroup at the end of a launch (qdiag_flush): rejection-sampling
+// Diagnostics of the exact QED evaluations a workgroup makes, counted in LDS
+// (one LDS atomic per wave and occurrence; no register held across the
+// launch, no contention on a global counter) and added to the species'
+// counters at the end of the launch (qdiag_flush): rejection-sampling
 // probabilities above 1 (one step samples at most one event, as the
-// reference's qed/rejection_sampling.py:78-79,96-97 do), photon-emission
-// z beyond the table (rate 0 here; the reference reads past the table's end,
+// reference's qed/rejection_sampling.py:78-79,96-97 do), photon-emission z
+// beyond the table (rate 0 here; the reference reads past the table's end,
 // qed/rejection_sampling_tables.py:40-51), optical-depth chi above the table
 // (clamped to its top node; the reference reads past it,
 // qed/optical_depth_tables.py:39-40), NaN chi (no event), the largest
-// probability.
+// probability (float bits), and the evaluations in which an event was
+// possible at all (p >= 2^-54, or chi >= chi_min under optical depth: the
+// lepton QED kernel's dense steps decide the next step's mode on it).
+enum { QD_P1 = 0, QD_ZC, QD_CC, QD_NAN, QD_PMAX, QD_LIVE, QD_N };
 struct QDiag {
-  Counts* c;
-  float pmax = 0.0f;
-  __device__ explicit QDiag(Counts* cnt) : c(cnt) {}
-  // a counter of the lanes (active at the call) for which `cond` holds: one
-  // atomic per wave, and no register kept across the launch
-  __device__ static void tally(int64_t* ctr, bool cond) {
+  uint32_t* sh;  // QD_N LDS words of the workgroup
+  __device__ static void tally(uint32_t* w, bool cond) {
     const uint64_t m = __ballot(cond);
-    if (m && wave_lane() == __builtin_ctzll(m))
-      atomicAdd((unsigned long long*)ctr, (unsigned long long)__popcll(m));
+    if (m && wave_lane() == __builtin_ctzll(m)) atomicAdd(w, (uint32_t)__popcll(m));
   }
   __device__ void prob(double p) {
-    tally(&c->p_over1, p > 1.0);
-    pmax = fmaxf(pmax, (float)p);
+    tally(&sh[QD_P1], p > 1.0);
+    tally(&sh[QD_LIVE], p >= 0x1p-54);
+    const float f = (float)p;
+    if (f > 0.0f) atomicMax(&sh[QD_PMAX], __float_as_uint(f));  // positive floats order as bits
   }
   // photon emission: z = (delta / ((1 - delta) chi))^(2/3) > z_max
   __device__ void zcheck(const Plan& P, double chi, double delta) {
-    tally(&c->z_clamp, delta > P.zmax15 * chi * (1.0 - delta));
+    tally(&sh[QD_ZC], delta > P.zmax15 * chi * (1.0 - delta));
   }
-  __device__ void nan_chi(double chi) { tally(&c->chi_nan, chi != chi); }
-  __device__ void clamp_chi(bool over) { tally(&c->chi_clamp, over); }
+  __device__ void nan_chi(double chi) { tally(&sh[QD_NAN], chi != chi); }
+  // optical depth, chi at or above chi_min: an event is possible
+  __device__ void od(bool over) {
+    tally(&sh[QD_CC], over);
+    tally(&sh[QD_LIVE], true);
+  }
+  __device__ void clamp(bool over) { tally(&sh[QD_CC], over); }
 };
 
-// The largest probability of the workgroup into the species' counter (the
-// counts go straight to it as they happen).  Called by every thread.
-__device__ inline void qdiag_flush(const QDiag& d) {
-  float m = d.pmax;
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  // positive floats order as their bits
-  if (wave_lane() == 0 && m > 0.0f)
-    atomicMax((unsigned long long*)&d.c->p_max_bits, (unsigned long long)__float_as_uint(m));
+// Zero the workgroup's counters (before the first barrier of the launch).
+__device__ inline void qdiag_init(uint32_t* sh) {
+  if (threadIdx.x < QD_N) sh[threadIdx.x] = 0;
+}
+
+// Add the workgroup's counters to the species' (after all its evaluations);
+// returns the evaluations with a possible event.  Called by every thread.
+__device__ inline uint32_t qdiag_flush(Counts* c, uint32_t* sh) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    count_add(&c->p_over1, sh[QD_P1]);
+    count_add(&c->z_clamp, sh[QD_ZC]);
+    count_add(&c->chi_clamp, sh[QD_CC]);
+    count_add(&c->chi_nan, sh[QD_NAN]);
+    if (sh[QD_PMAX])
+      atomicMax((unsigned long long*)&c->p_max_bits, (unsigned long long)sh[QD_PMAX]);
+  }
+  return sh[QD_LIVE];
 }
 
 // Exact QED decision of one particle at time t (simulation.py:103-114):
@@ -952,7 +973,7 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
   } else {
     // below chi_min (or NaN): skipped, tau unchanged
     if (chi >= P.od.chi_min) {
-      if (dg) dg->clamp_chi(chi > P.od.chi_max);
+      if (dg) dg->od(chi > P.od.chi_max);
       event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
                         PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
     }
@@ -1047,7 +1068,7 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
     }
   } else {
     if (chi >= P.od.chi_min) {
-      if (dg) dg->clamp_chi(chi > P.od.chi_max);
+      if (dg) dg->od(chi > P.od.chi_max);
       event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag,
                           lds_od, PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
     }
@@ -1112,7 +1133,7 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
     }
     float lc = __log10f((float)chi);
     const float lmax = (float)P.od.log_chi_max, lmin = (float)P.od.log_chi_min;
-    if (dg) dg->clamp_chi(lc > lmax);
+    if (dg) dg->clamp(lc > lmax);
     if (lc > lmax) lc = lmax;
     const float ld = (float)P.od.log_chi_delta;
     int idx = (int)((lc - lmin) / ld);
@@ -1126,7 +1147,7 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
     u_table = (double)r1;
   } else {
     if (!(chi >= P.od.chi_min)) return false;
-    if (dg) dg->clamp_chi(chi > P.od.chi_max);
+    if (dg) dg->od(chi > P.od.chi_max);
     event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, lds_od,
                         P.od.photon_cdf, P.od, &delta);
     if (!event) return false;
@@ -1323,6 +1344,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[NETK == 2 ? 1 : QROWS][WAVES];
   __shared__ uint32_t sh_q[NETK == 2 ? 1 : QROWS * TILE];
+  __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_hot = sp.cnt->n_hot;
   // tiles of up to QROWS rows, or of one row for the rate network
@@ -1340,12 +1362,14 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
   // dense step (set by this step's sweep): the list is the identity over the
-  // store, holes included; the culling bound is taken here
+  // store, holes included, and every particle gets the exact evaluation (a
+  // particle the sweep's bound would have culled gives "no event" there, and
+  // is pushed as the sweep pushes it)
   const bool dense = sp.cnt->dense != 0;
-  const bool spin_rate = P.spin_emission && sp.spin;
 
-  uint32_t alive = 0, proc = 0, listed = 0;
-  QDiag dg(sp.cnt);
+  uint32_t alive = 0, proc = 0;
+  qdiag_init(sh_qd);
+  QDiag dg{sh_qd};
   bool staged = false;
   if constexpr (NETK != 2) {
   const int64_t SPAN = span;
@@ -1366,6 +1390,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
         __syncthreads();
       }
+
       staged = true;
     }
 
@@ -1421,17 +1446,8 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
           ig = sfp_inv_gamma_lepton(ux, uy, uz);
         }
       }
-      // dense step: the sweep's culling bound, here (a culled particle is
-      // pushed below exactly as the sweep pushes it)
-      bool need = valid;
-      if (dense && valid) {
-        const double igd = MIX ? (double)igf : ig;
-        const float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
-        const float chi_b = (float)P.chi_coef * S * (float)(1.0 / igd) * 1.001f;
-        need = qed_possible(P, false, chi_b, igd, a.dt, spin_rate);
-        proc += 1;
-        listed += need ? 1u : 0u;
-      }
+      const bool need = valid;
+      if (dense && valid) proc += 1;
       bool ev = false;
       double delta = 0.0;
       if (need) {
@@ -1573,6 +1589,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         for (int64_t k = threadIdx.x; k < nn && k < 255; k += TILE) lds_od[k] = P.od_photon_nodes[k];
         __syncthreads();
       }
+
       staged = true;
     }
 
@@ -1604,15 +1621,8 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         ig = sfp_inv_gamma_lepton(ux, uy, uz);
       }
     }
-    bool need = valid;
-    if (dense && valid) {
-      const double igd = MIX ? (double)igf : ig;
-      const float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
-      const float chi_b = (float)P.chi_coef * S * (float)(1.0 / igd) * 1.001f;
-      need = qed_possible(P, false, chi_b, igd, a.dt, spin_rate);
-      proc += 1;
-      listed += need ? 1u : 0u;
-    }
+    const bool need = valid;
+    if (dense && valid) proc += 1;
     bool event = false;
     double delta = 0.0;
     if (NET) {
@@ -1720,16 +1730,17 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   }
   }
 
-  uint32_t cnt[3] = {alive, proc, listed};
-  uint64_t tot[3];
+  const uint32_t live = qdiag_flush(sp.cnt, sh_qd);
+  uint32_t cnt[2] = {alive, proc};
+  uint64_t tot[2];
   block_sum(cnt, sh_w, tot);
   if (threadIdx.x == 0) {
     count_add(&sp.cnt->psteps, tot[0]);
     count_add(&sp.cnt->processed, tot[1]);
-    // the particles this step's bound lists decide the next step's mode
-    count_add(&sp.cnt->dense_listed[a.step & 1u], tot[2]);
+    // dense step: the particles with a possible event decide the next
+    // step's mode (the sweep counts its list otherwise)
+    if (dense) count_add(&sp.cnt->dense_listed[a.step & 1u], live);
   }
-  qdiag_flush(dg);
   kernel_exit(scan, nwg, [&]() {
     if (NET) {
       const uint64_t c = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
@@ -2023,15 +2034,14 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   });
 }
 
-// Occupancy: the compiler's own register count (168 VGPRs, 3 waves per SIMD,
-// for the mixed-precision Gaussian-field instance) ran faster than a cap at
-// 128 VGPRs (4 waves), which spilled.  FP64 with general fields: a 2-wave
-// floor, as for k_hot (photon kernel 0.26 -> 0.22 s per 3000 steps of a
-// 1e6-electron focused-pulse collision).  Mixed precision with optical
-// depth: a 3-wave floor (168 VGPRs; the compiler chose 202, 2 waves) ran the
-// photon kernel of a 4e6-electron collision in 3.65 s instead of 3.90.
+// Occupancy: 3 waves per SIMD in mixed precision (168 VGPRs; the compiler's
+// own choice for some instances is 2), which ran faster than a cap at 128
+// VGPRs (4 waves, spilling) and, for optical depth, than the compiler's 2
+// waves (a 4e6-electron collision: photon kernel 3.90 -> 3.65 s); 2 waves in
+// FP64 (a 1e6-electron focused-pulse collision: 0.26 -> 0.22 s per 3000
+// steps against 1 wave).
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX && KIND == KIND_OD) ? 3 : 1)) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -2040,11 +2050,13 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
 
   __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
   __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
+  __shared__ double2 lds_env[KIND == KIND_RS ? 1023 : 1];  // rate envelopes of the bound
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[QROWS][WAVES];
   __shared__ uint32_t sh_q[QROWS * TILE];
+  __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_list = sp.cnt->n_hot;
   const int rows = qed_rows(n_list);
@@ -2076,7 +2088,8 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
   const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), past the bound (id read), multi-step pushes
-  QDiag dg(sp.cnt);
+  qdiag_init(sh_qd);
+  QDiag dg{sh_qd};
   bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
@@ -2086,6 +2099,10 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
     if (tile >= ntiles) break;
     if (!staged) {  // tables go to LDS once the workgroup has work
       stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
+      if (KIND == KIND_RS) {
+        for (int64_t k = threadIdx.x; k < P.z_N - 1 && k < 1023; k += TILE) lds_env[k] = P.rs_env[k];
+        __syncthreads();
+      }
       staged = true;
     }
 
@@ -2142,7 +2159,7 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
         float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
         float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
         if (KIND == KIND_RS) {
-          pb = rs_pbound(P, true, chi_b, ig, a.dt, false);
+          pb = rs_pbound(P, true, chi_b, ig, a.dt, false, lds_env);
           hot = pb < 0.0 || !(2.0 * pb < 0x1p-53);
         } else {
           hot = qed_possible(P, true, chi_b, ig, a.dt);
@@ -2279,7 +2296,7 @@ __global__ void __launch_bounds__(TILE, (!MIX && FM == SFP_FM_ANY) ? 2 : ((MIX &
     count_add(&sp.cnt->sleeps, tot[3]);
     count_add(&sp.cnt->asleep_n, tot[3]);
   }
-  qdiag_flush(dg);
+  (void)qdiag_flush(sp.cnt, sh_qd);
   kernel_exit(scan, nwg, [&]() {
     finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
     scan->epoch = epoch;
@@ -3234,13 +3251,19 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       a.pad = 0;
       tt = tt + dt;
     }
+    const bool tuning = autotune && sim_needs_tune(sim);
+    const bool replay = !tuning && use_graph && batch == SYNC_EVERY;
+    // a capture starts on an idle stream, before this batch's arguments are
+    // queued (the graph reads them from the slots when it runs)
+    if (replay && (sim->graph_stale || !sim->graph)) {
+      HIPCHK(hipStreamSynchronize(sim->stream));
+      CHK(sim_capture(sim));
+    }
     HIPCHK(hipMemcpyAsync(sim->args_dev, sim->args_host, sizeof(StepArgs) * batch,
                           hipMemcpyHostToDevice, sim->stream));
-    const bool tuning = autotune && sim_needs_tune(sim);
     if (tuning) {
       for (int64_t j = 0; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, (int)j));
-    } else if (use_graph && batch == SYNC_EVERY) {
-      if (sim->graph_stale || !sim->graph) CHK(sim_capture(sim));
+    } else if (replay) {
       HIPCHK(hipGraphLaunch(sim->graph, sim->stream));
       sim->graph_batches++;
     } else {
