@@ -2115,6 +2115,9 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       bool live = false;
       const int64_t i = k < n_list ? (int64_t)i_next : 0;
       i_next = (row + 1 < rows && k + TILE < n_list) ? sp.hot[k + TILE] : 0u;
+      // the id (the key of the photon's draws) loads with the state, not
+      // after the bound: most visited photons lie past it and draw
+      const uint64_t pid = k < n_list ? sp.id[i] : 0;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
       const bool cached = k < sp.ccap;
       bool woke = false;  // asleep until this step
@@ -2126,9 +2129,14 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
         // into this window's cache by the list kernel (oldk set)
         const bool moved = rebuild && cached && sp.oldk[k] != WAKE_NEVER;
         if (cached && (!rebuild || moved)) {
-          if (moved || C.wake(k) <= a.wrel) {
-            x = C.x(k); y = C.y(k); z = C.z(k);
-            ux = C.ux(k); uy = C.uy(k); uz = C.uz(k);
+          // the cached state loads with the wake step, not after it (an
+          // entry asleep since the rebuild is read and dropped)
+          const uint32_t wk = C.wake(k);
+          const double cx = C.x(k), cy = C.y(k), cz = C.z(k);
+          const double cux = C.ux(k), cuy = C.uy(k), cuz = C.uz(k);
+          if (moved || wk <= a.wrel) {
+            x = cx; y = cy; z = cz;
+            ux = cux; uy = cuy; uz = cuz;
             live = true;
             stale = moved;
           }
@@ -2172,7 +2180,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       // it out without the exact field, chi and rate.
       bool exact = hot;
       if (KIND == KIND_RS && hot && pb >= 0.0) {
-        const sfp_u32x4 d = sfp_draw4(P.seed, sp.id[i], a.step, (uint32_t)sp.tag, 0u);
+        const sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
         const double r2 = MIX ? (double)sfp32_u01(d.v[2]) : sfp_u01(d.v[2], d.v[3]);
         exact = r2 < 2.0 * pb;
       }
@@ -2181,10 +2189,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
         if (MIX) {
           event = qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x,
                                               y, z, (float)ux, (float)uy, (float)uz, (float)ig,
-                                              sp.id[i], tau, delta, 0.0f, 0.0f, 0.0f, &dg);
+                                              pid, tau, delta, 0.0f, 0.0f, 0.0f, &dg);
         } else {
           event = qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y,
-                                            z, ux, uy, uz, ig, sp.id[i], tau, delta, 0.0, 0.0, 0.0,
+                                            z, ux, uy, uz, ig, pid, tau, delta, 0.0, 0.0, 0.0,
                                             &dg);
         }
       }

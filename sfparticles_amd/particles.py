@@ -64,66 +64,44 @@ class Particles(object):
 
     def __init__(self, name: str, q: int, m: float, N: int = 0, props: Tuple = None,
                  RR: RadiationReactionType = RadiationReactionType.PHOTON, push=True) -> None:
-        object.__setattr__(self, "_h", {})
-        object.__setattr__(self, "_store", None)
-        object.__setattr__(self, "_cache", {})
-        object.__setattr__(self, "_version", 0)
-        object.__setattr__(self, "_ig_stale", False)
-        self.name = name
-        self.q = q * e
-        self.m = m * m_e
-        self.RR = RR
-        self.push = push
-        self.bw = False
-        self.radiating = False
-        self.spin = False
-        self.ae = 0.0
-        self.tag = 0
-        self._op_calls = 0
+        for k, v in (("_h", {}), ("_store", None), ("_cache", {}), ("_version", 0),
+                     ("_ig_stale", False)):
+            object.__setattr__(self, k, v)
         N = int(N)
-        self.attrs = []
+        # invalid arguments raise AssertionError, as the reference's tests expect
+        assert m >= 0, f"mass must be >= 0, got {m}"
+        assert m != 0 or q == 0, "a massless species must be neutral"
+        assert isinstance(RR, RadiationReactionType), \
+            f"RR must be a RadiationReactionType, got {RR!r}"
+        self.name, self.q, self.m, self.RR, self.push = name, q * e, m * m_e, RR, push
+        self.bw = self.radiating = self.spin = False
+        self.ae, self.tag, self._op_calls = 0.0, 0, 0
 
-        assert m >= 0, "negative mass"
-        if m == 0:
-            assert q == 0, "photons cannot have mass"
-        assert isinstance(RR, RadiationReactionType), "RR must be RadiationReactionType"
-
-        if props is None:
-            if m == 0:
-                assert N == 0, "cannot initialize photons with only N without props."
-            self.x = np.zeros(N)
-            self.y = np.zeros(N)
-            self.z = np.zeros(N)
-            self.ux = np.zeros(N)
-            self.uy = np.zeros(N)
-            self.uz = np.zeros(N)
+        # state: six arrays of length N, zero unless given
         if props:
-            assert len(props) == 6, "given properties must has length of 6"
-            self.x, self.y, self.z, self.ux, self.uy, self.uz = prepare_props(props, N)
-        self.attrs += ["x", "y", "z", "ux", "uy", "uz"]
-
-        if m > 0:
-            self.inv_gamma = 1. / np.sqrt(1 + self.ux ** 2 + self.uy ** 2 + self.uz ** 2)
+            assert len(props) == 6, f"props holds x, y, z, ux, uy, uz: 6 arrays, got {len(props)}"
+            state = prepare_props(props, N)
         else:
-            assert (self.ux ** 2 + self.uy ** 2 + self.uz ** 2 > 0).all(), "photon momentum cannot be 0"
-            self.inv_gamma = 1. / np.sqrt(self.ux ** 2 + self.uy ** 2 + self.uz ** 2)
-        self.attrs += ["inv_gamma"]
-        self.chi = np.zeros(N)
-        self.attrs += ["chi"]
-        self.Ez = np.zeros(N)
-        self.Ex = np.zeros(N)
-        self.Ey = np.zeros(N)
-        self.Bz = np.zeros(N)
-        self.Bx = np.zeros(N)
-        self.By = np.zeros(N)
-        self.attrs += ["Ex", "Ey", "Ez", "Bx", "By", "Bz"]
-        self.N_buffered = N
-        self.buffer_size = N
-        self._to_be_pruned = np.full(N, False)
-        self.attrs += ["_to_be_pruned"]
+            assert m != 0 or N == 0, "photons need their momenta: give props, not only N"
+            state = [np.zeros(N) for _ in range(6)]
+        self.attrs = []
+        for attr, arr in zip(("x", "y", "z", "ux", "uy", "uz"), state):
+            setattr(self, attr, arr)
+            self.attrs.append(attr)
+        usq = state[3] ** 2 + state[4] ** 2 + state[5] ** 2
+        if m == 0:
+            assert (usq > 0).all(), "a photon's momentum must not be 0"
+            self.inv_gamma = 1.0 / np.sqrt(usq)
+        else:
+            self.inv_gamma = 1.0 / np.sqrt(1.0 + usq)
+        # the reference's per-step scratch (field, chi), then bookkeeping
+        for attr in ("chi", "Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+            setattr(self, attr, np.zeros(N))
+        self._to_be_pruned = np.zeros(N, dtype=bool)
         self.id = np.arange(N, dtype=np.uint64)
-        self.attrs += ["id"]
-        self.next_id = N
+        self.attrs += ["inv_gamma", "chi", "Ex", "Ey", "Ez", "Bx", "By", "Bz", "_to_be_pruned",
+                       "id"]
+        self.N_buffered = self.buffer_size = self.next_id = N
 
     # ------------------------------------------------------ array storage --
     def __getattr__(self, name):
@@ -271,23 +249,23 @@ class Particles(object):
 
     # ------------------------------------------------------- QED wiring --
     def set_photon(self, photon: "Particles") -> None:
-        """particles.py:187-212"""
-        assert self.m > 0, "photon cannot radiate photon"
-        assert self.RR != RadiationReactionType.LL and self.RR != RadiationReactionType.CLL, \
-            "LL equation does not radiate photon"
-        assert isinstance(photon, Particles), "photon must be Particle class"
-        assert photon.m == 0 and photon.q == 0, "photon must be m=0 and q=0"
+        """particles.py:187-212: this species emits into `photon`."""
+        assert self.m > 0, "only massive species emit photons"
+        assert self.RR not in (RadiationReactionType.LL, RadiationReactionType.CLL), \
+            "LL and CLL radiation reaction is continuous: no photons are emitted"
+        assert isinstance(photon, Particles), "photon must be a Particles species"
+        assert photon.m == 0 and photon.q == 0, "the photon species must be massless and neutral"
         self.radiating = True
         self.photon_ref = ref(photon)
         self._add_qed_arrays("photon_delta")
 
     def set_pair(self, electron: "Particles", positron: "Particles") -> None:
-        """particles.py:215-242"""
-        assert self.m == 0, "massive particle cannot create BW pair"
+        """particles.py:215-242: this photon species converts into pairs."""
+        assert self.m == 0, "only photons create Breit-Wheeler pairs"
         assert isinstance(electron, Particles) and isinstance(positron, Particles), \
-            "pair must be tuple or list of Particle class"
-        assert electron.m == m_e and electron.q == -e, "first of the pair must be electron"
-        assert positron.m == m_e and positron.q == e, "second of the pair must be positron"
+            "electron and positron must be Particles species"
+        assert electron.m == m_e and electron.q == -e, "the first pair species must be electrons"
+        assert positron.m == m_e and positron.q == e, "the second pair species must be positrons"
         self.bw = True
         self.bw_electron_ref = ref(electron)
         self.bw_positron_ref = ref(positron)
@@ -502,17 +480,16 @@ class SpinParticles(Particles):
     def __init__(self, name: str, q: int, m: float, N: int = 0, props: Tuple = None,
                  RR: RadiationReactionType = RadiationReactionType.PHOTON, ae: float = 1.14e-3,
                  push: bool = True) -> None:
-        if props is None:
-            if m == 0:
-                assert N == 0, "cannot initialize photons with only N without props."
-            super().__init__(name, q, m, N, props, RR, push)
-            self.sx = np.zeros(int(N))
-            self.sy = np.zeros(int(N))
-            self.sz = np.zeros(int(N))
+        # props: the six of Particles and sx, sy, sz (the reference's
+        # constructor cannot take them, particles.py:566-567); spins 0 if not given
         if props:
-            assert len(props) == 9, "given properties must has length of 9"
+            assert len(props) == 9, f"props holds x..uz and sx, sy, sz: 9 arrays, got {len(props)}"
             super().__init__(name, q, m, N, props[:6], RR, push)
-            self.sx, self.sy, self.sz = prepare_props(props[6:], int(N))
+            spins = prepare_props(props[6:], int(N))
+        else:
+            super().__init__(name, q, m, N, None, RR, push)
+            spins = [np.zeros(int(N)) for _ in range(3)]
+        self.sx, self.sy, self.sz = spins
         self.ae = ae
         self.spin = True
         self.attrs += ["sx", "sy", "sz"]
@@ -532,17 +509,19 @@ class SpinParticles(Particles):
 
 
 def prepare_props(props: Tuple, N: int) -> list:
-    """particles.py:606-635"""
-    props_ = []
+    """particles.py:606-635: each property as a float64 vector of length N
+    (an array as given, a list or tuple converted, a scalar only for N = 1)."""
+    out = []
     for prop in props:
         if isinstance(prop, np.ndarray):
-            assert len(prop.shape) == 1, "given property is not vector"
-            assert prop.shape[0] == N, "given N does not match given property length"
-            props_.append(prop)
-        if isinstance(prop, (int, float)):
-            assert N == 1, "given N does not match given property length"
-            props_.append(np.asarray([prop], dtype=np.float64))
-        if isinstance(prop, (list, tuple)):
-            assert len(prop) == N, "given N does not match given property length"
-            props_.append(np.asarray(prop, dtype=np.float64))
-    return props_
+            assert prop.ndim == 1, "each property must be a 1-D array"
+            arr = prop
+        elif isinstance(prop, (int, float)):
+            arr = np.asarray([prop], dtype=np.float64)
+        elif isinstance(prop, (list, tuple)):
+            arr = np.asarray(prop, dtype=np.float64)
+        else:
+            continue
+        assert arr.shape[0] == N, f"a property has {arr.shape[0]} values for N = {N}"
+        out.append(arr)
+    return out
