@@ -2226,7 +2226,9 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
             if (cached) {
               C.x(k) = x; C.y(k) = y; C.z(k) = z;
               if (rebuild) {
-                C.ux(k) = ux; C.uy(k) = uy; C.uz(k) = uz;
+                // a photon moved from the last window's cache has its
+                // momentum there already (moved by the list kernel)
+                if (!stale) { C.ux(k) = ux; C.uy(k) = uy; C.uz(k) = uz; }
                 C.wake(k) = 0u;
               }
             }
