@@ -288,8 +288,9 @@ class Simulation(object):
               callback_every: int = 1):
         """simulation.py:74-161.  ``call_back(istep)`` runs after the steps
         istep with (istep + 1) a multiple of ``callback_every`` (1: after
-        every step, as in the reference); the steps in between run as one
-        call, without a host round trip (and as graph replays with
+        every step, as in the reference), seeing ``t`` and ``step`` of that
+        step's start as the reference's callback does; the steps in between
+        run as one call, without a host round trip (and as graph replays with
         ``graph=True``)."""
         callback_every = max(1, int(callback_every))
         if total is None:
@@ -306,9 +307,20 @@ class Simulation(object):
                 chunk = min(end - self.step, self.print_every - self.step % self.print_every)
             else:
                 chunk = end - self.step
+            t_last = self.t  # the clock at the start of the chunk's last step
+            for _ in range(chunk - 1):
+                t_last += self.dt  # the same additions the library makes
             self.run_steps(chunk)
             if call_back is not None and self.step % callback_every == 0:
-                call_back(self.step - 1)
+                # as in the reference (simulation.py:140-144) the callback sees
+                # the state after step istep, with the clock and the step
+                # counter not yet advanced past it
+                t_end, istep = self.t, self.step - 1
+                self.t, self.step = t_last, istep
+                try:
+                    call_back(istep)
+                finally:
+                    self.t, self.step = t_end, istep + 1
             if self.print_every and self.step % self.print_every == 0:
                 elapsed = perf_counter_ns() - tic
                 tic = perf_counter_ns()

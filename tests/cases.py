@@ -146,6 +146,30 @@ def run_oracle_case(case, nsteps=None, chunks=1, libm=False, spin_emission=False
     """Run a case through the oracle.  subset: indices of the seed electrons
     to run (with their ids; particles created get ids from the case's full
     seed count on), e.g. to replay part of a large GPU run."""
+    sim, sp = build_oracle_case(case, libm=libm, spin_emission=spin_emission, subset=subset)
+    steps = case["nsteps"] if nsteps is None else nsteps
+    per = steps // chunks
+    done = 0
+    for k in range(chunks):
+        m = per if k < chunks - 1 else steps - done
+        sim.start(m)
+        done += m
+    out = {}
+    for s in sp:
+        d = {"n": int(s.N_buffered)}
+        for a in ("x", "y", "z", "ux", "uy", "uz", "id"):
+            d[a] = np.array(getattr(s, a)[: s.N_buffered])
+        if s.tau is not None:
+            d["tau"] = np.array(s.tau[: s.N_buffered])
+        if s.spin:
+            for a in ("sx", "sy", "sz"):
+                d[a] = np.array(getattr(s, a)[: s.N_buffered])
+        out[s.name] = d
+    return out
+
+
+def build_oracle_case(case, libm=False, spin_emission=False, subset=None):
+    """The oracle's species and simulation of a case, not yet run."""
     import oracle
     from oracle.sim import OParticles, OSimulation
     from sfparticles_amd.qed.tables import load_rs_tables, load_od_tables
@@ -178,25 +202,7 @@ def run_oracle_case(case, nsteps=None, chunks=1, libm=False, spin_emission=False
     sim = OSimulation(*sp, dt=case["dt"], fields=case["field"], photon_threshold=case["threshold"],
                       qed=case["qed"], seed=case["seed"], rs_tables=load_rs_tables(),
                       od_tables=load_od_tables(), libm=libm, spin_emission=spin_emission)
-    steps = case["nsteps"] if nsteps is None else nsteps
-    per = steps // chunks
-    done = 0
-    for k in range(chunks):
-        m = per if k < chunks - 1 else steps - done
-        sim.start(m)
-        done += m
-    out = {}
-    for s in sp:
-        d = {"n": int(s.N_buffered)}
-        for a in ("x", "y", "z", "ux", "uy", "uz", "id"):
-            d[a] = np.array(getattr(s, a)[: s.N_buffered])
-        if s.tau is not None:
-            d["tau"] = np.array(s.tau[: s.N_buffered])
-        if s.spin:
-            for a in ("sx", "sy", "sz"):
-                d[a] = np.array(getattr(s, a)[: s.N_buffered])
-        out[s.name] = d
-    return out
+    return sim, sp
 
 
 def compare_final(gpu, ref, rtol=0.0):

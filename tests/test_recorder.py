@@ -1,7 +1,8 @@
 """Device-side trajectory recorder (Simulation.record / records).
 
 The reference records trajectories with a per-step callback that copies the
-electron arrays to the host (example/trajectory.py:62-70).  The recorder
+electron arrays to the host (example/trajectory.py:62-70), called after the
+step's push and before its clock advances (simulation.py:140-144).  The recorder
 stores the same six arrays on the device from a kernel at the end of each
 recorded step, also inside graph replays and across capacity halts.  The GPU
 tests hold it to the callback pattern bit for bit: every recorded row equals
@@ -51,7 +52,7 @@ def _callback_rows(case, nsteps, n, every, **kw):
         for j, k in enumerate(KEYS):
             row[:m, j] = np.array(getattr(e, k))[:m]
         rows.append(row)
-        ts.append(sim.t)
+        ts.append(sim.t + sim.dt)  # the callback sees the step's start time
 
     sim.start(nsteps, call_back=cb)
     return np.array(rows), np.array(ts)
@@ -129,5 +130,66 @@ def test_callback_every_k_steps():
     sim5.start(40, call_back=grab(every5, sim5, e5), callback_every=5)
     assert sorted(every5) == [4, 9, 14, 19, 24, 29, 34, 39]
     for istep, (x, step) in every5.items():
-        assert step == istep + 1
+        assert step == istep  # the reference's counter at its callback
         assert np.array_equal(x, every1[istep][0])
+
+
+def _oracle_rows(case, nsteps, n, every):
+    """The oracle's start(call_back=...) (simulation.py:140-141) recording the
+    first n electrons every `every` steps (NaN past the count), with the
+    time after the step and the clock and counter the callback sees."""
+    from tests.cases import build_oracle_case
+    sim, sp = build_oracle_case(case)
+    e = sp[0]
+    rows, ts, seen = [], [], []
+
+    def cb(istep):
+        seen.append((istep, sim.step, sim.t))
+        if (istep + 1) % every:
+            return
+        m = min(n, e.N_buffered)
+        row = np.full((n, 6), np.nan)
+        for j, k in enumerate(KEYS):
+            row[:m, j] = getattr(e, k)[:m]
+        rows.append(row)
+        ts.append(sim.t + sim.dt)
+
+    sim.start(nsteps, call_back=cb)
+    return np.array(rows), np.array(ts), seen
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_recorder_matches_oracle_callback(graph):
+    """Device recorder rows (eager and in graph replays, with store growth)
+    equal the oracle's per-step callback rows bit for bit, and the recorded
+    times the oracle clock after each recorded step."""
+    case = head_on_case(3000, 192, seed=15, a0=150.0)
+    want, t_want, _ = _oracle_rows(case, 192, 3200, 3)
+    got, t_got, sim, e, g = _recorded(case, 192, 3200, 3, graph=graph)
+    print("rows", got.shape, "photons", g.N_buffered, "graph batches", sim.stats(0)["graph_batches"])
+    assert got.shape == (64, 3200, 6) and g.N_buffered > 0
+    assert np.array_equal(got, want, equal_nan=True)
+    assert np.array_equal(t_got, t_want)
+
+
+@pytest.mark.gpu
+def test_callback_every_matches_oracle_callback():
+    """start(call_back=..., callback_every=k): at each call the electrons'
+    state, the clock and the step counter are those the oracle's callback
+    sees after the same step."""
+    case = head_on_case(2000, 60, seed=16, a0=150.0)
+    _, _, seen = _oracle_rows(case, 60, 1, 1)
+    ref = {}
+    from tests.cases import build_oracle_case
+    osim, osp = build_oracle_case(case)
+    osim.start(60, call_back=lambda i: ref.__setitem__(i, np.array(osp[0].x[:200])))
+    got = {}
+    sim, e, _ = _make(case)
+    sim.start(60, call_back=lambda i: got.__setitem__(i, (np.array(e.x)[:200], sim.step, sim.t)),
+              callback_every=6)
+    assert sorted(got) == list(range(5, 60, 6))
+    clock = {i: (step, t) for i, step, t in seen}
+    for istep, (x, step, t) in got.items():
+        assert np.array_equal(x, ref[istep])
+        assert (step, t) == clock[istep]
