@@ -360,10 +360,18 @@ int sfp_sim_run(void* sim, double* t, uint64_t step0, int64_t nsteps, int32_t fl
  * (phase 0), QED (1) or emission (2) kernel; 0 returns it to the tuner, whose default
  * is full occupancy.  No counterpart in the reference. */
 int sfp_sim_set_grid(void* sim, int32_t species, int32_t phase, int32_t grid);
-/* Grid in use, workgroup size and number of candidate grids of a kernel
- * (all 0 for a species without that kernel). */
+/* Pin the grid and the most rows of 256 particles per tile (1..32 for the
+ * sweep kernels, 1..16 for the QED kernels, clamped to the kernel's own
+ * most; 0: the kernel's most) of one kernel; grid 0 (with rows 0) returns
+ * both to the tuner, which tries, per grid, the kernel's most rows, half and
+ * a quarter of it.  Results do not depend on the shape.  No counterpart in
+ * the reference. */
+int sfp_sim_set_shape(void* sim, int32_t species, int32_t phase, int32_t grid, int32_t rows);
+/* Grid in use, workgroup size, most tile rows (0 for a kernel without row
+ * tiles; rows may be NULL) and number of candidate shapes of a kernel (all 0
+ * for a species without that kernel). */
 int sfp_sim_launch_shape(void* sim, int32_t species, int32_t phase, int32_t* grid,
-                         int32_t* tile, int32_t* ncand);
+                         int32_t* tile, int32_t* rows, int32_t* ncand);
 /* Accumulated time (ms) and launch count of one species' step kernel since
  * the last reset (requires SFP_RUN_TIMING; synchronous).  phase 0 is the
  * sweep kernel (envelope bound, push of particles that cannot undergo a QED

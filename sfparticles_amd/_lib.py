@@ -152,7 +152,9 @@ SIGNATURES = {
     "sfp_sim_set_timing_every": [P, I64],
     "sfp_store_histogram": [P, I32, I32, I32, D, D, P],
     "sfp_sim_set_grid": [P, I32, I32, I32],
-    "sfp_sim_launch_shape": [P, I32, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
+    "sfp_sim_set_shape": [P, I32, I32, I32, I32],
+    "sfp_sim_launch_shape": [P, I32, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32),
+                             C.POINTER(I32)],
     "sfp_sim_stats": [P, I32, C.POINTER(I64)],
     "sfp_sim_reset_stats": [P],
     "sfp_copy_f64": [P, P, I64, P],

@@ -259,19 +259,22 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
 // is recomputed from that unchanged state when it is appended, so a
 // look-back (a round trip through the device-coherent flags, several
 // microseconds) is paid once per tile, not once per row.  Short lists take
-// fewer rows per tile, so that every workgroup of the grid has a tile.
+// fewer rows per tile, so that every workgroup of the grid has a tile.  The
+// launch passes the most rows a tile may take (1..QROWS, a choice of the
+// launch-shape tuner; results do not depend on it).
 static constexpr int QROWS = 16;
 
-__device__ inline int qed_rows(int64_t n_list) {
+__device__ inline int qed_rows(int64_t n_list, int rcap) {
   // the fewest rounds of tiles (one tile per workgroup per round) that
-  // QROWS-row tiles need, then the fewest rows that fill that many rounds: a
+  // rcap-row tiles need, then the fewest rows that fill that many rounds: a
   // list of 1.5 rounds of 16-row tiles would leave half the workgroups idle in
   // the second; in 12-row tiles it fills two
+  rcap = rcap < 1 ? 1 : (rcap > QROWS ? QROWS : rcap);
   const int64_t per_round = (int64_t)gridDim.x * TILE;
-  const int64_t m = (n_list + per_round * QROWS - 1) / (per_round * QROWS);
+  const int64_t m = (n_list + per_round * rcap - 1) / (per_round * rcap);
   if (m < 1) return 1;
   const int64_t r = (n_list + per_round * m - 1) / (per_round * m);
-  return r < 1 ? 1 : (r > QROWS ? QROWS : (int)r);
+  return r < 1 ? 1 : (r > rcap ? rcap : (int)r);
 }
 
 // After the rows of a tile have left their event ballots in mask[row][wave]:
@@ -580,7 +583,7 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
 }
 
 // ------------------------------------------------------------ sweep kernel --
-// A tile is ROWS rows of 256 consecutive particles (thread t handles particle
+// A tile is up to ROWS rows of 256 consecutive particles (thread t handles particle
 // t of every row, so loads stay coalesced); the QED list keeps ascending
 // index order, i.e. (row, wave, lane) order, with one look-back per tile.
 //
@@ -598,7 +601,8 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
 static constexpr int ROWS = 32;
 
 template <bool PHOTON, bool QED, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+        int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -632,13 +636,14 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
       return;
     }
   }
-  // Rows per tile: ROWS once the store fills a tile per workgroup of the
-  // grid; fewer (a power of two) for shorter stores, which then still spread
-  // over many workgroups.  The list order does not depend on the tiling.
-  int rows = ROWS;
+  // Rows per tile: rmax (at most ROWS, the launch shape's choice) once the
+  // store fills a tile per workgroup of the grid; fewer (a power of two) for
+  // shorter stores, which then still spread over many workgroups.  The list
+  // order does not depend on the tiling.
+  int rows = rmax < 1 ? 1 : (rmax > ROWS ? ROWS : rmax);
   {
     const int64_t per = (n_old + (int64_t)TILE * gridDim.x - 1) / ((int64_t)TILE * gridDim.x);
-    if (per < ROWS) {
+    if (per < rows) {
       rows = 1;
       while (rows < per) rows <<= 1;
     }
@@ -1329,7 +1334,8 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // particle-steps/s).  The LDS of a workgroup (37-50 KB) allows at most 4
 // workgroups per CU.
 template <int KIND, int FM, bool MIX, int NETK>
-__global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+      int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1347,8 +1353,8 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_hot = sp.cnt->n_hot;
-  // tiles of up to QROWS rows, or of one row for the rate network
-  const int rows = NETK == 2 ? 1 : qed_rows(n_hot);
+  // tiles of up to rmax (<= QROWS) rows, or of one row for the rate network
+  const int rows = NETK == 2 ? 1 : qed_rows(n_hot, rmax);
   const int64_t span = (int64_t)rows * TILE;
   const int64_t ntiles = (n_hot + span - 1) / span;
   // workgroups without work leave before reading the destination counts,
@@ -1763,7 +1769,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
 // checked by k_hot (one per listed particle).
 template <int KIND, int FM>
 __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, int s,
-                                               const StepArgs* __restrict__ ap) {
+                                               const StepArgs* __restrict__ ap, int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -1883,7 +1889,8 @@ static constexpr uint32_t WAKE_NEVER = 0xffffffffu;
 static constexpr int AW = 32;
 static constexpr int64_t WAKE_PAD = (int64_t)TILE * AW;
 
-__global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+        int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -2041,7 +2048,8 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 // FP64 (a 1e6-electron focused-pulse collision: 0.26 -> 0.22 s per 3000
 // steps against 1 wave).
 template <int KIND, int FM, bool MIX>
-__global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap) {
+__global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+         int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
@@ -2059,7 +2067,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
   __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_list = sp.cnt->n_hot;
-  const int rows = qed_rows(n_list);
+  const int rows = qed_rows(n_list, rmax);
   const int64_t SPAN = (int64_t)rows * TILE;
   const int64_t ntiles = (n_list + SPAN - 1) / SPAN;
   // workgroups without work leave before reading the destination counts,
@@ -2379,7 +2387,7 @@ __global__ void k_catchup(const Plan* __restrict__ plan, int s, uint32_t wend, d
 }
 
 // ------------------------------------------------------------- dispatch ----
-typedef void (*kfn)(const Plan*, int, const StepArgs*);
+typedef void (*kfn)(const Plan*, int, const StepArgs*, int);
 
 template <int FM, bool MIX>
 static kfn sweep_fn(bool photon, bool qed) {
@@ -2426,7 +2434,7 @@ static kfn pick_emit(int fm, int kind) {
 }
 
 // ------------------------------------------------------------- host side ---
-static constexpr int MAX_CAND = 8;
+static constexpr int MAX_CAND = 16;
 static constexpr int64_t SYNC_EVERY = 64;
 
 // events around one kernel launch: for the caller's timing and/or for the
@@ -2437,6 +2445,12 @@ struct EvRec {
   bool timed;
 };
 
+// Launch shape of a persistent kernel: workgroups, and the most rows of 256
+// particles a tile may take (kernels without row tiles ignore it)
+struct Shape {
+  int grid, rows;
+};
+
 struct Sim {
   int nsp = 0;
   std::vector<sfp_species_desc> desc;
@@ -2445,11 +2459,12 @@ struct Sim {
   kfn sweep[SFP_MAX_SPECIES] = {};
   kfn hot[SFP_MAX_SPECIES] = {};
   kfn emit[SFP_MAX_SPECIES] = {};
-  // launch shapes: candidate persistent grids of each (species, phase) kernel,
-  // the one in use, and a grid the caller pinned (0: tuned)
-  std::vector<int> cand[SFP_MAX_SPECIES][NPH];
+  // launch shapes: candidate shapes of each (species, phase) kernel, the one
+  // in use, and a grid and tile rows the caller pinned (0: tuned)
+  std::vector<Shape> cand[SFP_MAX_SPECIES][NPH];
   int choice[SFP_MAX_SPECIES][NPH] = {};
   int pinned[SFP_MAX_SPECIES][NPH] = {};
+  int pinned_rows[SFP_MAX_SPECIES][NPH] = {};
   double tune_ms[SFP_MAX_SPECIES][NPH][MAX_CAND] = {};
   int tune_cnt[SFP_MAX_SPECIES][NPH][MAX_CAND] = {};
   int64_t tune_n[SFP_MAX_SPECIES][NPH] = {};  // work of the kernel when last tuned (+1)
@@ -2653,23 +2668,35 @@ static int sim_pick_kernels(Sim* sim) {
     sim->emit[s] = (sim->qed[s] && mixed && netk == 1) ? pick_emit(sim->fm, kind) : nullptr;
     for (int ph = 0; ph < NPH; ++ph) {
       kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
-      std::vector<int>& c = sim->cand[s][ph];
+      std::vector<Shape>& c = sim->cand[s][ph];
       c.clear();
       sim->choice[s][ph] = 0;
       if (!fn) continue;
+      // most tile rows: the sweep kernels (not the photon wake-list kernel)
+      // and the QED kernels with row tiles (not the rate network's one-row
+      // tiles); 0 where the kernel has none
+      const bool awake = photon && sim->qed[s];
+      const int rmax = ph == 0 ? (awake ? 0 : ROWS) : (ph == 1 && netk != 2 ? QROWS : 0);
       if (sim->pinned[s][ph] > 0) {
-        c.push_back(sim->pinned[s][ph]);
+        const int r = sim->pinned_rows[s][ph];
+        c.push_back({sim->pinned[s][ph], rmax == 0 ? 0 : (r > 0 ? std::min(r, rmax) : rmax)});
         continue;
       }
       // Tiles are taken from an atomic counter, so the grid only sets how
       // many workgroups compete for them: full occupancy first (the default),
       // then fewer per CU (shorter look-back chains and less contention when
-      // the work is small).  Results do not depend on the grid.
+      // the work is small); and per grid, the kernel's most rows per tile,
+      // then half and a quarter of it (more tiles, so fewer idle workgroups
+      // in the last round, at one look-back per tile).  Results depend on
+      // neither.
       int per_cu = 0;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, TILE, 0));
-      for (int k = std::max(per_cu, 1); (int)c.size() < MAX_CAND; k /= 2) {
-        c.push_back(k * sim->cus);
-        if (k == 1) break;
+      for (int k = std::max(per_cu, 1);; k /= 2) {
+        for (int r = rmax, q = 0; q < 3 && (int)c.size() < MAX_CAND; r /= 2, ++q) {
+          c.push_back({k * sim->cus, r});
+          if (r <= 1) break;
+        }
+        if (k == 1 || (int)c.size() >= MAX_CAND) break;
       }
     }
   }
@@ -2678,7 +2705,7 @@ static int sim_pick_kernels(Sim* sim) {
   return SFP_OK;
 }
 
-static int sim_grid(const Sim* sim, int s, int ph) { return sim->cand[s][ph][sim->choice[s][ph]]; }
+static Shape sim_shape(const Sim* sim, int s, int ph) { return sim->cand[s][ph][sim->choice[s][ph]]; }
 
 extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
                               const sfp_field* terms, int32_t nterms, void* tables,
@@ -2950,12 +2977,15 @@ static hipEvent_t sim_event(Sim* sim) {
 }
 
 // Launch the kernel of (species s, phase ph) for the step in argument slot j.
-// tune >= 0: a tuning launch, run with candidate grid (tune mod candidates)
-// and timed for the tuner.
+// tune >= 0: a tuning launch of the batch's step `tune`, run with candidate
+// shape (tune + tune / 8) mod candidates and timed for the tuner.  The photon
+// list's cost runs in an 8-step cycle (a full rebuild, then cheap steps);
+// shifting by one candidate every 8 steps gives each candidate steps of every
+// phase of that cycle, even for an even number of candidates.
 static int launch(Sim* sim, int s, int ph, int j, int tune) {
   kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
-  const std::vector<int>& cand = sim->cand[s][ph];
-  const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
+  const std::vector<Shape>& cand = sim->cand[s][ph];
+  const int c = tune >= 0 ? (tune + tune / 8) % (int)cand.size() : sim->choice[s][ph];
   const bool timed = sim->timing && sim->args_host[j].step % (uint64_t)sim->timing_every == 0;
   const bool ev = timed || tune >= 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2964,8 +2994,8 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
     e1 = sim_event(sim);
     if (e0) HIPCHK(hipEventRecord(e0, sim->stream));
   }
-  hipLaunchKernelGGL(fn, dim3(cand[c]), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev, s,
-                     (const StepArgs*)(sim->args_dev + j));
+  hipLaunchKernelGGL(fn, dim3(cand[c].grid), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev,
+                     s, (const StepArgs*)(sim->args_dev + j), cand[c].rows);
   HIPCHK(hipGetLastError());
   if (ev && e0 && e1) {
     HIPCHK(hipEventRecord(e1, sim->stream));
@@ -3095,7 +3125,12 @@ static bool sim_needs_tune(const Sim* sim) {
 }
 
 // After a tuning batch (steps interleaved over the candidates): keep the
-// candidate with the least time per launch.
+// candidate with the least time per launch if it beats the default shape
+// (candidate 0: full occupancy, the kernel's most rows) by more than 3%,
+// else the default.  A candidate has 5-30 timed launches per batch, whose
+// mean varies by a few percent from batch to batch; without the margin the
+// tuner followed that noise to shapes that ran the whole step 1-3% slower
+// (a 1e7-electron collision, 128 steps about its centre).
 static int sim_tune_pick(Sim* sim) {
   CHK(sim_collect_events(sim));
   for (int s = 0; s < sim->nsp; ++s)
@@ -3113,6 +3148,9 @@ static int sim_tune_pick(Sim* sim) {
           any = true;
         }
       }
+      if (any && best != 0 && sim->tune_cnt[s][ph][0] &&
+          best_ms > 0.97 * sim->tune_ms[s][ph][0] / sim->tune_cnt[s][ph][0])
+        best = 0;
       if (best != sim->choice[s][ph]) sim->graph_stale = true;
       sim->choice[s][ph] = best;
       sim->tune_n[s][ph] = sim_work(sim, s, ph);
@@ -3378,17 +3416,33 @@ extern "C" int sfp_sim_set_grid(void* p, int32_t species, int32_t phase, int32_t
   if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH || grid < 0)
     return set_err(SFP_E_ARG, "bad species/phase/grid");
   sim->pinned[species][phase] = grid;
+  sim->pinned_rows[species][phase] = 0;
+  return sim_pick_kernels(sim);
+}
+
+extern "C" int sfp_sim_set_shape(void* p, int32_t species, int32_t phase, int32_t grid,
+                                 int32_t rows) {
+  Sim* sim = (Sim*)p;
+  if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH || grid < 0)
+    return set_err(SFP_E_ARG, "bad species/phase/grid");
+  if (rows < 0 || rows > ROWS)
+    return set_err(SFP_E_ARG, "tile rows %d outside 0..%d", (int)rows, ROWS);
+  if (rows > 0 && grid == 0) return set_err(SFP_E_ARG, "tile rows pinned without a grid");
+  sim->pinned[species][phase] = grid;
+  sim->pinned_rows[species][phase] = rows;
   return sim_pick_kernels(sim);
 }
 
 extern "C" int sfp_sim_launch_shape(void* p, int32_t species, int32_t phase, int32_t* grid,
-                                    int32_t* tile, int32_t* ncand) {
+                                    int32_t* tile, int32_t* rows, int32_t* ncand) {
   Sim* sim = (Sim*)p;
   if (!sim || species < 0 || species >= sim->nsp || phase < 0 || phase >= NPH)
     return set_err(SFP_E_ARG, "bad species/phase");
   const bool has = !sim->cand[species][phase].empty();
-  *grid = has ? sim_grid(sim, species, phase) : 0;
+  const Shape sh = has ? sim_shape(sim, species, phase) : Shape{0, 0};
+  *grid = sh.grid;
   *tile = has ? TILE : 0;
+  if (rows) *rows = sh.rows;
   *ncand = (int32_t)sim->cand[species][phase].size();
   return SFP_OK;
 }

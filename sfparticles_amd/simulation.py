@@ -247,13 +247,21 @@ class Simulation(object):
         self._prepare()
         L.call("sfp_sim_set_flag_clear_every", self._sim, int(steps))
 
-    def launch_shape(self, species: int, phase: int):
-        """(grid, workgroup size, number of candidate grids) of a kernel."""
+    def set_shape(self, species: int, phase: int, grid: int, rows: int = 0):
+        """Pin the workgroup count and the most rows of 256 particles per tile
+        of a kernel (rows 0: the kernel's most); grid 0 returns both to the
+        tuner."""
         self._prepare()
-        g, t, n = C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("sfp_sim_set_shape", self._sim, int(species), int(phase), int(grid), int(rows))
+
+    def launch_shape(self, species: int, phase: int):
+        """(grid, workgroup size, most tile rows, number of candidate shapes)
+        of a kernel."""
+        self._prepare()
+        g, t, r, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         L.call("sfp_sim_launch_shape", self._sim, int(species), int(phase), C.byref(g),
-               C.byref(t), C.byref(n))
-        return int(g.value), int(t.value), int(n.value)
+               C.byref(t), C.byref(r), C.byref(n))
+        return int(g.value), int(t.value), int(r.value), int(n.value)
 
     def stats(self, species: int) -> dict:
         """Counters of one species since the last reset: particle-steps (live

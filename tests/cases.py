@@ -116,8 +116,11 @@ def build_gpu_case(case, precision="fp64", sampler=None, grids=None, reserve=Non
     sim = sfa.Simulation(*sp, dt=case["dt"], fields=Fields(terms=case["field"]), print_every=None,
                          photon_threshold=case["threshold"], seed=case["seed"], precision=precision,
                          sampler=sampler, **sim_kw)
-    for (si, ph), grid in (grids or {}).items():
-        sim.set_grid(si, ph, grid)
+    for (si, ph), shape in (grids or {}).items():
+        if isinstance(shape, tuple):
+            sim.set_shape(si, ph, *shape)
+        else:
+            sim.set_grid(si, ph, shape)
     # capacities reserved up front (stores otherwise start at the seed count
     # and grow through capacity halts)
     for s, cap in zip(sp, reserve or ()):

@@ -9,8 +9,9 @@ bit-exact agreement with plain runs:
     runs within their statistical spread (created particles get other ids in
     the ensemble, hence other draws);
   * every species of a run replayed as captured graphs, with the online
-    tuner, or with pinned small grids is bit-identical to the eager run, down
-    to the order and ids of the created particles;
+    tuner (grid and tile rows), or with pinned small grids and tiles of few
+    rows is bit-identical to the eager run, down to the order and ids of the
+    created particles;
   * the on-device per-shot histogram against numpy on the downloaded arrays.
 CPU: the ensemble generator's draw order and the host checks.
 """
@@ -114,7 +115,9 @@ def test_graph_autotune_and_grids_are_bit_identical():
     case = head_on_case(20000, 640, seed=4, a0=150.0)
     ref, _ = run_gpu_case(case)
     for kw in (dict(graph=True), dict(autotune=True), dict(graph=True, autotune=True),
-               dict(grids={(0, 0): 256, (0, 1): 256, (2, 0): 256, (2, 1): 512})):
+               dict(grids={(0, 0): 256, (0, 1): 256, (2, 0): 256, (2, 1): 512}),
+               dict(grids={(0, 0): (512, 4), (0, 1): (256, 3), (1, 0): (768, 1),
+                           (1, 1): (256, 1), (2, 1): (512, 2)})):
         got, sim = run_gpu_case(case, **kw)
         st = sim.stats(0)
         print(kw, "graph batches", st["graph_batches"], "captures", st["captures"],
@@ -124,8 +127,31 @@ def test_graph_autotune_and_grids_are_bit_identical():
             assert st["graph_batches"] >= 5
         if kw.get("autotune"):
             assert st["tune_batches"] >= 1
+            # the tuner's candidates: grids times 3 tile-row caps for the
+            # electron sweep and QED kernels
+            for ph, rmax in ((0, 32), (1, 16)):
+                g, t, r, n = sim.launch_shape(0, ph)
+                assert n >= 3 and n % 3 == 0 and r in (rmax, rmax // 2, rmax // 4)
+        if "grids" in kw and isinstance(kw["grids"][(0, 0)], tuple):
+            assert sim.launch_shape(0, 0)[:3] == (512, 256, 4)
+            assert sim.launch_shape(0, 1)[:3] == (256, 256, 3)
         compare_final(got, ref)
         del sim
+
+
+@pytest.mark.gpu
+def test_launch_shape_pins_are_checked():
+    from sfparticles_amd._lib import SFPError
+    case = head_on_case(2000, 8, seed=4, a0=150.0)
+    _, sim = run_gpu_case(case)
+    with pytest.raises(SFPError):
+        sim.set_shape(0, 0, 256, 33)
+    with pytest.raises(SFPError):
+        sim.set_shape(0, 1, 0, 4)
+    sim.set_shape(0, 1, 256, 32)  # clamped to the QED kernel's 16 rows
+    assert sim.launch_shape(0, 1)[:4] == (256, 256, 16, 1)
+    sim.set_shape(0, 1, 0)
+    assert sim.launch_shape(0, 1)[3] > 1
 
 
 @pytest.mark.gpu

@@ -131,8 +131,12 @@ def test_trained_network_matches_table_spectrum(trained):
     lc = rng.uniform(-3, 2, 400000)
     d_net, _ = s.sample(lc, rng.random(lc.size), "mlp")
     d_tab, _ = s.sample(lc, rng.random(lc.size), "table")
-    p = stats.ks_2samp(np.log(d_net), np.log(d_tab)).pvalue
-    print("all chi: KS p", p)
+    r = stats.ks_2samp(np.log(d_net), np.log(d_tab))
+    print("all chi: KS distance", r.statistic, "p", r.pvalue)
+    # over the whole range the largest gap between the two CDFs of delta
+    # stays under 0.5 % (measured 0.3 %, KS p 0.06 with 4e5 draws each; the
+    # p = 0.01 critical distance at this size is 0.36 %)
+    assert r.statistic < 5e-3
 
 
 @pytest.mark.gpu
