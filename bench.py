@@ -312,6 +312,15 @@ def calibrate_copy(n_doubles, reps):
                       "reps": reps, "copy_GBps": round(16 * n_doubles / dt / 1e9, 1)}), flush=True)
 
 
+def hbm_used_gb():
+    """Device memory in use (total - free), GB."""
+    import ctypes as C
+    from sfparticles_amd import _lib as L
+    fr, tot = C.c_int64(), C.c_int64()
+    L.call("sfp_mem_info", C.byref(fr), C.byref(tot))
+    return (tot.value - fr.value) / 1e9
+
+
 def cascade_run(n, steps, precision, seed=0, x0_ctau=3.0, reserve=(0, 0, 0)):
     """The cascade workload: n electrons at rest uniformly in a ball of radius
     0.1 um at the node of a rotating-E standing wave (two counter-propagating
@@ -409,14 +418,18 @@ def ensemble_run(nshots, count_max, steps, check_shots, seed=0, ppe=64):
 
     sim, sp = build(slice(0, n), fields, "mixed", "mlp", True)
     L.call("sfp_synchronize")
+    hbm_peak = hbm_used_gb()
     t0 = time.perf_counter()
     done = 0
     while done < steps:
         m = min(500, steps - done)
         sim.run_steps(m)
         done += m
+        used = hbm_used_gb()
+        hbm_peak = max(hbm_peak, used)
         print(f"[bench] ensemble {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
-              f"{[s_.N_buffered for s_ in sp]}", file=sys.stderr, flush=True)
+              f"{[s_.N_buffered for s_ in sp]}, HBM in use {used:.1f} GB", file=sys.stderr,
+              flush=True)
     L.call("sfp_synchronize")
     elapsed = time.perf_counter() - t0
     st = [sim.stats(i) for i in range(3)]
@@ -449,7 +462,8 @@ def ensemble_run(nshots, count_max, steps, check_shots, seed=0, ppe=64):
                 electrons=int(n), shots=int(nshots), final_counts=counts,
                 graph_batches=st[0]["graph_batches"], captures=st[0]["captures"],
                 tuning_batches=st[0]["tune_batches"], grids=shapes,
-                capacity_replays=st[0]["replays"], checks_vs_fp64_alone=checks)
+                capacity_replays=st[0]["replays"], deferred_conversions=st[2]["deferrals"],
+                hbm_used_gb_max=round(hbm_peak, 1), checks_vs_fp64_alone=checks)
 
 
 def main():

@@ -98,6 +98,8 @@ int sfp_free(void* dev);
 int sfp_memcpy_htod(void* dst_dev, const void* src_host, int64_t bytes);
 int sfp_memcpy_dtoh(void* dst_host, const void* src_dev, int64_t bytes);
 int sfp_synchronize(void);
+/* Free and total device memory in bytes (for sizing stores up front). */
+int sfp_mem_info(int64_t* free_bytes, int64_t* total_bytes);
 
 /* Upload QED tables into library-owned device memory; either pointer may be NULL. */
 int sfp_tables_create(const sfp_rs_tables* rs, const sfp_od_tables* od, void** tables);
@@ -448,8 +450,11 @@ int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
  *      on dense steps
  * [15] dense steps: steps on which the QED kernel visited every lepton by
  *      index, the sweep standing down (taken when the last step's culling
- *      bound listed at least 7/8 of the particles; results do not change) */
-#define SFP_SIM_NSTATS 16
+ *      bound listed at least 7/8 of the particles; results do not change)
+ * [16] photons: steps on which conversions found no room in the pair
+ *      stores and were made after the stores grew (deferred conversions;
+ *      results do not change) */
+#define SFP_SIM_NSTATS 17
 int sfp_sim_stats_n(void* sim, int32_t species, int64_t* stats, int32_t n);
 /* Diagnostics of the exact QED evaluations of one species since the last
  * reset (synchronous; no counterpart in the reference, whose behaviour in

@@ -89,6 +89,7 @@ SIGNATURES = {
     "sfp_memcpy_htod": [P, P, I64],
     "sfp_memcpy_dtoh": [P, P, I64],
     "sfp_synchronize": [],
+    "sfp_mem_info": [C.POINTER(I64), C.POINTER(I64)],
     "sfp_tables_create": [P, P, C.POINTER(P)],
     "sfp_tables_destroy": [P],
     "sfp_eval_field": [P, I32, P, P, P, D, I64, P, P, P, P, P, P, P, P],

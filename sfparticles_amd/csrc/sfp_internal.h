@@ -100,7 +100,17 @@ struct Counts {
   int64_t chi_clamp;
   int64_t chi_nan;
   int64_t p_max_bits;
+  // photons: conversions of this step that found no room in the pair stores
+  // (left unconverted, marked PRUNED_DEFER, converted by the deferred-append
+  // kernel once the host has grown the stores), the first list position of
+  // one as ~position (0: none), and the recoveries that did so
+  int64_t defer_n;
+  uint64_t defer_kneg;
+  int64_t deferrals;
 };
+
+// pruned byte of a photon whose conversion waits for room in the pair stores
+static constexpr uint8_t PRUNED_DEFER = 2;
 
 extern const size_t kAttrSize[SFP_NATTR];
 

@@ -1122,6 +1122,14 @@ extern "C" int sfp_synchronize(void) {
   return SFP_OK;
 }
 
+extern "C" int sfp_mem_info(int64_t* free_bytes, int64_t* total_bytes) {
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = (int64_t)fr;
+  if (total_bytes) *total_bytes = (int64_t)tot;
+  return SFP_OK;
+}
+
 extern "C" const char* sfp_last_error(void) { return g_err.c_str(); }
 extern "C" int sfp_version(void) { return 1; }
 

@@ -1298,11 +1298,22 @@ __device__ inline void stage_tables(const Plan& P, void* lds_rs, double2* lds_od
 }
 
 // Counts of the destinations after the launch (run by its last workgroup).
+// room: the appends the destinations had room for (the photon kernel); the
+// ranks past it were deferred, and the launch halts the step so that the
+// host grows the destinations and runs the deferred-append kernel.
 __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntiles,
                                       const SpeciesDev& sp, const SpeciesDev* dA,
                                       const SpeciesDev* dB, int64_t baseA, int64_t baseB,
-                                      int64_t idA, int64_t idB) {
+                                      int64_t idA, int64_t idB, int64_t room = INT64_MAX,
+                                      uint64_t step = 0) {
   uint64_t total = ntiles > 0 ? (ld_flag(&P.flags[s][ph][ntiles - 1]) & FLAG_VAL_MASK) : 0;
+  if ((int64_t)total > room) {
+    sp.cnt->defer_n = (int64_t)total - room;
+    total = (uint64_t)room;
+    P.halt->species = s;
+    P.halt->step = (int64_t)step;
+    P.halt->flag = 1;
+  }
   if (total) atomicAdd((unsigned long long*)&sp.cnt->spawned, (unsigned long long)total);
   if (dA) {
     dA->cnt->n = baseA + (int64_t)total;
@@ -2041,6 +2052,55 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
   });
 }
 
+// One conversion of the photon at list position k, recomputed from its
+// unchanged state: the pair at the photon's position, electron with delta of
+// its momentum, positron with 1 - delta (cpu.py:142-191), at store positions
+// posA / posB with ids idA / idB; the photon becomes a hole (cpu.py:189-190),
+// not pushed.
+template <int KIND, int FM, bool MIX, typename Node>
+__device__ inline void photon_convert(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
+                                      const WinCache& C, const Node* lds_rs,
+                                      const double2* lds_od, bool rebuild, int64_t k,
+                                      const SpeciesDev* dA, const SpeciesDev* dB, int64_t posA,
+                                      int64_t posB, uint64_t idA, uint64_t idB) {
+  const int64_t i = (int64_t)sp.hot[k];
+  const bool cached = k < sp.ccap;
+  double x, y, z, ux, uy, uz;
+  bool woke = false;
+  const bool moved = rebuild && cached && sp.oldk[k] != WAKE_NEVER;
+  if (cached && (!rebuild || moved)) {
+    x = C.x(k); y = C.y(k); z = C.z(k);
+    ux = C.ux(k); uy = C.uy(k); uz = C.uz(k);
+  } else {
+    x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
+    ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
+    woke = rebuild && sp.wake[i] == a.wrel;
+  }
+  const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+  if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
+  uint32_t shot = 0;
+  const sfp_fieldset* FS = fields_of(P, sp, i, shot);
+  double tau = KIND == KIND_OD ? sp.tau[i] : 0.0;
+  double delta = 0.0;
+  if (MIX) {
+    (void)qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y, z,
+                                      (float)ux, (float)uy, (float)uz, (float)ig, sp.id[i], tau,
+                                      delta);
+  } else {
+    (void)qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z, ux,
+                                    uy, uz, ig, sp.id[i], tau, delta);
+  }
+  const double dp = 1.0 - delta;
+  if (dA)
+    spawn_lepton<FM, MIX>(P, FS, shot, *dA, a, posA, idA, x, y, z, delta * ux, delta * uy,
+                          delta * uz);
+  if (dB)
+    spawn_lepton<FM, MIX>(P, FS, shot, *dB, a, posB, idB, x, y, z, dp * ux, dp * uy, dp * uz);
+  sp.pruned[i] = 1;
+  sp.wake[i] = WAKE_NEVER;
+  if (cached) C.wake(k) = WAKE_NEVER;
+}
+
 // Occupancy: 3 waves per SIMD in mixed precision (168 VGPRs; the compiler's
 // own choice for some instances is 2), which ran faster than a cap at 128
 // VGPRs (4 waves, spilling) and, for optical depth, than the compiler's 2
@@ -2076,9 +2136,17 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
   if (blockIdx.x >= nwg) return;
   const SpeciesDev* dA = sp.pair_e_dst >= 0 ? &P.sp[sp.pair_e_dst] : nullptr;
   const SpeciesDev* dB = sp.pair_p_dst >= 0 ? &P.sp[sp.pair_p_dst] : nullptr;
-  if (!room_or_halt(P, s, a, dA, dB, n_list)) return;
   const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
   const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
+  // Room in the pair stores: the conversions ranked past it are deferred
+  // (the photon is left as it was, marked PRUNED_DEFER) to k_defer, which the
+  // host runs after growing the stores by what the step actually converted.
+  // The stores thus need room for the conversions, not for one per listed
+  // photon (a billion listed photons in a large ensemble).
+  int64_t room = INT64_MAX;
+  if (dA) room = dA->cap - baseA;
+  if (dB && dB->cap - baseB < room) room = dB->cap - baseB;
+  room = room < 0 ? 0 : room;
   ScanState* scan = P.scan[s][1];
   const uint32_t epoch = next_epoch(scan);
   const int lane = wave_lane();
@@ -2257,51 +2325,21 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       }
     }
 
-    // the tile's conversions in list order, recomputed from the photons'
-    // unchanged state: the pair at the photon's position, electron with
-    // delta of its momentum, positron with 1 - delta (cpu.py:142-191); the
-    // photon becomes a hole (cpu.py:189-190), not pushed
+    // the tile's conversions in list order (photon_convert); those ranked
+    // past the room wait for k_defer, the photon left as it was
     const uint32_t nev = tile_events(sh_mask, rows, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t k = tile * SPAN + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
-      const int64_t i = (int64_t)sp.hot[k];
-      const bool cached = k < sp.ccap;
-      double x, y, z, ux, uy, uz;
-      bool woke = false;
-      const bool moved = rebuild && cached && sp.oldk[k] != WAKE_NEVER;
-      if (cached && (!rebuild || moved)) {
-        x = C.x(k); y = C.y(k); z = C.z(k);
-        ux = C.ux(k); uy = C.uy(k); uz = C.uz(k);
-      } else {
-        x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-        ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-        woke = rebuild && sp.wake[i] == a.wrel;
+      if (rank >= room) {
+        sp.pruned[sp.hot[k]] = PRUNED_DEFER;
+        // the tile's first deferred conversion records its list position
+        if (rank == (room > (int64_t)sh_pref ? room : (int64_t)sh_pref))
+          atomicMax((unsigned long long*)&sp.cnt->defer_kneg, (unsigned long long)~(uint64_t)k);
+        continue;
       }
-      const double ig = sfp_inv_gamma_photon(ux, uy, uz);
-      if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
-      uint32_t shot = 0;
-      const sfp_fieldset* FS = fields_of(P, sp, i, shot);
-      double tau = KIND == KIND_OD ? sp.tau[i] : 0.0;
-      double delta = 0.0;
-      if (MIX) {
-        (void)qed_event32<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node32*)lds_rs, lds_od, x, y,
-                                          z, (float)ux, (float)uy, (float)uz, (float)ig, sp.id[i],
-                                          tau, delta);
-      } else {
-        (void)qed_event<true, KIND, FM>(P, FS, sp, a, (const sfp_rs_node*)lds_rs, lds_od, x, y, z,
-                                        ux, uy, uz, ig, sp.id[i], tau, delta);
-      }
-      const double dp = 1.0 - delta;
-      if (dA)
-        spawn_lepton<FM, MIX>(P, FS, shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z,
-                              delta * ux, delta * uy, delta * uz);
-      if (dB)
-        spawn_lepton<FM, MIX>(P, FS, shot, *dB, a, baseB + rank, (uint64_t)(idB + rank), x, y, z,
-                              dp * ux, dp * uy, dp * uz);
-      sp.pruned[i] = 1;
-      sp.wake[i] = WAKE_NEVER;
-      if (cached) C.wake(k) = WAKE_NEVER;
+      photon_convert<KIND, FM, MIX>(P, sp, a, C, lds_rs, lds_od, rebuild, k, dA, dB, baseA + rank,
+                                    baseB + rank, (uint64_t)(idA + rank), (uint64_t)(idB + rank));
     }
   }
 
@@ -2316,7 +2354,79 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
   }
   (void)qdiag_flush(sp.cnt, sh_qd);
   kernel_exit(scan, nwg, [&]() {
-    finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
+    finish_appends(P, s, 1, ntiles, sp, dA, dB, baseA, baseB, idA, idB, room, a.step);
+    scan->epoch = epoch;
+  });
+}
+
+// Deferred conversions of a step (after k_photon halted it for want of room
+// and the host grew the pair stores by what it deferred): the photons marked
+// PRUNED_DEFER, from the first one's list position on, ranked in list order
+// behind the conversions k_photon appended -- the positions and ids they
+// would have had -- and converted from their unchanged state as k_photon
+// converts.  The step then continues with the next kernel.
+template <int KIND, int FM, bool MIX>
+__global__ void __launch_bounds__(TILE) k_defer(const Plan* __restrict__ plan, int s,
+                                                const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const SpeciesDev& sp = P.sp[s];
+  using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
+  __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
+  __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
+  __shared__ int64_t sh_tile;
+  __shared__ uint64_t sh_pref;
+  __shared__ uint64_t sh_mask[1][WAVES];
+  __shared__ uint32_t sh_q[TILE];
+
+  const int64_t n_list = sp.cnt->n_hot;
+  const int64_t n_def = sp.cnt->defer_n;
+  const uint64_t kneg = sp.cnt->defer_kneg;
+  const int64_t k0 = kneg ? (int64_t)~kneg : n_list;
+  const int64_t ntiles = k0 < n_list ? (n_list - k0 + TILE - 1) / TILE : 0;
+  const uint32_t nwg = participants(ntiles);
+  if (blockIdx.x >= nwg) return;
+  const SpeciesDev* dA = sp.pair_e_dst >= 0 ? &P.sp[sp.pair_e_dst] : nullptr;
+  const SpeciesDev* dB = sp.pair_p_dst >= 0 ? &P.sp[sp.pair_p_dst] : nullptr;
+  if (!room_or_halt(P, s, a, dA, dB, n_def)) return;
+  const int64_t baseA = dA ? dA->cnt->n : 0, baseB = dB ? dB->cnt->n : 0;
+  const int64_t idA = dA ? dA->cnt->next_id : 0, idB = dB ? dB->cnt->next_id : 0;
+  ScanState* scan = P.scan[s][2];
+  const uint32_t epoch = next_epoch(scan);
+  const int lane = wave_lane();
+  const int wv = threadIdx.x / 64;
+  const bool rebuild = list_rebuild(a);
+  const WinCache C = win_cache(sp, (uint32_t)sp.cnt->cpar);
+  bool staged = false;
+  while (true) {
+    if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
+    __syncthreads();
+    const int64_t tile = sh_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    if (!staged) {
+      stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
+      staged = true;
+    }
+    const int64_t k = k0 + tile * TILE + threadIdx.x;
+    const bool mark = k < n_list && sp.pruned[sp.hot[k]] == PRUNED_DEFER;
+    const uint64_t m = __ballot(mark);
+    if (lane == 0) sh_mask[0][wv] = m;
+    const uint32_t nev = tile_events(sh_mask, 1, P.flags[s][2], tile, epoch, sh_q, &sh_pref);
+    for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
+      const int64_t kk = k0 + tile * TILE + (int64_t)sh_q[e];
+      const int64_t rank = (int64_t)sh_pref + (int64_t)e;
+      photon_convert<KIND, FM, MIX>(P, sp, a, C, lds_rs, lds_od, rebuild, kk, dA, dB,
+                                    baseA + rank, baseB + rank, (uint64_t)(idA + rank),
+                                    (uint64_t)(idB + rank));
+    }
+  }
+  kernel_exit(scan, nwg, [&]() {
+    finish_appends(P, s, 2, ntiles, sp, dA, dB, baseA, baseB, idA, idB);
+    sp.cnt->defer_n = 0;
+    sp.cnt->defer_kneg = 0;
+    sp.cnt->deferrals += 1;
     scan->epoch = epoch;
   });
 }
@@ -2427,6 +2537,20 @@ static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk = 0) {
                : pick_hot_p<false>(fm, photon, kind, 0);
 }
 
+typedef void (*kfn_defer)(const Plan*, int, const StepArgs*);
+
+template <int FM>
+static kfn_defer defer_fn(int kind, bool mixed) {
+  if (mixed) return kind == KIND_OD ? k_defer<KIND_OD, FM, true> : k_defer<KIND_RS, FM, true>;
+  return kind == KIND_OD ? k_defer<KIND_OD, FM, false> : k_defer<KIND_RS, FM, false>;
+}
+
+static kfn_defer pick_defer(int fm, int kind, bool mixed) {
+  if (fm == SFP_FM_GAUSSIAN) return defer_fn<SFP_FM_GAUSSIAN>(kind, mixed);
+  if (fm == SFP_FM_PLANE) return defer_fn<SFP_FM_PLANE>(kind, mixed);
+  return defer_fn<SFP_FM_ANY>(kind, mixed);
+}
+
 static kfn pick_emit(int fm, int kind) {
   if (fm == SFP_FM_GAUSSIAN) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_GAUSSIAN> : k_emit<KIND_RS, SFP_FM_GAUSSIAN>;
   if (fm == SFP_FM_PLANE) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_PLANE> : k_emit<KIND_RS, SFP_FM_PLANE>;
@@ -2459,6 +2583,7 @@ struct Sim {
   kfn sweep[SFP_MAX_SPECIES] = {};
   kfn hot[SFP_MAX_SPECIES] = {};
   kfn emit[SFP_MAX_SPECIES] = {};
+  kfn_defer defer[SFP_MAX_SPECIES] = {};  // photons: deferred conversions
   // launch shapes: candidate shapes of each (species, phase) kernel, the one
   // in use, and a grid and tile rows the caller pinned (0: tuned)
   std::vector<Shape> cand[SFP_MAX_SPECIES][NPH];
@@ -2666,6 +2791,7 @@ static int sim_pick_kernels(Sim* sim) {
       netk = 1;
     sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, netk) : nullptr;
     sim->emit[s] = (sim->qed[s] && mixed && netk == 1) ? pick_emit(sim->fm, kind) : nullptr;
+    sim->defer[s] = (sim->qed[s] && photon) ? pick_defer(sim->fm, kind, mixed) : nullptr;
     for (int ph = 0; ph < NPH; ++ph) {
       kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
       std::vector<Shape>& c = sim->cand[s][ph];
@@ -3013,9 +3139,9 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
   }
   for (int s = s0; s < sim->nsp; ++s) {
     int first = (s == s0) ? ph0 : 0;
-    if (first == 0) CHK(launch(sim, s, 0, j, tune));
-    if (sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
-    if (sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
+    if (first <= 0) CHK(launch(sim, s, 0, j, tune));
+    if (first <= 1 && sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
+    if (first <= 2 && sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
   }
   if (sim->plan.rec.on) {
     const int g = (int)std::min<int64_t>((sim->plan.rec.n + 255) / 256, 1024);
@@ -3052,19 +3178,23 @@ static int sim_capture(Sim* sim) {
   return SFP_OK;
 }
 
-// After a halt: grow the destinations of the halted species so that every
-// listed particle can append, clear the flag; the caller replays.
-static int sim_recover(Sim* sim, int s) {
+// After a halt: grow the destinations of the halted species by the room the
+// halted kernel asks for -- one append per listed particle (the lepton QED
+// kernel halts before it starts), or the conversions the photon kernel
+// deferred (it ran through and halted at its end: *deferred, the caller runs
+// k_defer) -- and clear the flag; the caller replays.
+static int sim_recover(Sim* sim, int s, bool* deferred) {
   const sfp_species_desc& d = sim->desc[s];
-  int64_t n_hot = sim->counts_host[s].n_hot;
+  const Counts& c = sim->counts_host[s];
+  *deferred = c.defer_n > 0;
+  const int64_t extra = *deferred ? c.defer_n : c.n_hot;
   for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst}) {
     if (dst < 0) continue;
     Store* st = sim->stores[dst];
-    const int64_t need = sim->counts_host[dst].n + n_hot;
+    const int64_t need = sim->counts_host[dst].n + extra;
     if (need > st->cap) {
-      // the room the halted kernel asks for (one append per listed particle)
-      // and a quarter more, so that growing lists do not halt every step;
-      // where HBM cannot hold that, a sixteenth more
+      // that room and a quarter more, so that growing lists do not halt
+      // every step; where HBM cannot hold that, a sixteenth more
       int64_t want = need + need / 4 + 4096;
       if (sfp_store_reserve(st, want) != SFP_OK) {
         (void)hipGetLastError();
@@ -3247,20 +3377,29 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   if (nsteps < 0 || nsteps >= (int64_t)1 << 31)
     return set_err(SFP_E_ARG, "nsteps must be in [0, 2^31) per run call");
   // window cache of the photon visit lists: a quarter of the store (longer
-  // lists keep their tail in the store); sized at the start of a run only, so
-  // that it stays valid while stores grow inside the run
+  // lists keep their tail in the store), and no more than a quarter of the
+  // HBM free at sizing (a store of billions of photons leaves no room for a
+  // cache of a quarter of it); sized at the start of a run only, so that it
+  // stays valid while stores grow inside the run
   for (int s = 0; s < sim->nsp; ++s) {
     if (!sim->plan.sp[s].photon || !sim->qed[s]) continue;
     const int64_t want = sim->stores[s]->cap / 4;
     if (want > sim->cache_cap[s]) {
-      if (sim->cache_mem[s]) HIPCHK(hipFree(sim->cache_mem[s]));
-      sim->cache_mem[s] = nullptr;
-      sim->cache_cap[s] = 0;
-      const int64_t cc = want + want / 2;
       // two sets (this window's and the last one's) of 7 words per entry
-      HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * 2 * 7 * sizeof(double)));
-      sim->cache_cap[s] = cc;
-      dirty = true;
+      const int64_t per = 2 * 7 * (int64_t)sizeof(double);
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      const int64_t limit = (int64_t)((fr + (size_t)(sim->cache_cap[s] * per)) / 4 / (size_t)per);
+      const int64_t cc = std::min<int64_t>(want + want / 2, limit);
+      // grown only by a quarter or more (the limit moves with free memory)
+      if (cc > sim->cache_cap[s] + sim->cache_cap[s] / 4) {
+        if (sim->cache_mem[s]) HIPCHK(hipFree(sim->cache_mem[s]));
+        sim->cache_mem[s] = nullptr;
+        sim->cache_cap[s] = 0;
+        HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * (size_t)per));
+        sim->cache_cap[s] = cc;
+        dirty = true;
+      }
     }
   }
   if (dirty) CHK(sim_refresh_plan(sim));
@@ -3332,8 +3471,18 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
                        s, (long long)sim->halt_host->step);
       last_s = s;
       last_h = h;
-      CHK(sim_recover(sim, s));
-      CHK(launch_step(sim, (int)h, s, 1, -1));
+      bool deferred = false;
+      CHK(sim_recover(sim, s, &deferred));
+      if (deferred) {
+        // the photon kernel ran through; its deferred conversions, then the
+        // rest of the step
+        hipLaunchKernelGGL(sim->defer[s], dim3(sim->cus * 4), dim3(TILE), 0, sim->stream,
+                           (const Plan*)sim->plan_dev, s, (const StepArgs*)(sim->args_dev + h));
+        HIPCHK(hipGetLastError());
+        CHK(launch_step(sim, (int)h, s, 2, -1));
+      } else {
+        CHK(launch_step(sim, (int)h, s, 1, -1));
+      }
       for (int64_t j = h + 1; j < batch; ++j) CHK(launch_step(sim, (int)j, 0, 0, -1));
       CHK(sim_read_counts(sim));
     }
@@ -3508,6 +3657,7 @@ extern "C" int sfp_sim_stats_n(void* p, int32_t species, int64_t* stats, int32_t
   int64_t v[SFP_SIM_NSTATS];
   CHK(sfp_sim_stats(p, species, v));
   v[15] = sim->counts_host[species].dense_steps;
+  v[16] = sim->counts_host[species].deferrals;
   for (int32_t k = 0; k < n && k < SFP_SIM_NSTATS; ++k) stats[k] = v[k];
   return SFP_OK;
 }
@@ -3536,6 +3686,7 @@ extern "C" int sfp_sim_reset_stats(void* p) {
     Counts c = sim->counts_host[s];
     c.psteps = c.processed = c.spawned = c.hot_total = c.slept = c.sleeps = c.emit_total = 0;
     c.dense_steps = c.p_over1 = c.z_clamp = c.chi_clamp = c.chi_nan = c.p_max_bits = 0;
+    c.deferrals = 0;
     HIPCHK(hipMemcpy(sim->stores[s]->cnt, &c, sizeof(c), hipMemcpyHostToDevice));
   }
   sim->syncs = 0;

@@ -115,10 +115,15 @@ def test_cascade_od_bit_exact():
     # a0 = 500 and pulses starting 1 ctau from the node keep the cascade at
     # ~1e4 particles (64 seeds -> ~230 e-, ~170 e+, ~7000 photons by step 1600)
     case = cascade_case(n=64, nsteps=1600, seed=2, a0=500.0, x0_ctau=1.0)
-    gpu, _ = run_gpu_case(case)
+    gpu, sim = run_gpu_case(case)
     ref = run_oracle_case(case)
-    print("counts", _names(gpu))
+    st = sim.stats(2)
+    print("counts", _names(gpu), "photon stats", st)
     assert ref["positron"]["n"] > 0  # pairs were created
+    # the pair stores started full: conversions that found no room were
+    # deferred, made after the stores grew, at the positions and ids they
+    # would have had
+    assert st["deferrals"] > 0
     compare_final(gpu, ref)
 
 
