@@ -8,14 +8,21 @@ rejection-sampling photon emission + Breit-Wheeler pairs (electrons,
 positrons, photons), photon threshold 2 m c^2, Philox seed 0; FP64.
 
 A step is one pass of the fused step kernels over every species.  Inputs are
-uploaded to HBM before the timed region; the timed region is exactly --steps
-steps starting at step 0 of the collision.  --warmup steps run first on a
+uploaded to HBM before the timed region.  With --steps 3000 (or more) the
+timed region is the whole run from step 0; with fewer, the run is
+fast-forwarded untimed to --steps / 2 steps before the collision centre (step
+1500, where the pulse centre meets the bunch centre) and the window of
+--steps steps around it is timed, closing with the catch-up that brings
+sleeping photons to the window's last step.  --warmup steps run first on a
 separate, smaller copy of the workload (kernel load, allocator warm-up).
 
 Prints one JSON line.  Also reported: the dominant kernel's achieved
 algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around its
-launches on every 7th step of the timed region, --timing-every) and the CPU oracle timed on this machine's cores on a bounded
-sample of the same workload.
+launches on every step of the timed window, --timing-every), the whole step's
+algorithmic bandwidth, and the CPU oracle timed on this machine's cores over
+the same window on a sample of the same state.  --summarize-profile turns
+the rocprofv3 output of a profiled run into the markdown kept under
+profiles/.
 
 --workload cascade runs the cascade configuration instead: 1e4 electrons at
 rest at the node of a rotating-E standing wave of two counter-propagating
@@ -200,7 +207,7 @@ def sample_state(species, every):
     return out
 
 
-def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=7,
+def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=1,
             field="gaussian", cpu_every=0):
     from sfparticles_amd import _lib as L
     if warmup > 0:
@@ -288,12 +295,167 @@ def cpu_baseline(base, start, steps, every):
     return psteps / (t1 - t0), t1 - t0, oracle.num_threads(), n0
 
 
-def load_traffic(kernel):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f)
-    return None
+def _short(name):
+    """Kernel name without 'void ' and the parameter list."""
+    name = name[5:] if name.startswith("void ") else name
+    depth = 0
+    for i, ch in enumerate(name):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def _windows(seq, steps):
+    """Timed windows of a dispatch sequence [(name, ...)]: the dispatches after
+    one k_catchup up to and including the next, when exactly `steps` step
+    prologues lie between them (the settle before the timed steps, the timed
+    steps, the settle that ends them)."""
+    cu = [i for i, r in enumerate(seq) if r[0].startswith("k_catchup")]
+    out = []
+    for a, b in zip(cu, cu[1:]):
+        seg = seq[a + 1:b + 1]
+        if sum(1 for r in seg if r[0].startswith("k_step_prologue")) == steps:
+            out.append(seg)
+    return out
+
+
+def summarize_profile(d):
+    """Markdown summary of a profiled run of this benchmark: rocprofv3 output
+    under d/kt (kernel trace of the bench command, its JSON line in d/kt.json),
+    d/fetch, d/write (FETCH_SIZE, WRITE_SIZE), d/cal_f, d/cal_w (the same
+    counters over the copy kernel of --calibrate-copy) and d/sq (SQ
+    counters); commands in d/commands.txt.  Per timed window (one per path,
+    in the order fp64, mixed, mixed+mlp) and kernel: dispatches, mean
+    duration, HBM bytes per dispatch (counters scaled by the copy kernel's
+    ratio of counted to known bytes) and the fraction of wave cycles spent
+    waiting."""
+    import collections
+    out = []
+    line = json.load(open(os.path.join(d, "kt.json")))
+    steps = line["steps"]
+    paths = ["fp64", "mixed", "mixed+mlp"]
+    cfg = line["config"]
+    head = {"fp64-tables": "fp64", "mixed-fp32/fp64-tables": "mixed"}.get(cfg["path"], "mixed+mlp")
+    if os.path.exists(os.path.join(d, "commands.txt")):
+        out += ["Commands (repository root on one MI355X, `TMPDIR=/tmp`):", "", "```",
+                open(os.path.join(d, "commands.txt")).read().strip(), "```", ""]
+    out += [f"Bench line of the kernel-trace run: value {line['value']:.4g} particle-steps/s "
+            f"({head}), {line['ms_per_step']} ms/step, window {cfg.get('window')}.", "",
+            "Roofline in that line: `" + json.dumps(line["roofline"]) + "`", ""]
+    # kernel trace
+    kt = _csv_rows(os.path.join(d, "kt"), "kernel_trace.csv")
+    kt.sort(key=lambda r: int(r["Start_Timestamp"]))
+    seq = [(_short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+           for r in kt]
+    wins = _windows(seq, steps)
+    trace = {}
+    for path, seg in zip(paths, wins):
+        t = collections.defaultdict(list)
+        for name, a, b in seg:
+            t[name].append((b - a) / 1e3)
+        # from the first step's prologue (host work before it is not step time)
+        first = next(a for name, a, b in seg if name.startswith("k_step_prologue"))
+        trace[path] = (t, (seg[-1][2] - first) / 1e3)
+
+    def counters(sub, names):
+        rows = _csv_rows(os.path.join(d, sub), "counter_collection.csv")
+        byd = collections.OrderedDict()
+        for r in rows:
+            e = byd.setdefault(int(r["Dispatch_Id"]), [_short(r["Kernel_Name"]),
+                                                       collections.defaultdict(float)])
+            e[1][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp = [(v[0], v[1]) for _, v in sorted(byd.items())]
+        res = {}
+        for path, seg in zip(paths, _windows(disp, steps)):
+            agg = collections.defaultdict(lambda: collections.defaultdict(list))
+            for name, c in seg:
+                for k in names:
+                    if k in c:
+                        agg[name][k].append(c[k])
+            res[path] = agg
+        return res, disp
+
+    have = {k: os.path.isdir(os.path.join(d, k)) for k in ("fetch", "write", "sq", "cal_f", "cal_w")}
+    scale = {}
+    for sub, cname in (("cal_f", "FETCH_SIZE"), ("cal_w", "WRITE_SIZE")):
+        if have[sub]:
+            _, disp = counters(sub, [cname])
+            v = [c[cname] for n, c in disp if n.startswith("k_copy")]
+            if v:
+                # --calibrate-copy: 2^27 doubles read and written per dispatch
+                scale[cname] = float(np.median(v)) * 1024.0 / (8.0 * (1 << 27))
+    fetch = counters("fetch", ["FETCH_SIZE"])[0] if have["fetch"] else {}
+    write = counters("write", ["WRITE_SIZE"])[0] if have["write"] else {}
+    sqn = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
+           "SQ_ACTIVE_INST_VALU", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD"]
+    sq = counters("sq", sqn)[0] if have["sq"] else {}
+    if scale:
+        out += ["Counter scale (counted / known bytes of the copy kernel, 1 GiB read and "
+                "1 GiB written per dispatch): " + ", ".join(f"{k} {v:.3f}" for k, v in scale.items())
+                + "; HBM bytes below are the counters divided by it.", ""]
+    for path in paths:
+        if path not in trace:
+            continue
+        t, wall = trace[path]
+        out += [f"## {path}: timed window ({steps} steps and the closing catch-up), "
+                f"{wall / 1e3:.3f} ms from the first step's start to the last dispatch's end",
+                "",
+                "| kernel | dispatches | mean us | total us | HBM read MB / dispatch | "
+                "HBM written MB / dispatch | wait / wave cycles | busy cycles / dispatch |",
+                "|---|---|---|---|---|---|---|---|"]
+        for name, v in sorted(t.items(), key=lambda kv: -sum(kv[1])):
+            rd = wr = wt = bz = ""
+            f = fetch.get(path, {}).get(name, {}).get("FETCH_SIZE")
+            if f and "FETCH_SIZE" in scale:
+                rd = f"{np.mean(f) * 1024 / scale['FETCH_SIZE'] / 1e6:.1f}"
+            w = write.get(path, {}).get(name, {}).get("WRITE_SIZE")
+            if w and "WRITE_SIZE" in scale:
+                wr = f"{np.mean(w) * 1024 / scale['WRITE_SIZE'] / 1e6:.1f}"
+            q = sq.get(path, {}).get(name, {})
+            if q.get("SQ_WAVE_CYCLES") and q.get("SQ_WAIT_ANY"):
+                wt = f"{sum(q['SQ_WAIT_ANY']) / max(sum(q['SQ_WAVE_CYCLES']), 1):.2f}"
+            if q.get("SQ_BUSY_CYCLES"):
+                bz = f"{np.mean(q['SQ_BUSY_CYCLES']):.3g}"
+            out.append(f"| `{name}` | {len(v)} | {np.mean(v):.1f} | {sum(v):.0f} | {rd} | {wr} | "
+                       f"{wt} | {bz} |")
+        out.append("")
+        if path == head:
+            rf = line["roofline"]
+            kn = {"k_hot": "k_hot", "k_photon": "k_photon", "k_sweep": "k_sweep",
+                  "k_awake": "k_awake", "k_emit": "k_emit"}
+            key = next((k for k in kn if rf["kernel"].startswith(k)), None)
+            # the roofline kernel serves one species per launch; the trace mean
+            # over its dispatches of the species the line names
+            cand = [n for n in t if key and n.startswith(key)]
+            if cand:
+                name = max(cand, key=lambda n: sum(t[n]))
+                durs = t[name]
+                if key in ("k_hot", "k_sweep"):
+                    durs = durs[0::2]  # electron launches (electron, positron per step)
+                mean_ms = float(np.mean(durs)) / 1e3
+                ach = rf["alg_bytes_per_launch"] / (mean_ms / 1e3) / 1e9
+                who = ("electron launches (the kernel serves both leptons)"
+                       if key in ("k_hot", "k_sweep") else "all launches")
+                out += [f"Roofline kernel `{name}`, {who}: trace mean {mean_ms:.4f} ms per "
+                        f"launch against the "
+                        f"line's {rf['avg_launch_ms']} ms (HIP events); "
+                        f"{rf['alg_bytes_per_launch'] / 1e6:.1f} MB algorithmic per launch -> "
+                        f"{ach:.0f} GB/s, frac {ach / HBM_PEAK_GBS:.4f} (line: {rf['frac']}).",
+                        ""]
+    return "\n".join(out)
 
 
 def calibrate_copy(n_doubles, reps):
@@ -478,12 +640,17 @@ def main():
     ap.add_argument("--field", choices=["gaussian", "focused"], default="gaussian",
                     help="head-on workload: the paraxial Gaussian pulse (the benchmark) or the "
                          "same pulse in the focused expansion")
-    ap.add_argument("--timing-every", type=int, default=7,
-                    help="time the step kernels of every k-th step with HIP events (k coprime "
-                         "to the photon list's 8-step rebuild period samples all its phases)")
+    ap.add_argument("--timing-every", type=int, default=1,
+                    help="time the step kernels of every k-th step with HIP events (default: "
+                         "every step, so that the dominant kernel's mean launch time is the "
+                         "mean over the whole window, as a kernel trace of the same run gives "
+                         "it; the events cost about 2%% of the step time)")
     ap.add_argument("--chunk", type=int, default=250,
                     help="head-on: steps per Simulation.run_steps call (a progress line each)")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="print a markdown summary of the rocprofv3 output of a profiled run "
+                         "of this benchmark under DIR (no GPU work)")
     ap.add_argument("--workload", choices=["head-on", "ensemble", "cascade"], default="head-on")
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--count-max", type=float, default=1e6)
@@ -505,6 +672,9 @@ def main():
         print("bench.py runs on exactly one GPU: the particle step is measured on one "
               "MI355X (no multi-GPU path is built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.summarize_profile:
+        print(summarize_profile(args.summarize_profile))
+        return
     if args.calibrate_copy:
         calibrate_copy(1 << 27, 5)
         return

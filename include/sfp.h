@@ -384,10 +384,11 @@ int sfp_sim_kernel_time(void* sim, int32_t species, int32_t phase, double* ms,
                         int64_t* launches);
 int sfp_sim_reset_timing(void* sim);
 /* With SFP_RUN_TIMING, time only the steps whose index `every` divides
- * (default 1: every step); the two events around a launch cost about 8 us
- * of step time, so a sample (every coprime to the photon list's 8-step
- * rebuild period samples every phase of it) keeps the timed run close to
- * an untimed one.  No counterpart in the reference (instrumentation). */
+ * (default 1: every step); the two events around a launch cost about 2% of
+ * the step time of a 1e7-electron collision, so a sample (every coprime to
+ * the photon list's 16-step rebuild period samples every phase of it) keeps
+ * the timed run closer to an untimed one.  No counterpart in the reference
+ * (instrumentation). */
 int sfp_sim_set_timing_every(void* sim, int64_t every);
 /* The stable ranks of appended particles come from a decoupled look-back
  * over per-tile flag words tagged with a 29-bit launch epoch.  The library

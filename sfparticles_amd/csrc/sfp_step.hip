@@ -37,7 +37,7 @@ static constexpr int TILE = 256;
 // Photons fall asleep and wake on steps that are multiples of WIN (counted
 // from the start of a run); the photon visit list is rebuilt on those steps
 // and kept (and extended) in between.
-static constexpr uint32_t WIN = 8;
+static constexpr uint32_t WIN = 16;
 static constexpr int WAVES = TILE / 64;
 // kernels of a species and step: 0 sweep, 1 QED, 2 emission (leptons with the
 // learned photon-energy sampler)
@@ -82,8 +82,12 @@ struct SpeciesDev {
   int32_t photon_dst, pair_e_dst, pair_p_dst, pad;
 };
 
-// One set of the photon window cache: x, y, z, ux, uy, uz and the wake step of
-// the first n list entries, as 7 rows of n words (set `par` of the two)
+// One set of the photon window cache: x, y, z, ux, uy, uz, the wake step and
+// the id of the first n list entries, as 8 rows of n words (set `par` of the
+// two).  The id is the key of a photon's draws: from the cache it loads
+// coalesced, from the store it is a gather of one word per 64-byte line
+// wherever awake photons lie sparse among sleeping ones.
+static constexpr int WIN_WORDS = 8;
 struct WinCache {
   double* b;
   int64_t n;
@@ -94,9 +98,10 @@ struct WinCache {
   __device__ double& uy(int64_t k) const { return b[4 * n + k]; }
   __device__ double& uz(int64_t k) const { return b[5 * n + k]; }
   __device__ uint32_t& wake(int64_t k) const { return *(uint32_t*)(b + 6 * n + k); }
+  __device__ uint64_t& id(int64_t k) const { return *(uint64_t*)(b + 7 * n + k); }
 };
 __device__ inline WinCache win_cache(const SpeciesDev& sp, uint32_t par) {
-  return WinCache{sp.cache + (int64_t)(par & 1u) * 7 * sp.ccap, sp.ccap};
+  return WinCache{sp.cache + (int64_t)(par & 1u) * WIN_WORDS * sp.ccap, sp.ccap};
 }
 
 struct ScanState {
@@ -1930,6 +1935,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
         C.x(k) = sp.x[j]; C.y(k) = sp.y[j]; C.z(k) = sp.z[j];
         C.ux(k) = sp.ux[j]; C.uy(k) = sp.uy[j]; C.uz(k) = sp.uz[j];
         C.wake(k) = sp.pruned[j] ? WAKE_NEVER : sp.wake[j];
+        C.id(k) = sp.id[j];
       }
     }
     kernel_exit(scan, nwg, [&]() {
@@ -2024,6 +2030,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
           if (q < sp.ccap) {
             Cn.x(q) = Co.x(sl); Cn.y(q) = Co.y(sl); Cn.z(q) = Co.z(sl);
             Cn.ux(q) = Co.ux(sl); Cn.uy(q) = Co.uy(sl); Cn.uz(q) = Co.uz(sl);
+            Cn.id(q) = Co.id(sl);
             ok = sl;
           } else {
             sp.x[i] = Co.x(sl); sp.y[i] = Co.y(sl); sp.z[i] = Co.z(sl);
@@ -2151,7 +2158,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
   const uint32_t epoch = next_epoch(scan);
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
-  // Window cache: the list is rebuilt every 8th step and kept in between,
+  // Window cache: the list is rebuilt every WIN-th step and kept in between,
   // so the state of its first ccap entries lives in arrays indexed by list
   // position for the window (coalesced).  A rebuild takes a photon that was
   // awake through the last window from that window's cache (the other set,
@@ -2191,9 +2198,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       bool live = false;
       const int64_t i = k < n_list ? (int64_t)i_next : 0;
       i_next = (row + 1 < rows && k + TILE < n_list) ? sp.hot[k + TILE] : 0u;
-      // the id (the key of the photon's draws) loads with the state, not
-      // after the bound: most visited photons lie past it and draw
-      const uint64_t pid = k < n_list ? sp.id[i] : 0;
+      // the id (the key of the photon's draws; from the window cache where
+      // the state comes from it) loads with the state, not after the bound:
+      // most visited photons lie past it and draw
+      uint64_t pid = 0;
       double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, tau = 0;
       const bool cached = k < sp.ccap;
       bool woke = false;  // asleep until this step
@@ -2210,14 +2218,19 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
           const uint32_t wk = C.wake(k);
           const double cx = C.x(k), cy = C.y(k), cz = C.z(k);
           const double cux = C.ux(k), cuy = C.uy(k), cuz = C.uz(k);
+          const uint64_t cid = C.id(k);
           if (moved || wk <= a.wrel) {
             x = cx; y = cy; z = cz;
             ux = cux; uy = cuy; uz = cuz;
+            pid = cid;
             live = true;
             stale = moved;
           }
         } else {
+          // the id loads with the wake step (most visited photons draw)
           const uint32_t wk = sp.wake[i];
+          const uint64_t sid = sp.id[i];
+          pid = sid;
           if (wk <= a.wrel) {
             const uint8_t pr = sp.pruned[i];
             x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
@@ -2304,7 +2317,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
               if (rebuild) {
                 // a photon moved from the last window's cache has its
                 // momentum there already (moved by the list kernel)
-                if (!stale) { C.ux(k) = ux; C.uy(k) = uy; C.uz(k) = uz; }
+                if (!stale) {
+                  C.ux(k) = ux; C.uy(k) = uy; C.uz(k) = uz;
+                  C.id(k) = pid;
+                }
                 C.wake(k) = 0u;
               }
             }
@@ -2477,22 +2493,67 @@ __global__ void k_record(const Plan* __restrict__ plan, const StepArgs* __restri
 // end of a run: QED photons asleep past its last step get the position of
 // that step (the store must be current when the run returns), and keep
 // sleeping from it
-__global__ void k_catchup(const Plan* __restrict__ plan, int s, uint32_t wend, double cdt_half) {
+__device__ inline bool catchup_one(uint32_t w, uint32_t& s0, uint8_t pr, double& x, double& y,
+                                   double& z, double ux, double uy, double uz, uint32_t wend,
+                                   double cdt_half) {
+  if (w == WAKE_NEVER || w < wend || s0 >= wend || pr) return false;
+  const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+  const uint32_t m = 2u * (wend - s0);
+  x = sfp_repeat_add(x, cdt_half * ig * ux, m);
+  y = sfp_repeat_add(y, cdt_half * ig * uy, m);
+  z = sfp_repeat_add(z, cdt_half * ig * uz, m);
+  s0 = wend;
+  return true;
+}
+
+// Four consecutive photons per thread, every load issued up front (the wake
+// and since steps, the pruned flags, position and momentum, as 16-byte
+// vectors): one memory latency per group instead of three dependent ones,
+// and the state of photons that need no update is read and dropped (most
+// photons sleep at the end of a run).  A group with an update writes its
+// positions and since steps back whole, as vectors (single-element stores
+// four words apart left the writes uncoalesced: three times the bytes).
+// All arrays are allocated for the store's capacity, and a full group lies
+// below the count.
+__global__ void __launch_bounds__(256) k_catchup(const Plan* __restrict__ plan, int s,
+                                                 uint32_t wend, double cdt_half) {
   const SpeciesDev& sp = plan->sp[s];
   const int64_t n = sp.cnt->n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t w = sp.wake[i];
-    if (w == WAKE_NEVER || w < wend) continue;
-    const uint32_t s0 = sp.since[i];
-    if (s0 >= wend || sp.pruned[i]) continue;
-    const double ux = sp.ux[i], uy = sp.uy[i], uz = sp.uz[i];
-    const double ig = sfp_inv_gamma_photon(ux, uy, uz);
-    const uint32_t m = 2u * (wend - s0);
-    sp.x[i] = sfp_repeat_add(sp.x[i], cdt_half * ig * ux, m);
-    sp.y[i] = sfp_repeat_add(sp.y[i], cdt_half * ig * uy, m);
-    sp.z[i] = sfp_repeat_add(sp.z[i], cdt_half * ig * uz, m);
-    sp.since[i] = wend;
+  const int64_t ngrp = (n + 3) / 4;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngrp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = 4 * g;
+    if (i0 + 4 <= n) {
+      const uint4 w = *(const uint4*)(sp.wake + i0);
+      uint4 s0 = *(const uint4*)(sp.since + i0);
+      const uchar4 pr = *(const uchar4*)(sp.pruned + i0);
+      double2 x0 = *(const double2*)(sp.x + i0), x1 = *(const double2*)(sp.x + i0 + 2);
+      double2 y0 = *(const double2*)(sp.y + i0), y1 = *(const double2*)(sp.y + i0 + 2);
+      double2 z0 = *(const double2*)(sp.z + i0), z1 = *(const double2*)(sp.z + i0 + 2);
+      const double2 a0 = *(const double2*)(sp.ux + i0), a1 = *(const double2*)(sp.ux + i0 + 2);
+      const double2 b0 = *(const double2*)(sp.uy + i0), b1 = *(const double2*)(sp.uy + i0 + 2);
+      const double2 c0 = *(const double2*)(sp.uz + i0), c1 = *(const double2*)(sp.uz + i0 + 2);
+      bool any = catchup_one(w.x, s0.x, pr.x, x0.x, y0.x, z0.x, a0.x, b0.x, c0.x, wend, cdt_half);
+      any |= catchup_one(w.y, s0.y, pr.y, x0.y, y0.y, z0.y, a0.y, b0.y, c0.y, wend, cdt_half);
+      any |= catchup_one(w.z, s0.z, pr.z, x1.x, y1.x, z1.x, a1.x, b1.x, c1.x, wend, cdt_half);
+      any |= catchup_one(w.w, s0.w, pr.w, x1.y, y1.y, z1.y, a1.y, b1.y, c1.y, wend, cdt_half);
+      if (any) {
+        *(double2*)(sp.x + i0) = x0; *(double2*)(sp.x + i0 + 2) = x1;
+        *(double2*)(sp.y + i0) = y0; *(double2*)(sp.y + i0 + 2) = y1;
+        *(double2*)(sp.z + i0) = z0; *(double2*)(sp.z + i0 + 2) = z1;
+        *(uint4*)(sp.since + i0) = s0;
+      }
+    } else {
+      for (int64_t i = i0; i < n; ++i) {
+        uint32_t s0 = sp.since[i];
+        double x = sp.x[i], y = sp.y[i], z = sp.z[i];
+        if (catchup_one(sp.wake[i], s0, sp.pruned[i], x, y, z, sp.ux[i], sp.uy[i], sp.uz[i],
+                        wend, cdt_half)) {
+          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+          sp.since[i] = s0;
+        }
+      }
+    }
   }
 }
 
@@ -3104,15 +3165,17 @@ static hipEvent_t sim_event(Sim* sim) {
 
 // Launch the kernel of (species s, phase ph) for the step in argument slot j.
 // tune >= 0: a tuning launch of the batch's step `tune`, run with candidate
-// shape (tune + tune / 8) mod candidates and timed for the tuner.  The photon
-// list's cost runs in an 8-step cycle (a full rebuild, then cheap steps);
-// shifting by one candidate every 8 steps gives each candidate steps of every
-// phase of that cycle, even for an even number of candidates.
+// shape tune mod candidates and timed for the tuner -- except on the photon
+// list's rebuild steps (every WIN-th step and a run's first), which cost
+// several times a step in between: the few of them in a batch would fall to
+// some candidates only, so the tuner compares the steps in between.
 static int launch(Sim* sim, int s, int ph, int j, int tune) {
   kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
   const std::vector<Shape>& cand = sim->cand[s][ph];
-  const int c = tune >= 0 ? (tune + tune / 8) % (int)cand.size() : sim->choice[s][ph];
-  const bool timed = sim->timing && sim->args_host[j].step % (uint64_t)sim->timing_every == 0;
+  const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
+  const StepArgs& ah = sim->args_host[j];
+  if (tune >= 0 && (ah.wrel % WIN == 0 || ah.rel == 0)) tune = -2;  // run, not timed
+  const bool timed = sim->timing && ah.step % (uint64_t)sim->timing_every == 0;
   const bool ev = timed || tune >= 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ev) {
@@ -3385,8 +3448,8 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
     if (!sim->plan.sp[s].photon || !sim->qed[s]) continue;
     const int64_t want = sim->stores[s]->cap / 4;
     if (want > sim->cache_cap[s]) {
-      // two sets (this window's and the last one's) of 7 words per entry
-      const int64_t per = 2 * 7 * (int64_t)sizeof(double);
+      // two sets (this window's and the last one's) of WIN_WORDS words per entry
+      const int64_t per = 2 * WIN_WORDS * (int64_t)sizeof(double);
       size_t fr = 0, tot = 0;
       HIPCHK(hipMemGetInfo(&fr, &tot));
       const int64_t limit = (int64_t)((fr + (size_t)(sim->cache_cap[s] * per)) / 4 / (size_t)per);
@@ -3518,7 +3581,7 @@ extern "C" int sfp_sim_set_recorder(void* p, int32_t species, int64_t n, int64_t
   if (!sim || species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "bad species");
   if (sim->plan.sp[species].photon)
     return set_err(SFP_E_ARG, "the recorder takes a lepton species (photon state is cached "
-                              "between the 8-step list rebuilds)");
+                              "between the photon list rebuilds)");
   if (n > 0 && (every < 1 || max_rows < 1))
     return set_err(SFP_E_ARG, "every and max_rows must be >= 1");
   Recorder& R = sim->plan.rec;

@@ -359,7 +359,7 @@ def test_photon_sleep_carries_across_run_calls(qed, graph):
                              seed=case["seed"], graph=graph)
         return sim, (e, p, g)
 
-    # calls of uneven length, none a multiple of the 8-step list window; the
+    # calls of uneven length, none a multiple of the 16-step list window; the
     # last one through start(), which prunes the converted photons at its end
     sim, sp = make()
 
