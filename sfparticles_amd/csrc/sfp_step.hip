@@ -587,6 +587,44 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
   return 1;
 }
 
+// A photon under rejection sampling converts at a step only if its draw r2
+// for that step (keyed by its id and the step) lies below the conversion
+// probability, and the probability of every step of a window is below the
+// bound pb of the envelope along the photon's straight path over the window.
+// A photon whose own draws for the window's steps all lie at or above 2 pb
+// (the margin the exact front's prefilter uses) therefore does not convert
+// in the window: it sleeps through it as a photon out of the field does,
+// bit-identically.  Steps of the window from this one on (this step's draw
+// has been tested by the caller), 1 if no window is proven.
+template <int FM, bool MIX>
+__device__ inline uint32_t sleep_steps_drawn(const Plan& P, const sfp_fieldset* FS,
+                                             const StepArgs& a, double x, double y, double z,
+                                             double ux, double uy, double uz, double ig,
+                                             uint64_t pid, uint32_t tag, const double2* env_lds) {
+  const double vx = SFP_C * ig * ux, vy = SFP_C * ig * uy, vz = SFP_C * ig * uz;
+  const uint32_t q[2] = {64u, WIN};
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t end = (a.wrel / q[k] + 1u) * q[k];
+    if (end < a.wrel + 2u) continue;
+    const uint32_t d = end - a.wrel;
+    float S = sfp_field_bound_f32_path<FM>(FS, x, y, z, a.t, vx, vy, vz,
+                                           a.dt * (double)d * 1.000001);
+    float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+    const double pb = rs_pbound(P, true, chi_b, ig, a.dt, false, env_lds);
+    const double lim = 2.0 * pb;
+    // no bound, or one a window's draws would likely fall under
+    if (pb < 0.0 || !(lim < 0x1p-16)) continue;
+    bool ok = true;
+    for (uint32_t j = 1; j < d && ok; ++j) {
+      const sfp_u32x4 dr = sfp_draw4(P.seed, pid, a.step + j, tag, 0u);
+      const double r2 = MIX ? (double)sfp32_u01(dr.v[2]) : sfp_u01(dr.v[2], dr.v[3]);
+      ok = !(r2 < lim);
+    }
+    if (ok) return d;
+  }
+  return 1;
+}
+
 // ------------------------------------------------------------ sweep kernel --
 // A tile is up to ROWS rows of 256 consecutive particles (thread t handles particle
 // t of every row, so loads stay coalesced); the QED list keeps ascending
@@ -1892,7 +1930,10 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
 //              whether a pair is possible; if not, the photon is pushed one
 //              step or -- on rebuild steps -- put to sleep for as many steps
 //              as the bound along its straight path allows (its position
-//              advanced in closed form, sfp_repeat_add, when it wakes); if
+//              advanced in closed form, sfp_repeat_add, when it wakes); on
+//              rebuild steps a photon for which a pair is possible but whose
+//              own draws for the next 64 or 16 steps all lie above twice the
+//              bound's probability sleeps as well (sleep_steps_drawn); if
 //              so, the exact Breit-Wheeler front runs, events are ranked in
 //              list order (= index order) and the pairs are appended already
 //              pushed; the converted photon becomes a hole and is never
@@ -2299,6 +2340,9 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
           nst = a.rel_end - a.rel;
         } else if (!hot && rebuild) {
           nst = sleep_steps<FM>(P, FS, a, true, x, y, z, ux, uy, uz, ig);
+        } else if (KIND == KIND_RS && rebuild && pb >= 0.0) {
+          nst = sleep_steps_drawn<FM, MIX>(P, FS, a, x, y, z, ux, uy, uz, ig, pid,
+                                           (uint32_t)sp.tag, lds_env);
         }
         if (sp.push) {
           if (nst == 1) {
