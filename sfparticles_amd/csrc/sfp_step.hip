@@ -939,7 +939,12 @@ struct QDiag {
     tally(&sh[QD_P1], p > 1.0);
     tally(&sh[QD_LIVE], p >= 0x1p-54);
     const float f = (float)p;
-    if (f > 0.0f) atomicMax(&sh[QD_PMAX], __float_as_uint(f));  // positive floats order as bits
+    // positive floats order as bits; the atomic only where f exceeds the
+    // maximum read so far (a stale read is smaller, never larger): after the
+    // first rows of a workgroup it is rare, where one per lane on the same
+    // LDS word would serialise
+    if (f > 0.0f && __float_as_uint(f) > *(volatile uint32_t*)&sh[QD_PMAX])
+      atomicMax(&sh[QD_PMAX], __float_as_uint(f));
   }
   // photon emission: z = (delta / ((1 - delta) chi))^(2/3) > z_max
   __device__ void zcheck(const Plan& P, double chi, double delta) {
