@@ -74,10 +74,10 @@ CPU_WINDOW = 60
 #            (8 B); writes x, y, z (24 B) of every photon it pushes, a 4-byte
 #            wake step per multi-step push, and per conversion the pruned flag
 #            and wake step (5 B).
-#   leptons with the learned sampler: the QED kernel writes a 40-byte record
-#            per emission candidate instead of its state; the emission kernel
-#            reads the record and the state (x, y, z, u: 48 B) of every
-#            candidate and writes its state (48 B).
+#   leptons with the learned sampler: the QED kernel writes an 88-byte record
+#            per emission candidate (the sampler's inputs and the candidate's
+#            state) instead of its state; the emission kernel reads the record
+#            and writes the candidate's state (x, y, z, u: 48 B).
 #   Optical depth: + tau read and written (16 B) per exact QED evaluation.
 #   Per appended particle: x, y, z, u, id and flag (57 B) in each destination.
 def kernel_bytes(photon, phase, st, od=False):
@@ -92,9 +92,9 @@ def kernel_bytes(photon, phase, st, od=False):
         return 49 * st["processed"] + 48 * (st["processed"] - hot) + 4 * hot
     emit = st.get("emit", 0)
     if phase == 2:
-        return (40 + 48 + 48) * emit + 57 * st["spawned"]
+        return (88 + 48) * emit + 57 * st["spawned"]
     if emit:
-        return 60 * hot + 48 * (hot - emit) + 40 * emit + tau
+        return 60 * hot + 48 * (hot - emit) + 88 * emit + tau
     return 60 * hot + 48 * hot + tau + 57 * st["spawned"]
 
 
@@ -348,15 +348,38 @@ def _csv_rows(d, suffix):
     return rows
 
 
+def _by_species(seq):
+    """Names of the per-species kernels tagged with their species: a step
+    launches the lepton kernels for electrons, then for positrons, then the
+    photon kernels, each step opening with k_step_prologue."""
+    out, seen = [], {}
+    for r in seq:
+        name = r[0]
+        if name.startswith("k_step_prologue"):
+            seen = {}
+        elif name.startswith(("k_sweep", "k_hot", "k_emit", "k_defer")):
+            k = seen.get(name, 0)
+            seen[name] = k + 1
+            name = f"{name} [{('e-', 'e+')[min(k, 1)]}]"
+        elif name.startswith(("k_awake", "k_photon")):
+            name = f"{name} [photon]"
+        out.append((name,) + tuple(r[1:]))
+    return out
+
+
 def _windows(seq, steps):
     """Timed windows of a dispatch sequence [(name, ...)]: the dispatches after
-    one k_catchup up to and including the next, when exactly `steps` step
-    prologues lie between them (the settle before the timed steps, the timed
-    steps, the settle that ends them)."""
-    cu = [i for i, r in enumerate(seq) if r[0].startswith("k_catchup")]
+    one settle (k_catchup and the k_catchup_slow after it) up to and
+    including the next, when exactly `steps` step prologues lie between them
+    (the settle before the timed steps, the timed steps, the settle that
+    ends them)."""
+    cu = [i for i, r in enumerate(seq) if r[0] == "k_catchup"]
+
+    def end(i):  # the settle's last dispatch
+        return i + 1 if i + 1 < len(seq) and seq[i + 1][0] == "k_catchup_slow" else i
     out = []
     for a, b in zip(cu, cu[1:]):
-        seg = seq[a + 1:b + 1]
+        seg = seq[end(a) + 1:end(b) + 1]
         if sum(1 for r in seg if r[0].startswith("k_step_prologue")) == steps:
             out.append(seg)
     return out
@@ -374,7 +397,8 @@ def summarize_profile(d):
     waiting."""
     import collections
     out = []
-    line = json.load(open(os.path.join(d, "kt.json")))
+    with open(os.path.join(d, "kt.json")) as fh:  # the bench line: the last JSON line
+        line = json.loads([ln for ln in fh.read().splitlines() if ln.startswith("{")][-1])
     steps = line["steps"]
     paths = ["fp64", "mixed", "mixed+mlp"]
     cfg = line["config"]
@@ -388,8 +412,8 @@ def summarize_profile(d):
     # kernel trace
     kt = _csv_rows(os.path.join(d, "kt"), "kernel_trace.csv")
     kt.sort(key=lambda r: int(r["Start_Timestamp"]))
-    seq = [(_short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
-           for r in kt]
+    seq = _by_species([(_short(r["Kernel_Name"]), int(r["Start_Timestamp"]),
+                        int(r["End_Timestamp"])) for r in kt])
     wins = _windows(seq, steps)
     trace = {}
     for path, seg in zip(paths, wins):
@@ -407,7 +431,7 @@ def summarize_profile(d):
             e = byd.setdefault(int(r["Dispatch_Id"]), [_short(r["Kernel_Name"]),
                                                        collections.defaultdict(float)])
             e[1][r["Counter_Name"]] += float(r["Counter_Value"])
-        disp = [(v[0], v[1]) for _, v in sorted(byd.items())]
+        disp = _by_species([(v[0], v[1]) for _, v in sorted(byd.items())])
         res = {}
         for path, seg in zip(paths, _windows(disp, steps)):
             agg = collections.defaultdict(lambda: collections.defaultdict(list))
@@ -444,10 +468,11 @@ def summarize_profile(d):
                 f"{wall / 1e3:.3f} ms from the first step's start to the last dispatch's end",
                 "",
                 "| kernel | dispatches | mean us | total us | HBM read MB / dispatch | "
-                "HBM written MB / dispatch | wait / wave cycles | busy cycles / dispatch |",
-                "|---|---|---|---|---|---|---|---|"]
+                "HBM written MB / dispatch | wait / wave cycles | VALU active / wave cycles | "
+                "busy cycles / dispatch | VALU insts / dispatch | VMEM read insts / dispatch |",
+                "|---|---|---|---|---|---|---|---|---|---|---|"]
         for name, v in sorted(t.items(), key=lambda kv: -sum(kv[1])):
-            rd = wr = wt = bz = ""
+            rd = wr = wt = va = bz = vi = vm = ""
             f = fetch.get(path, {}).get(name, {}).get("FETCH_SIZE")
             if f and "FETCH_SIZE" in scale:
                 rd = f"{np.mean(f) * 1024 / scale['FETCH_SIZE'] / 1e6:.1f}"
@@ -457,10 +482,16 @@ def summarize_profile(d):
             q = sq.get(path, {}).get(name, {})
             if q.get("SQ_WAVE_CYCLES") and q.get("SQ_WAIT_ANY"):
                 wt = f"{sum(q['SQ_WAIT_ANY']) / max(sum(q['SQ_WAVE_CYCLES']), 1):.2f}"
+            if q.get("SQ_WAVE_CYCLES") and q.get("SQ_ACTIVE_INST_VALU"):
+                va = f"{sum(q['SQ_ACTIVE_INST_VALU']) / max(sum(q['SQ_WAVE_CYCLES']), 1):.2f}"
             if q.get("SQ_BUSY_CYCLES"):
                 bz = f"{np.mean(q['SQ_BUSY_CYCLES']):.3g}"
+            if q.get("SQ_INSTS_VALU"):
+                vi = f"{np.mean(q['SQ_INSTS_VALU']):.3g}"
+            if q.get("SQ_INSTS_VMEM_RD"):
+                vm = f"{np.mean(q['SQ_INSTS_VMEM_RD']):.3g}"
             out.append(f"| `{name}` | {len(v)} | {np.mean(v):.1f} | {sum(v):.0f} | {rd} | {wr} | "
-                       f"{wt} | {bz} |")
+                       f"{wt} | {va} | {bz} | {vi} | {vm} |")
         out.append("")
         if path == head:
             rf = line["roofline"]
@@ -469,16 +500,15 @@ def summarize_profile(d):
             key = next((k for k in kn if rf["kernel"].startswith(k)), None)
             # the roofline kernel serves one species per launch; the trace mean
             # over its dispatches of the species the line names
-            cand = [n for n in t if key and n.startswith(key)]
+            sp_tag = "[photon]" if "(photon)" in rf["kernel"] else (
+                "[e+]" if "(positron)" in rf["kernel"] else "[e-]")
+            cand = [n for n in t if key and n.startswith(key) and n.endswith(sp_tag)]
             if cand:
                 name = max(cand, key=lambda n: sum(t[n]))
                 durs = t[name]
-                if key in ("k_hot", "k_sweep"):
-                    durs = durs[0::2]  # electron launches (electron, positron per step)
                 mean_ms = float(np.mean(durs)) / 1e3
                 ach = rf["alg_bytes_per_launch"] / (mean_ms / 1e3) / 1e9
-                who = ("electron launches (the kernel serves both leptons)"
-                       if key in ("k_hot", "k_sweep") else "all launches")
+                who = "its launches for the species the line names"
                 out += [f"Roofline kernel `{name}`, {who}: trace mean {mean_ms:.4f} ms per "
                         f"launch against the "
                         f"line's {rf['avg_launch_ms']} ms (HIP events); "

@@ -323,8 +323,45 @@ SFP_HD sfp_u32x4 sfp_draw4(uint64_t seed, uint64_t id, uint64_t step, uint32_t s
  * crossings, zeros, subnormal or huge values and non-finite operands take
  * single real additions.  Negative x uses the mirror image (rounding to
  * nearest is symmetric). */
-SFP_HD double sfp_repeat_add(double x, double h, uint32_t n) {
+/* The usual case in one piece: every one of the n additions stays inside
+ * x's binade (and is no tie).  All in FP64 and exact: k = |x| / u and
+ * r = h / u are scalings by a power of two, q = round(r) and the room
+ * k - (2^52 + 1) or (2^53 - 1) - k are integers below 2^53, and so is the
+ * product n |q| whenever it does not exceed the room (a product of integers
+ * that is below 2^53 is exact; one at or above rounds to at least 2^53), so
+ * the test n |q| <= room and the sum k + n q are exact.  Returns 0 when the
+ * case does not apply (the caller then takes the loop below). */
+SFP_HD int sfp_repeat_add_binade(double x, double h, uint32_t n, double* out) {
+  const double ax = x < 0.0 ? -x : x;
+  const int be = (int)(sfp_d2u(ax) >> 52);
+  if (be < 54 || be > 2045) return 0;  /* zero, tiny, huge or non-finite x */
+  const double sc = sfp_u2d((uint64_t)(2098 - be) << 52);  /* 2^(1075 - be) = 1 / u */
+  const double r = (x < 0.0 ? -h : h) * sc;
+  if (!(r < 4503599627370496.0 && r > -4503599627370496.0)) return 0;  /* also NaN h */
+  const double fr = sfp_floor(r);
+  const double f = r - fr;
+  if (f == 0.5) return 0;  /* a tie: the parity of k decides */
+  const double q = f < 0.5 ? fr : fr + 1.0;
+  const double k = ax * sc;  /* the integer significand, in [2^52, 2^53) */
+  if (q == 0.0) {  /* x does not move unless it sits at the binade's bottom */
+    if (r >= 0.0 || k >= 4503599627370497.0) {
+      *out = x;
+      return 1;
+    }
+    return 0;
+  }
+  const double room = q > 0.0 ? 9007199254740991.0 - k : k - 4503599627370497.0;
+  const double aq = q > 0.0 ? q : -q;
+  if (!((double)n * aq <= room)) return 0;
+  const double X = (k + (double)n * q) * sfp_u2d((uint64_t)(be - 52) << 52);
+  *out = x < 0.0 ? -X : X;
+  return 1;
+}
+
+SFP_HD double sfp_repeat_add_closed(double x, double h, uint32_t n) {
   if (n == 0) return x;
+  double y1;
+  if (sfp_repeat_add_binade(x, h, n, &y1)) return y1;
   if (h == 0.0 || !(h - h == 0.0) || !(x - x == 0.0)) {
     // the first addition fixes the sign of a zero / produces inf or NaN;
     // every later one returns its input
@@ -395,6 +432,18 @@ SFP_HD double sfp_repeat_add(double x, double h, uint32_t n) {
     }
   }
   return x;
+}
+
+/* n additions: up to SFP_REPEAT_LOOP of them one by one (the definition;
+ * branch-free, and cheaper than the closed form's control flow for a short
+ * sleep, such as the 2 x 20 of a 20-step run), more in closed form. */
+#define SFP_REPEAT_LOOP 64u
+SFP_HD double sfp_repeat_add(double x, double h, uint32_t n) {
+  if (n <= SFP_REPEAT_LOOP) {
+    for (uint32_t k = 0; k < n; ++k) x = x + h;
+    return x;
+  }
+  return sfp_repeat_add_closed(x, h, n);
 }
 
 #endif /* SFP_MATH_H */

@@ -52,6 +52,12 @@ struct EmitRec {
   uint32_t net;
   float in0, in1;
   double chi, u_tab, delta;
+  // the candidate's state as the mixed step uses it (FP64 position, FP32
+  // momentum and spin), so that the emission kernel reads records in order
+  // instead of gathering scattered store lines
+  double x, y, z;
+  float ux, uy, uz;
+  float sx, sy, sz;
 };
 #define SFP_MAX_SPECIES 8
 
@@ -1596,6 +1602,9 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         r.chi = chi_n;
         r.u_tab = u_tab;
         r.delta = delta;
+        r.x = x; r.y = y; r.z = z;
+        r.ux = fx; r.uy = fy; r.uz = fz;
+        r.sx = (float)sx; r.sy = (float)sy; r.sz = (float)sz;
         sp.emit[rank] = r;
         if (KIND == KIND_OD) sp.tau[i] = tau;
         continue;
@@ -1712,6 +1721,9 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         r.chi = chi_n;
         r.u_tab = u_tab;
         r.delta = delta;
+        r.x = x; r.y = y; r.z = z;
+        r.ux = fx; r.uy = fy; r.uz = fz;
+        r.sx = (float)sx; r.sy = (float)sy; r.sz = (float)sz;
         sp.emit[q] = r;
         if (KIND == KIND_OD) sp.tau[i] = tau;
       } else if (valid) {
@@ -1871,18 +1883,14 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
     const bool valid = k < n_c;
     EmitRec r;
     r.i = 0; r.net = 0; r.in0 = 0.0f; r.in1 = 0.0f; r.chi = 0.0; r.u_tab = 0.0; r.delta = 0.0;
-    double x = 0, y = 0, z = 0, ux = 0, uy = 0, uz = 0, sx = 0, sy = 0, sz = 0;
-    int64_t i = 0;
-    if (valid) {
-      r = sp.emit[k];
-      i = (int64_t)r.i;
-      x = sp.x[i]; y = sp.y[i]; z = sp.z[i];
-      ux = sp.ux[i]; uy = sp.uy[i]; uz = sp.uz[i];
-      if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
-    }
+    r.x = 0.0; r.y = 0.0; r.z = 0.0;
+    r.ux = 0.0f; r.uy = 0.0f; r.uz = 0.0f; r.sx = 0.0f; r.sy = 0.0f; r.sz = 0.0f;
+    if (valid) r = sp.emit[k];
+    const int64_t i = (int64_t)r.i;
+    double x = r.x, y = r.y, z = r.z;
     uint32_t shot = 0;
     const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
-    float fx = (float)ux, fy = (float)uy, fz = (float)uz;
+    float fx = r.ux, fy = r.uy, fz = r.uz;
     float igf = valid ? __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz) : 1.0f;
     double delta = r.delta;
     const bool net = valid && r.net;
@@ -1907,7 +1915,7 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
         fz *= f;
         igf = __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz);
       }
-      float s0 = (float)sx, s1 = (float)sy, s2 = (float)sz;
+      float s0 = r.sx, s1 = r.sy, s2 = r.sz;
       lepton_push32<FM>(P, FS, sp, a, x, y, z, fx, fy, fz, s0, s1, s2, igf);
       sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
       sp.ux[i] = fx; sp.uy[i] = fy; sp.uz[i] = fz;
@@ -2555,55 +2563,129 @@ __device__ inline bool catchup_one(uint32_t w, uint32_t& s0, uint8_t pr, double&
   return true;
 }
 
-// Four consecutive photons per thread, every load issued up front (the wake
-// and since steps, the pruned flags, position and momentum, as 16-byte
-// vectors): one memory latency per group instead of three dependent ones,
-// and the state of photons that need no update is read and dropped (most
-// photons sleep at the end of a run).  A group with an update writes its
-// positions and since steps back whole, as vectors (single-element stores
-// four words apart left the writes uncoalesced: three times the bytes).
-// All arrays are allocated for the store's capacity, and a full group lies
-// below the count.
+// One photon's catch-up when it lags by at most SFP_REPEAT_LOOP additions
+// (one by one) or each coordinate's additions stay inside its binade
+// (sfp_repeat_add_binade): 0 nothing to do, 1 done, 2 the photon needs the
+// general repeated addition (a binade edge, a tie, a coordinate near 0).
+__device__ inline int catchup_binade(uint32_t w, uint32_t& s0, uint8_t pr, double& x, double& y,
+                                     double& z, double ux, double uy, double uz, uint32_t wend,
+                                     double cdt_half) {
+  if (w == WAKE_NEVER || w < wend || s0 >= wend || pr) return 0;
+  const double ig = sfp_inv_gamma_photon(ux, uy, uz);
+  const uint32_t m = 2u * (wend - s0);
+  if (m <= SFP_REPEAT_LOOP) {  // a short lag: the additions one by one
+    const double hx = cdt_half * ig * ux, hy = cdt_half * ig * uy, hz = cdt_half * ig * uz;
+    for (uint32_t k = 0; k < m; ++k) {
+      x = x + hx;
+      y = y + hy;
+      z = z + hz;
+    }
+    s0 = wend;
+    return 1;
+  }
+  double x1, y1, z1;
+  if (!sfp_repeat_add_binade(x, cdt_half * ig * ux, m, &x1) ||
+      !sfp_repeat_add_binade(y, cdt_half * ig * uy, m, &y1) ||
+      !sfp_repeat_add_binade(z, cdt_half * ig * uz, m, &z1))
+    return 2;
+  x = x1; y = y1; z = z1;
+  s0 = wend;
+  return 1;
+}
+
+// Two consecutive photons per thread, every load issued up front (the wake
+// and since steps, the pruned flags, position and momentum, as 8- and
+// 16-byte vectors, so that each load instruction of a wave reads one
+// contiguous stretch); the state of photons that need no update is read and
+// dropped (most photons sleep at the end of a run).  A pair with an update
+// writes its positions and since steps back whole, as vectors.  Lanes try the
+// one-binade closed form only, which keeps the kernel at the speed of its
+// memory traffic; the photons that need the general repeated addition (a
+// binade edge, a tie, a coordinate near 0: a few per cent) are marked in a
+// bit mask, two 64-bit ballots per wave and iteration (the wave's first
+// and second photons of its 64 pairs), written by the wave's first lane --
+// no atomics, and every entry below the count is rewritten by each
+// catch-up, so none is cleared -- and k_catchup_slow advances them.  (With
+// the general loop inlined as the fallback, the few lanes that took it held
+// whole waves in it and the kernel ran at half the speed; a list with one
+// atomic per wave on a shared counter was slower still.)  The masks live in
+// the visit list, which is unused between run calls and holds 8 bytes per
+// photon of capacity against the masks' 1/8.  All arrays are allocated for
+// the store's capacity, and a full pair lies below the count.
 __global__ void __launch_bounds__(256) k_catchup(const Plan* __restrict__ plan, int s,
                                                  uint32_t wend, double cdt_half) {
   const SpeciesDev& sp = plan->sp[s];
   const int64_t n = sp.cnt->n;
-  const int64_t ngrp = (n + 3) / 4;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngrp;
+  const int64_t npair = (n + 1) / 2;
+  uint4* masks = (uint4*)sp.hot;
+  // g of a wave's first lane is a multiple of 64 (workgroups of 256, a
+  // stride of whole workgroups): the wave covers pairs g .. g + 63
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npair;
        g += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i0 = 4 * g;
-    if (i0 + 4 <= n) {
-      const uint4 w = *(const uint4*)(sp.wake + i0);
-      uint4 s0 = *(const uint4*)(sp.since + i0);
-      const uchar4 pr = *(const uchar4*)(sp.pruned + i0);
-      double2 x0 = *(const double2*)(sp.x + i0), x1 = *(const double2*)(sp.x + i0 + 2);
-      double2 y0 = *(const double2*)(sp.y + i0), y1 = *(const double2*)(sp.y + i0 + 2);
-      double2 z0 = *(const double2*)(sp.z + i0), z1 = *(const double2*)(sp.z + i0 + 2);
-      const double2 a0 = *(const double2*)(sp.ux + i0), a1 = *(const double2*)(sp.ux + i0 + 2);
-      const double2 b0 = *(const double2*)(sp.uy + i0), b1 = *(const double2*)(sp.uy + i0 + 2);
-      const double2 c0 = *(const double2*)(sp.uz + i0), c1 = *(const double2*)(sp.uz + i0 + 2);
-      bool any = catchup_one(w.x, s0.x, pr.x, x0.x, y0.x, z0.x, a0.x, b0.x, c0.x, wend, cdt_half);
-      any |= catchup_one(w.y, s0.y, pr.y, x0.y, y0.y, z0.y, a0.y, b0.y, c0.y, wend, cdt_half);
-      any |= catchup_one(w.z, s0.z, pr.z, x1.x, y1.x, z1.x, a1.x, b1.x, c1.x, wend, cdt_half);
-      any |= catchup_one(w.w, s0.w, pr.w, x1.y, y1.y, z1.y, a1.y, b1.y, c1.y, wend, cdt_half);
-      if (any) {
-        *(double2*)(sp.x + i0) = x0; *(double2*)(sp.x + i0 + 2) = x1;
-        *(double2*)(sp.y + i0) = y0; *(double2*)(sp.y + i0 + 2) = y1;
-        *(double2*)(sp.z + i0) = z0; *(double2*)(sp.z + i0 + 2) = z1;
-        *(uint4*)(sp.since + i0) = s0;
+    const int64_t i0 = 2 * g;
+    int r0 = 0, r1 = 0;
+    if (i0 + 2 <= n) {
+      const uint2 w = *(const uint2*)(sp.wake + i0);
+      uint2 s0 = *(const uint2*)(sp.since + i0);
+      const uchar2 pr = *(const uchar2*)(sp.pruned + i0);
+      double2 x = *(const double2*)(sp.x + i0);
+      double2 y = *(const double2*)(sp.y + i0);
+      double2 z = *(const double2*)(sp.z + i0);
+      const double2 a = *(const double2*)(sp.ux + i0);
+      const double2 b = *(const double2*)(sp.uy + i0);
+      const double2 c = *(const double2*)(sp.uz + i0);
+      r0 = catchup_binade(w.x, s0.x, pr.x, x.x, y.x, z.x, a.x, b.x, c.x, wend, cdt_half);
+      r1 = catchup_binade(w.y, s0.y, pr.y, x.y, y.y, z.y, a.y, b.y, c.y, wend, cdt_half);
+      if (r0 == 1 || r1 == 1) {
+        *(double2*)(sp.x + i0) = x;
+        *(double2*)(sp.y + i0) = y;
+        *(double2*)(sp.z + i0) = z;
+        *(uint2*)(sp.since + i0) = s0;
       }
-    } else {
-      for (int64_t i = i0; i < n; ++i) {
-        uint32_t s0 = sp.since[i];
-        double x = sp.x[i], y = sp.y[i], z = sp.z[i];
-        if (catchup_one(sp.wake[i], s0, sp.pruned[i], x, y, z, sp.ux[i], sp.uy[i], sp.uz[i],
-                        wend, cdt_half)) {
-          sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
-          sp.since[i] = s0;
-        }
-      }
+    }  // else: the last photon of an odd count, left to k_catchup_slow
+    const uint64_t m0 = __ballot(r0 == 2), m1 = __ballot(r1 == 2);
+    if (wave_lane() == 0)
+      masks[g >> 6] = make_uint4((uint32_t)m0, (uint32_t)(m0 >> 32), (uint32_t)m1,
+                                 (uint32_t)(m1 >> 32));
+  }
+}
+
+__device__ inline void catchup_general(const SpeciesDev& sp, int64_t i, uint32_t wend,
+                                       double cdt_half) {
+  uint32_t s1 = sp.since[i];
+  double x = sp.x[i], y = sp.y[i], z = sp.z[i];
+  if (catchup_one(sp.wake[i], s1, sp.pruned[i], x, y, z, sp.ux[i], sp.uy[i], sp.uz[i], wend,
+                  cdt_half)) {
+    sp.x[i] = x; sp.y[i] = y; sp.z[i] = z;
+    sp.since[i] = s1;
+  }
+}
+
+// The photons k_catchup marked, with the general repeated addition, and the
+// last photon of an odd count.
+__global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ plan, int s,
+                                                      uint32_t wend, double cdt_half) {
+  const SpeciesDev& sp = plan->sp[s];
+  const int64_t n = sp.cnt->n;
+  const int64_t nm = ((n + 1) / 2 + 63) / 64;
+  const uint4* masks = (const uint4*)sp.hot;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t j = t0; j < nm; j += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 mk = masks[j];
+    uint64_t m0 = (uint64_t)mk.x | ((uint64_t)mk.y << 32);
+    uint64_t m1 = (uint64_t)mk.z | ((uint64_t)mk.w << 32);
+    while (m0) {
+      const int b = __ffsll((unsigned long long)m0) - 1;
+      m0 &= m0 - 1;
+      catchup_general(sp, 2 * (64 * j + b), wend, cdt_half);
+    }
+    while (m1) {
+      const int b = __ffsll((unsigned long long)m1) - 1;
+      m1 &= m1 - 1;
+      catchup_general(sp, 2 * (64 * j + b) + 1, wend, cdt_half);
     }
   }
+  if (t0 == 0 && (n & 1)) catchup_general(sp, n - 1, wend, cdt_half);
 }
 
 // ------------------------------------------------------------- dispatch ----
@@ -2810,8 +2892,9 @@ static int sim_refresh_plan(Sim* sim) {
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
       // photons keep their visit list from step to step: grown inside a run,
       // the list keeps its entries; the photons' old cache positions (oldk)
-      // follow the list in the same block
-      int64_t want = std::max<int64_t>(st->cap, 1);
+      // follow the list in the same block; at least 64 entries, so that the
+      // catch-up's masks (16 bytes per 128 photons) fit in a small store's
+      int64_t want = std::max<int64_t>(st->cap, 64);
       uint32_t* h = nullptr;
       HIPCHK(hipMalloc(&h, sizeof(uint32_t) * 2 * want));
       if (sim->hot_mem[s]) {
@@ -3415,6 +3498,9 @@ static int sim_settle(Sim* sim) {
   for (int s = 0; s < sim->nsp; ++s) {
     if (!sim->wake_mem[s] || !sim->qed[s] || !sim->plan.sp[s].push) continue;
     hipLaunchKernelGGL(k_catchup, dim3(sim->cus * 8), dim3(256), 0, sim->stream,
+                       (const Plan*)sim->plan_dev, s, sim->lag_wend, cdt_half);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_catchup_slow, dim3(sim->cus * 2), dim3(256), 0, sim->stream,
                        (const Plan*)sim->plan_dev, s, sim->lag_wend, cdt_half);
     HIPCHK(hipGetLastError());
     Store* st = sim->stores[s];

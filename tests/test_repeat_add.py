@@ -1,6 +1,7 @@
-"""The closed form for n repeated floating-point additions (sfp_repeat_add,
-used by the step to push a photon several steps at once) against the plain
-loop of n additions, bit for bit, on the host build of the same header.
+"""The closed form for n repeated floating-point additions
+(sfp_repeat_add_closed, used by the step, through sfp_repeat_add, to push a
+photon several steps at once) against the plain loop of n additions, bit for
+bit, on the host build of the same header.
 
 Cases: magnitudes across many binades, both signs, binade and zero
 crossings, increments that are exact ties at the current ulp (both parities),
@@ -21,6 +22,9 @@ SRC = r"""
 #include <stdint.h>
 #include "sfp_math.h"
 void closed(const double* x, const double* h, const uint32_t* n, double* out, int64_t cnt) {
+  for (int64_t i = 0; i < cnt; ++i) out[i] = sfp_repeat_add_closed(x[i], h[i], n[i]);
+}
+void public_(const double* x, const double* h, const uint32_t* n, double* out, int64_t cnt) {
   for (int64_t i = 0; i < cnt; ++i) out[i] = sfp_repeat_add(x[i], h[i], n[i]);
 }
 void naive(const double* x, const double* h, const uint32_t* n, double* out, int64_t cnt) {
@@ -43,7 +47,7 @@ def lib():
     subprocess.check_call(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off",
                            "-I", os.path.join(ROOT, "sfparticles_amd", "csrc"), "-o", so, c])
     L = C.CDLL(so)
-    for fn in (L.closed, L.naive):
+    for fn in (L.closed, L.naive, L.public_):
         fn.argtypes = [C.c_void_p] * 4 + [C.c_int64]
     return L
 
@@ -54,16 +58,21 @@ def _run(L, x, h, n):
     n = np.ascontiguousarray(n, np.uint32)
     a = np.empty_like(x)
     b = np.empty_like(x)
+    c = np.empty_like(x)
     L.closed(x.ctypes.data, h.ctypes.data, n.ctypes.data, a.ctypes.data, x.size)
     L.naive(x.ctypes.data, h.ctypes.data, n.ctypes.data, b.ctypes.data, x.size)
-    return a, b
+    L.public_(x.ctypes.data, h.ctypes.data, n.ctypes.data, c.ctypes.data, x.size)
+    return a, b, c
 
 
 def _check(L, x, h, n):
-    a, b = _run(L, x, h, n)
-    same = (a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))
-    bad = np.nonzero(~same)[0]
-    assert bad.size == 0, [(x[i], h[i], int(n[i]), a[i], b[i]) for i in bad[:5]]
+    """The closed form and the public entry (one by one up to 64 additions,
+    the closed form beyond) against the plain loop."""
+    a, b, c = _run(L, x, h, n)
+    for r in (a, c):
+        same = (r.view(np.uint64) == b.view(np.uint64)) | (np.isnan(r) & np.isnan(b))
+        bad = np.nonzero(~same)[0]
+        assert bad.size == 0, [(x[i], h[i], int(n[i]), r[i], b[i]) for i in bad[:5]]
 
 
 def test_random_magnitudes(lib):
@@ -146,3 +155,45 @@ def test_special_values(lib):
                   1e292, np.nan])
     for n in (0, 1, 2, 7, 1000):
         _check(lib, x, h, np.full(x.size, n))
+
+
+def test_steps_that_just_fit_in_the_binade(lib):
+    """Integer increments of q ulps for which n q equals the room left in the
+    binade, exceeds it by one, or falls one short: the test that all n steps
+    stay inside the binade, with q below and above 2^32 ulps (n q = room + 1
+    lands exactly on the next binade's end, which is still exact)."""
+    rng = np.random.Generator(np.random.PCG64(6))
+    xs, hs, ns = [], [], []
+    for _ in range(3000):
+        q = int(rng.choice([1, 3, 1000, 2 ** 20 + 7, 2 ** 33 + 5, 2 ** 40 + 3]))
+        n = int(rng.integers(1, 3000))
+        if n * q >= 2 ** 52 - 2:
+            n = (2 ** 52 - 2) // q
+        slack = int(rng.integers(-3, 2))
+        e = int(rng.integers(-30, 0))
+        if rng.random() < 0.5:  # upwards: room = 2^53 - 1 - k
+            k = 2 ** 53 - 1 - n * q - slack
+            sign = 1.0
+        else:  # downwards: room = k - 2^52 - 1
+            k = 2 ** 52 + 1 + n * q + slack
+            sign = -1.0
+        k = min(max(k, 2 ** 52), 2 ** 53 - 1)
+        xs.append(np.ldexp(float(k), e - 52))
+        hs.append(sign * np.ldexp(float(q), e - 52))
+        ns.append(n)
+    x, h, n = np.array(xs), np.array(hs), np.array(ns)
+    flip = rng.random(x.size) < 0.5  # negative x mirrors the direction
+    _check(lib, np.where(flip, -x, x), np.where(flip, -h, h), n)
+
+
+def test_binade_bottom_small_decrements(lib):
+    """x exactly a power of two and h a decrement of less than half an ulp of
+    x's binade: the first addition lands in the binade below, where the ulp
+    is half as large, unless |h| is at most a quarter of x's ulp."""
+    rng = np.random.Generator(np.random.PCG64(7))
+    m = 4000
+    x = rng.choice([-1.0, 1.0], m) * np.ldexp(1.0, rng.integers(-40, 10, m))
+    u = np.spacing(np.abs(x))
+    frac = rng.choice([0.1, 0.25, 0.3, 0.45, 0.49, 0.5, 0.7], m)
+    h = -np.sign(x) * frac * u
+    _check(lib, x, h, rng.integers(1, 300, m))
