@@ -47,6 +47,23 @@ static inline int op_grid(int64_t n) {
 
 __device__ inline int wave_lane() { return (int)__lane_id(); }
 
+// Device memory pointers held in structs the kernels read (the step plan):
+// in the device pass they carry the global address space, so that accesses
+// through them compile to global loads and stores.  A pointer read from
+// memory is otherwise generic, and generic (flat) accesses also count against
+// the LDS counter, so every LDS table lookup would wait for the state loads
+// in flight.  The host pass sees plain pointers (same size and layout).
+#ifdef __HIP_DEVICE_COMPILE__
+#define SFP_G __attribute__((address_space(1)))
+#else
+#define SFP_G
+#endif
+// a device allocation as the plan holds it (host code)
+template <class T>
+static inline SFP_G T* gptr(T* p) {
+  return (SFP_G T*)p;
+}
+
 // QED tables in device memory
 struct Tables {
   sfp_rs_node* rs_nodes = nullptr;  // [z_N-1]

@@ -64,24 +64,24 @@ struct EmitRec {
 enum { KIND_RS = 0, KIND_OD = 1 };
 
 struct SpeciesDev {
-  double *x, *y, *z, *ux, *uy, *uz;
-  uint64_t* id;
-  uint8_t* pruned;
-  double* tau;
-  double *sx, *sy, *sz;
-  uint32_t* hot;   // this step's QED list (ascending particle indices)
-  uint32_t* wake;  // photons: step of the next visit (multi-step push), or null
-  uint32_t* since;  // QED photons asleep: the step whose position the store holds
-  uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
-  EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
+  SFP_G double *x, *y, *z, *ux, *uy, *uz;
+  SFP_G uint64_t* id;
+  SFP_G uint8_t* pruned;
+  SFP_G double* tau;
+  SFP_G double *sx, *sy, *sz;
+  SFP_G uint32_t* hot;   // this step's QED list (ascending particle indices)
+  SFP_G uint32_t* wake;  // photons: step of the next visit (multi-step push), or null
+  SFP_G uint32_t* since;  // QED photons asleep: the step whose position the store holds
+  SFP_G uint32_t* shot;  // ensemble: the shot each particle belongs to (selects its field), or null
+  SFP_G EmitRec* emit;   // leptons with the learned sampler: this step's emission candidates
   // photons: state of the first `ccap` entries of the visit list, by list
   // position, for the window the list is kept; two sets (win_cache), this
   // window's and the last one's (see k_photon)
-  double* cache;
+  SFP_G double* cache;
   int64_t ccap;
-  uint32_t* slot;  // photons: position in the visit list it was last put on
-  uint32_t* oldk;  // photons, per list entry at a rebuild: position in the last window's cache, or none
-  Counts* cnt;
+  SFP_G uint32_t* slot;  // photons: position in the visit list it was last put on
+  SFP_G uint32_t* oldk;  // photons, per list entry at a rebuild: position in the last window's cache, or none
+  SFP_G Counts* cnt;
   int64_t cap;
   double q, ae;
   int32_t rr, push, spin, photon, tag, qed;
@@ -95,16 +95,16 @@ struct SpeciesDev {
 // wherever awake photons lie sparse among sleeping ones.
 static constexpr int WIN_WORDS = 8;
 struct WinCache {
-  double* b;
+  SFP_G double* b;
   int64_t n;
-  __device__ double& x(int64_t k) const { return b[k]; }
-  __device__ double& y(int64_t k) const { return b[n + k]; }
-  __device__ double& z(int64_t k) const { return b[2 * n + k]; }
-  __device__ double& ux(int64_t k) const { return b[3 * n + k]; }
-  __device__ double& uy(int64_t k) const { return b[4 * n + k]; }
-  __device__ double& uz(int64_t k) const { return b[5 * n + k]; }
-  __device__ uint32_t& wake(int64_t k) const { return *(uint32_t*)(b + 6 * n + k); }
-  __device__ uint64_t& id(int64_t k) const { return *(uint64_t*)(b + 7 * n + k); }
+  __device__ SFP_G double& x(int64_t k) const { return b[k]; }
+  __device__ SFP_G double& y(int64_t k) const { return b[n + k]; }
+  __device__ SFP_G double& z(int64_t k) const { return b[2 * n + k]; }
+  __device__ SFP_G double& ux(int64_t k) const { return b[3 * n + k]; }
+  __device__ SFP_G double& uy(int64_t k) const { return b[4 * n + k]; }
+  __device__ SFP_G double& uz(int64_t k) const { return b[5 * n + k]; }
+  __device__ SFP_G uint32_t& wake(int64_t k) const { return *(SFP_G uint32_t*)(b + 6 * n + k); }
+  __device__ SFP_G uint64_t& id(int64_t k) const { return *(SFP_G uint64_t*)(b + 7 * n + k); }
 };
 __device__ inline WinCache win_cache(const SpeciesDev& sp, uint32_t par) {
   return WinCache{sp.cache + (int64_t)(par & 1u) * WIN_WORDS * sp.ccap, sp.ccap};
@@ -127,8 +127,8 @@ struct Halt {
 // x, y, z, ux, uy, uz of particles [0, n) of one species go to row r of buf
 // ([max_rows][n][6]) and the time to t[r].
 struct Recorder {
-  double* buf;
-  double* t;
+  SFP_G double* buf;
+  SFP_G double* t;
   int64_t n, every, max_rows;
   uint64_t step0;
   int32_t species, on;
@@ -136,28 +136,28 @@ struct Recorder {
 
 struct Plan {
   SpeciesDev sp[SFP_MAX_SPECIES];
-  ScanState* scan[SFP_MAX_SPECIES][NPH];  // [species][0 sweep list, 1 event ranks, 2 emissions]
-  uint64_t* flags[SFP_MAX_SPECIES][NPH];
+  SFP_G ScanState* scan[SFP_MAX_SPECIES][NPH];  // [species][0 sweep list, 1 event ranks, 2 emissions]
+  SFP_G uint64_t* flags[SFP_MAX_SPECIES][NPH];
   sfp_fieldset fields;
-  const sfp_rs_node* rs_nodes;
-  const sfp_rs_node32* rs_nodes32;  // the same nodes in FP32 (mixed precision)
-  const double2* rs_env;  // rate envelopes (Tables::rs_env)
-  const double2* rs_ai;        // spin-resolved emission: Ai nodes, FP64 and FP32,
-  const float2* rs_ai32;       // and the envelopes with the spin term
-  const double2* rs_env_spin;
+  const SFP_G sfp_rs_node* rs_nodes;
+  const SFP_G sfp_rs_node32* rs_nodes32;  // the same nodes in FP32 (mixed precision)
+  const SFP_G double2* rs_env;  // rate envelopes (Tables::rs_env)
+  const SFP_G double2* rs_ai;        // spin-resolved emission: Ai nodes, FP64 and FP32,
+  const SFP_G float2* rs_ai32;       // and the envelopes with the spin term
+  const SFP_G double2* rs_env_spin;
   int32_t spin_emission;       // leptons with spin use the spin-resolved rate (RS)
   double rate_unit;       // alpha m c^2 / hbar
   int64_t z_N;
   double z_min, z_max, z_delta;
-  const double2* od_photon_nodes;
-  const double2* od_pair_nodes;
+  const SFP_G double2* od_photon_nodes;
+  const SFP_G double2* od_pair_nodes;
   sfp_od_dev od;
   double threshold;
   double chi_coef;  // e hbar / (m^2 c^3)
   double zmax15;    // z_max^1.5: rate is 0 unless delta/(1-delta) <= zmax15 chi (photon)
   uint64_t seed;
-  Halt* halt;
-  const sfp_fieldset* shot_fields;  // ensemble: one field set per shot, or null
+  SFP_G Halt* halt;
+  const SFP_G sfp_fieldset* shot_fields;  // ensemble: one field set per shot, or null
   uint32_t nshots;
   const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
   const sfp_mlp_bins* net_bins;  // its per-bin table fallback
@@ -2859,22 +2859,22 @@ static int sim_refresh_plan(Sim* sim) {
   for (int s = 0; s < sim->nsp; ++s) {
     Store* st = sim->stores[s];
     SpeciesDev& d = P.sp[s];
-    d.x = (double*)st->p[SFP_X];
-    d.y = (double*)st->p[SFP_Y];
-    d.z = (double*)st->p[SFP_Z];
-    d.ux = (double*)st->p[SFP_UX];
-    d.uy = (double*)st->p[SFP_UY];
-    d.uz = (double*)st->p[SFP_UZ];
-    d.id = (uint64_t*)st->p[SFP_ID];
-    d.pruned = (uint8_t*)st->p[SFP_PRUNED];
-    d.tau = (double*)st->p[SFP_TAU];
-    d.sx = (double*)st->p[SFP_SX];
-    d.sy = (double*)st->p[SFP_SY];
-    d.sz = (double*)st->p[SFP_SZ];
-    d.cnt = st->cnt;
+    d.x = gptr((double*)st->p[SFP_X]);
+    d.y = gptr((double*)st->p[SFP_Y]);
+    d.z = gptr((double*)st->p[SFP_Z]);
+    d.ux = gptr((double*)st->p[SFP_UX]);
+    d.uy = gptr((double*)st->p[SFP_UY]);
+    d.uz = gptr((double*)st->p[SFP_UZ]);
+    d.id = gptr((uint64_t*)st->p[SFP_ID]);
+    d.pruned = gptr((uint8_t*)st->p[SFP_PRUNED]);
+    d.tau = gptr((double*)st->p[SFP_TAU]);
+    d.sx = gptr((double*)st->p[SFP_SX]);
+    d.sy = gptr((double*)st->p[SFP_SY]);
+    d.sz = gptr((double*)st->p[SFP_SZ]);
+    d.cnt = gptr(st->cnt);
     d.cap = st->cap;
     // shots take effect once their fields are set
-    d.shot = sim->shot_fields ? (uint32_t*)st->p[SFP_SHOT] : nullptr;
+    d.shot = gptr(sim->shot_fields ? (uint32_t*)st->p[SFP_SHOT] : nullptr);
     if (st->cap > (int64_t)UINT32_MAX)
       return set_err(SFP_E_ARG, "species %d: capacity above 2^32 particles", s);
     for (int ph = 0; ph < NPH; ++ph) {
@@ -2886,8 +2886,8 @@ static int sim_refresh_plan(Sim* sim) {
         HIPCHK(hipMemset(sim->flags[ph][s], 0, sizeof(uint64_t) * want));
         sim->flags_cap[ph][s] = want;
       }
-      P.flags[s][ph] = sim->flags[ph][s];
-      P.scan[s][ph] = sim->scan_mem + NPH * s + ph;
+      P.flags[s][ph] = gptr(sim->flags[ph][s]);
+      P.scan[s][ph] = gptr(sim->scan_mem + NPH * s + ph);
     }
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
       // photons keep their visit list from step to step: grown inside a run,
@@ -2907,8 +2907,8 @@ static int sim_refresh_plan(Sim* sim) {
       sim->hot_mem[s] = h;
       sim->hot_cap[s] = want;
     }
-    d.hot = sim->hot_mem[s];
-    d.oldk = sim->hot_mem[s] ? sim->hot_mem[s] + sim->hot_cap[s] : nullptr;
+    d.hot = gptr(sim->hot_mem[s]);
+    d.oldk = gptr(sim->hot_mem[s] ? sim->hot_mem[s] + sim->hot_cap[s] : nullptr);
     if (sim->emit[s] && st->cap > sim->emit_cap[s]) {
       // one step's emission candidates: written by k_hot, read by k_emit
       if (sim->emit_mem[s]) HIPCHK(hipFree(sim->emit_mem[s]));
@@ -2917,7 +2917,7 @@ static int sim_refresh_plan(Sim* sim) {
       HIPCHK(hipMalloc(&sim->emit_mem[s], sizeof(EmitRec) * want));
       sim->emit_cap[s] = want;
     }
-    d.emit = sim->emit[s] ? sim->emit_mem[s] : nullptr;
+    d.emit = gptr(sim->emit[s] ? sim->emit_mem[s] : nullptr);
     if (d.photon && st->cap > sim->wake_cap[s]) {
       // grown inside a run: sleeping photons keep their wake steps; whole
       // tiles of k_awake, so that its vector loads stay inside the array
@@ -2945,13 +2945,13 @@ static int sim_refresh_plan(Sim* sim) {
       sim->since_mem[s] = sn;
       sim->wake_cap[s] = want;
     }
-    d.wake = sim->wake_mem[s];
-    d.since = sim->since_mem[s];
-    d.slot = sim->since_mem[s] ? sim->since_mem[s] + sim->wake_cap[s] : nullptr;
+    d.wake = gptr(sim->wake_mem[s]);
+    d.since = gptr(sim->since_mem[s]);
+    d.slot = gptr(sim->since_mem[s] ? sim->since_mem[s] + sim->wake_cap[s] : nullptr);
     {
       const int64_t cc = sim->cache_cap[s];
       double* c = (double*)sim->cache_mem[s];
-      d.cache = c;
+      d.cache = gptr(c);
       d.ccap = c ? cc : 0;
     }
   }
@@ -3088,12 +3088,12 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
       return set_err(SFP_E_ARG, "rejection-sampling tables required");
     if (sim->tables->z_N - 1 > 1023)
       return set_err(SFP_E_ARG, "rejection-sampling table larger than 1024 nodes");
-    P.rs_nodes = sim->tables->rs_nodes;
-    P.rs_nodes32 = (const sfp_rs_node32*)sim->tables->rs_nodes32;
-    P.rs_env = sim->tables->rs_env;
-    P.rs_ai = sim->tables->rs_ai;
-    P.rs_ai32 = sim->tables->rs_ai32;
-    P.rs_env_spin = sim->tables->rs_env_spin;
+    P.rs_nodes = gptr(sim->tables->rs_nodes);
+    P.rs_nodes32 = gptr((const sfp_rs_node32*)sim->tables->rs_nodes32);
+    P.rs_env = gptr(sim->tables->rs_env);
+    P.rs_ai = gptr(sim->tables->rs_ai);
+    P.rs_ai32 = gptr(sim->tables->rs_ai32);
+    P.rs_env_spin = gptr(sim->tables->rs_env_spin);
     P.rate_unit = SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR;
     P.z_N = sim->tables->z_N;
     P.z_min = sim->tables->z_min;
@@ -3117,8 +3117,8 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   if (sim->tables && sim->tables->has_od && sim->tables->od.chi_N - 1 <= 255) {
     // (also the learned sampler's total rate and fallback under rejection sampling)
     P.od = sim->tables->od;
-    P.od_photon_nodes = sim->tables->od_photon_nodes;
-    P.od_pair_nodes = sim->tables->od_pair_nodes;
+    P.od_photon_nodes = gptr(sim->tables->od_photon_nodes);
+    P.od_pair_nodes = gptr(sim->tables->od_pair_nodes);
   }
   HIPCHK(hipMalloc(&sim->plan_dev, sizeof(Plan)));
   // default priority; a blocking stream, so the synchronous copies of the
@@ -3132,7 +3132,7 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   HIPCHK(hipMemset(sim->halt, 0, sizeof(Halt)));
   HIPCHK(hipHostMalloc((void**)&sim->halt_host, sizeof(Halt), 0));
   HIPCHK(hipHostMalloc((void**)&sim->counts_host, sizeof(Counts) * nspecies, 0));
-  P.halt = sim->halt;
+  P.halt = gptr(sim->halt);
   for (int ph = 0; ph < NPH; ++ph) {
     sim->flags[ph].assign(nspecies, nullptr);
     sim->flags_cap[ph].assign(nspecies, 0);
@@ -3231,7 +3231,7 @@ extern "C" int sfp_sim_set_shots(void* p, const sfp_field* terms, const int32_t*
   HIPCHK(hipMemcpy(sim->shot_fields, fs.data(), sizeof(sfp_fieldset) * nshots,
                    hipMemcpyHostToDevice));
   sim->nshots = nshots;
-  sim->plan.shot_fields = sim->shot_fields;
+  sim->plan.shot_fields = gptr(sim->shot_fields);
   sim->plan.nshots = (uint32_t)nshots;
   // one kernel instance for all shots: the common mode, or the general one
   sim->fm = mode == SFP_FM_ANY ? SFP_FM_ANY : (sfp_field_mode(&sim->plan.fields) == mode ? mode
