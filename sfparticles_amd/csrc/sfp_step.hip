@@ -325,9 +325,14 @@ __device__ inline uint32_t tile_events(const uint64_t (*mask)[WAVES], int rows, 
 
 // The field set of particle i: its shot's in an ensemble (indices at or above
 // the number of shots take the last shot), else the common one.
+// The field set of particle i: its shot's in an ensemble (SH), else the
+// simulation's, the same for every lane (its constants load into scalar
+// registers once per wave; per lane they took a vector load each, and the
+// lepton QED kernel ran 17 % slower).
+template <bool SH>
 __device__ inline const sfp_fieldset* fields_of(const Plan& P, const SpeciesDev& sp, int64_t i,
                                                uint32_t& shot) {
-  if (sp.shot) {
+  if (SH) {
     shot = min(sp.shot[i], P.nshots - 1u);
     return &P.shot_fields[shot];
   }
@@ -649,7 +654,7 @@ __device__ inline uint32_t sleep_steps_drawn(const Plan& P, const sfp_fieldset* 
 // (104 and 203 VGPRs), and 6 or 4 waves ran slower.
 static constexpr int ROWS = 32;
 
-template <bool PHOTON, bool QED, int FM, bool MIX>
+template <bool PHOTON, bool QED, int FM, bool MIX, bool SH>
 __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
         int rmax) {
   const StepArgs a = *ap;
@@ -787,7 +792,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
         proc += 1;
       }
       uint32_t shot = 0;
-      const sfp_fieldset* FS = active ? fields_of(P, sp, i, shot) : &P.fields;
+      const sfp_fieldset* FS = active ? fields_of<SH>(P, sp, i, shot) : &P.fields;
       bool hot = false;
       if (QED && active && sp.push) {
         // chi <= (e hbar/(m^2 c^3)) gamma (|E| + c|B|) (gamma >= |u|); the FP32
@@ -1398,7 +1403,7 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // ran faster (a 1e6-electron focused-pulse collision: 3.2e10 -> 3.5e10
 // particle-steps/s).  The LDS of a workgroup (37-50 KB) allows at most 4
 // workgroups per CU.
-template <int KIND, int FM, bool MIX, int NETK>
+template <int KIND, int FM, bool MIX, int NETK, bool SH>
 __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
       int rmax) {
   const StepArgs a = *ap;
@@ -1506,7 +1511,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
                      ? (dense ? (uint32_t)(k + 2 * TILE) : sp.hot[k + 2 * TILE]) : 0u;
       }
       uint32_t shot = 0;
-      const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
+      const sfp_fieldset* FS = valid ? fields_of<SH>(P, sp, i, shot) : &P.fields;
       float fx = (float)ux, fy = (float)uy, fz = (float)uz;
       float igf = 1.0f;
       double ig = 1.0;
@@ -1577,7 +1582,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
       if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       const uint64_t pid = sp.id[i];
       uint32_t shot = 0;
-      const sfp_fieldset* FS = fields_of(P, sp, i, shot);
+      const sfp_fieldset* FS = fields_of<SH>(P, sp, i, shot);
       float fx = (float)ux, fy = (float)uy, fz = (float)uz;
       float igf = 1.0f;
       double ig = 1.0;
@@ -1684,7 +1689,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
       pid = sp.id[i];
     }
     uint32_t shot = 0;
-    const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
+    const sfp_fieldset* FS = valid ? fields_of<SH>(P, sp, i, shot) : &P.fields;
     float fx = (float)ux, fy = (float)uy, fz = (float)uz;
     float igf = 1.0f;
     double ig = 1.0;
@@ -1838,7 +1843,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
 // 227-232), the stable rank and append of the photon, then recoil
 // (cpu.py:78-88) and push of the lepton.  The room for the appends was
 // checked by k_hot (one per listed particle).
-template <int KIND, int FM>
+template <int KIND, int FM, bool SH>
 __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, int s,
                                                const StepArgs* __restrict__ ap, int rmax) {
   const StepArgs a = *ap;
@@ -1889,7 +1894,7 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
     const int64_t i = (int64_t)r.i;
     double x = r.x, y = r.y, z = r.z;
     uint32_t shot = 0;
-    const sfp_fieldset* FS = valid ? fields_of(P, sp, i, shot) : &P.fields;
+    const sfp_fieldset* FS = valid ? fields_of<SH>(P, sp, i, shot) : &P.fields;
     float fx = r.ux, fy = r.uy, fz = r.uz;
     float igf = valid ? __frsqrt_rn(1.0f + fx * fx + fy * fy + fz * fz) : 1.0f;
     double delta = r.delta;
@@ -2118,7 +2123,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
 // its momentum, positron with 1 - delta (cpu.py:142-191), at store positions
 // posA / posB with ids idA / idB; the photon becomes a hole (cpu.py:189-190),
 // not pushed.
-template <int KIND, int FM, bool MIX, typename Node>
+template <int KIND, int FM, bool MIX, bool SH, typename Node>
 __device__ inline void photon_convert(const Plan& P, const SpeciesDev& sp, const StepArgs& a,
                                       const WinCache& C, const Node* lds_rs,
                                       const double2* lds_od, bool rebuild, int64_t k,
@@ -2140,7 +2145,7 @@ __device__ inline void photon_convert(const Plan& P, const SpeciesDev& sp, const
   const double ig = sfp_inv_gamma_photon(ux, uy, uz);
   if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
   uint32_t shot = 0;
-  const sfp_fieldset* FS = fields_of(P, sp, i, shot);
+  const sfp_fieldset* FS = fields_of<SH>(P, sp, i, shot);
   double tau = KIND == KIND_OD ? sp.tau[i] : 0.0;
   double delta = 0.0;
   if (MIX) {
@@ -2168,7 +2173,7 @@ __device__ inline void photon_convert(const Plan& P, const SpeciesDev& sp, const
 // waves (a 4e6-electron collision: photon kernel 3.90 -> 3.65 s); 2 waves in
 // FP64 (a 1e6-electron focused-pulse collision: 0.26 -> 0.22 s per 3000
 // steps against 1 wave).
-template <int KIND, int FM, bool MIX>
+template <int KIND, int FM, bool MIX, bool SH>
 __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
          int rmax) {
   const StepArgs a = *ap;
@@ -2303,7 +2308,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       const double ig = live ? sfp_inv_gamma_photon(ux, uy, uz) : 1.0;
       if (woke && sp.push) photon_wake_position(sp, a, i, ig, x, y, z, ux, uy, uz);
       uint32_t shot = 0;
-      const sfp_fieldset* FS = live ? fields_of(P, sp, i, shot) : &P.fields;
+      const sfp_fieldset* FS = live ? fields_of<SH>(P, sp, i, shot) : &P.fields;
       bool hot = false;
       double pb = -1.0;
       if (live && sp.push) {
@@ -2411,7 +2416,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
           atomicMax((unsigned long long*)&sp.cnt->defer_kneg, (unsigned long long)~(uint64_t)k);
         continue;
       }
-      photon_convert<KIND, FM, MIX>(P, sp, a, C, lds_rs, lds_od, rebuild, k, dA, dB, baseA + rank,
+      photon_convert<KIND, FM, MIX, SH>(P, sp, a, C, lds_rs, lds_od, rebuild, k, dA, dB, baseA + rank,
                                     baseB + rank, (uint64_t)(idA + rank), (uint64_t)(idB + rank));
     }
   }
@@ -2438,7 +2443,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
 // behind the conversions k_photon appended -- the positions and ids they
 // would have had -- and converted from their unchanged state as k_photon
 // converts.  The step then continues with the next kernel.
-template <int KIND, int FM, bool MIX>
+template <int KIND, int FM, bool MIX, bool SH>
 __global__ void __launch_bounds__(TILE) k_defer(const Plan* __restrict__ plan, int s,
                                                 const StepArgs* __restrict__ ap) {
   const StepArgs a = *ap;
@@ -2490,7 +2495,7 @@ __global__ void __launch_bounds__(TILE) k_defer(const Plan* __restrict__ plan, i
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t kk = k0 + tile * TILE + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
-      photon_convert<KIND, FM, MIX>(P, sp, a, C, lds_rs, lds_od, rebuild, kk, dA, dB,
+      photon_convert<KIND, FM, MIX, SH>(P, sp, a, C, lds_rs, lds_od, rebuild, kk, dA, dB,
                                     baseA + rank, baseB + rank, (uint64_t)(idA + rank),
                                     (uint64_t)(idB + rank));
     }
@@ -2691,62 +2696,82 @@ __global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ p
 // ------------------------------------------------------------- dispatch ----
 typedef void (*kfn)(const Plan*, int, const StepArgs*, int);
 
-template <int FM, bool MIX>
+template <int FM, bool MIX, bool SH>
 static kfn sweep_fn(bool photon, bool qed) {
-  if (photon) return qed ? k_awake : k_sweep<true, false, FM, false>;
-  return qed ? k_sweep<false, true, FM, MIX> : k_sweep<false, false, FM, MIX>;
+  if (photon) return qed ? k_awake : k_sweep<true, false, FM, false, false>;
+  return qed ? k_sweep<false, true, FM, MIX, SH> : k_sweep<false, false, FM, MIX, SH>;
 }
 
-template <int FM, bool MIX>
+template <int FM, bool MIX, bool SH>
 static kfn hot_fn(bool photon, int kind, int netk) {
-  if (photon) return kind == KIND_OD ? k_photon<KIND_OD, FM, MIX> : k_photon<KIND_RS, FM, MIX>;
-  if (MIX && netk == 2 && kind == KIND_RS) return k_hot<KIND_RS, FM, MIX, (MIX ? 2 : 0)>;
+  if (photon)
+    return kind == KIND_OD ? k_photon<KIND_OD, FM, MIX, SH> : k_photon<KIND_RS, FM, MIX, SH>;
+  if (MIX && netk == 2 && kind == KIND_RS) return k_hot<KIND_RS, FM, MIX, (MIX ? 2 : 0), SH>;
   if (MIX && netk == 1)
-    return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, (MIX ? 1 : 0)> : k_hot<KIND_RS, FM, MIX, (MIX ? 1 : 0)>;
-  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, 0> : k_hot<KIND_RS, FM, MIX, 0>;
+    return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, (MIX ? 1 : 0), SH>
+                           : k_hot<KIND_RS, FM, MIX, (MIX ? 1 : 0), SH>;
+  return kind == KIND_OD ? k_hot<KIND_OD, FM, MIX, 0, SH> : k_hot<KIND_RS, FM, MIX, 0, SH>;
 }
 
-template <bool MIX>
+template <bool MIX, bool SH>
 static kfn pick_sweep_p(int fm, bool photon, bool qed) {
-  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN, MIX>(photon, qed);
-  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE, MIX>(photon, qed);
-  return sweep_fn<SFP_FM_ANY, MIX>(photon, qed);
+  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN, MIX, SH>(photon, qed);
+  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE, MIX, SH>(photon, qed);
+  return sweep_fn<SFP_FM_ANY, MIX, SH>(photon, qed);
 }
 
-template <bool MIX>
+template <bool MIX, bool SH>
 static kfn pick_hot_p(int fm, bool photon, int kind, int netk) {
-  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX>(photon, kind, netk);
-  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX>(photon, kind, netk);
-  return hot_fn<SFP_FM_ANY, MIX>(photon, kind, netk);
+  if (fm == SFP_FM_GAUSSIAN) return hot_fn<SFP_FM_GAUSSIAN, MIX, SH>(photon, kind, netk);
+  if (fm == SFP_FM_PLANE) return hot_fn<SFP_FM_PLANE, MIX, SH>(photon, kind, netk);
+  return hot_fn<SFP_FM_ANY, MIX, SH>(photon, kind, netk);
 }
 
-static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed) {
-  return mixed ? pick_sweep_p<true>(fm, photon, qed) : pick_sweep_p<false>(fm, photon, qed);
+// sh: an ensemble (a field set per shot)
+static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed, bool sh) {
+  if (sh)
+    return mixed ? pick_sweep_p<true, true>(fm, photon, qed) : pick_sweep_p<false, true>(fm, photon, qed);
+  return mixed ? pick_sweep_p<true, false>(fm, photon, qed) : pick_sweep_p<false, false>(fm, photon, qed);
 }
 
-static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk = 0) {
-  return mixed ? pick_hot_p<true>(fm, photon, kind, netk)
-               : pick_hot_p<false>(fm, photon, kind, 0);
+static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk, bool sh) {
+  if (sh)
+    return mixed ? pick_hot_p<true, true>(fm, photon, kind, netk)
+                 : pick_hot_p<false, true>(fm, photon, kind, 0);
+  return mixed ? pick_hot_p<true, false>(fm, photon, kind, netk)
+               : pick_hot_p<false, false>(fm, photon, kind, 0);
 }
 
 typedef void (*kfn_defer)(const Plan*, int, const StepArgs*);
 
-template <int FM>
+template <int FM, bool SH>
 static kfn_defer defer_fn(int kind, bool mixed) {
-  if (mixed) return kind == KIND_OD ? k_defer<KIND_OD, FM, true> : k_defer<KIND_RS, FM, true>;
-  return kind == KIND_OD ? k_defer<KIND_OD, FM, false> : k_defer<KIND_RS, FM, false>;
+  if (mixed) return kind == KIND_OD ? k_defer<KIND_OD, FM, true, SH> : k_defer<KIND_RS, FM, true, SH>;
+  return kind == KIND_OD ? k_defer<KIND_OD, FM, false, SH> : k_defer<KIND_RS, FM, false, SH>;
 }
 
-static kfn_defer pick_defer(int fm, int kind, bool mixed) {
-  if (fm == SFP_FM_GAUSSIAN) return defer_fn<SFP_FM_GAUSSIAN>(kind, mixed);
-  if (fm == SFP_FM_PLANE) return defer_fn<SFP_FM_PLANE>(kind, mixed);
-  return defer_fn<SFP_FM_ANY>(kind, mixed);
+template <bool SH>
+static kfn_defer pick_defer_p(int fm, int kind, bool mixed) {
+  if (fm == SFP_FM_GAUSSIAN) return defer_fn<SFP_FM_GAUSSIAN, SH>(kind, mixed);
+  if (fm == SFP_FM_PLANE) return defer_fn<SFP_FM_PLANE, SH>(kind, mixed);
+  return defer_fn<SFP_FM_ANY, SH>(kind, mixed);
 }
 
-static kfn pick_emit(int fm, int kind) {
-  if (fm == SFP_FM_GAUSSIAN) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_GAUSSIAN> : k_emit<KIND_RS, SFP_FM_GAUSSIAN>;
-  if (fm == SFP_FM_PLANE) return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_PLANE> : k_emit<KIND_RS, SFP_FM_PLANE>;
-  return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_ANY> : k_emit<KIND_RS, SFP_FM_ANY>;
+static kfn_defer pick_defer(int fm, int kind, bool mixed, bool sh) {
+  return sh ? pick_defer_p<true>(fm, kind, mixed) : pick_defer_p<false>(fm, kind, mixed);
+}
+
+template <bool SH>
+static kfn pick_emit_p(int fm, int kind) {
+  if (fm == SFP_FM_GAUSSIAN)
+    return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_GAUSSIAN, SH> : k_emit<KIND_RS, SFP_FM_GAUSSIAN, SH>;
+  if (fm == SFP_FM_PLANE)
+    return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_PLANE, SH> : k_emit<KIND_RS, SFP_FM_PLANE, SH>;
+  return kind == KIND_OD ? k_emit<KIND_OD, SFP_FM_ANY, SH> : k_emit<KIND_RS, SFP_FM_ANY, SH>;
+}
+
+static kfn pick_emit(int fm, int kind, bool sh) {
+  return sh ? pick_emit_p<true>(fm, kind) : pick_emit_p<false>(fm, kind);
 }
 
 // ------------------------------------------------------------- host side ---
@@ -2971,9 +2996,10 @@ static int sim_pick_kernels(Sim* sim) {
   }
   const bool mixed = sim->params.precision == SFP_PRECISION_MIXED;
   const int kind = sim->params.qed_model == SFP_QED_OD ? KIND_OD : KIND_RS;
+  const bool sh = sim->shot_fields != nullptr;
   for (int s = 0; s < sim->nsp; ++s) {
     const bool photon = sim->plan.sp[s].photon;
-    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed);
+    sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed, sh);
     // learned models: the rate network for spin-carrying leptons under
     // spin-resolved emission, else the photon-energy sampler
     int netk = 0;
@@ -2982,9 +3008,9 @@ static int sim_pick_kernels(Sim* sim) {
       netk = 2;
     else if (!photon && sim->net)
       netk = 1;
-    sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, netk) : nullptr;
-    sim->emit[s] = (sim->qed[s] && mixed && netk == 1) ? pick_emit(sim->fm, kind) : nullptr;
-    sim->defer[s] = (sim->qed[s] && photon) ? pick_defer(sim->fm, kind, mixed) : nullptr;
+    sim->hot[s] = sim->qed[s] ? pick_hot(sim->fm, photon, kind, mixed, netk, sh) : nullptr;
+    sim->emit[s] = (sim->qed[s] && mixed && netk == 1) ? pick_emit(sim->fm, kind, sh) : nullptr;
+    sim->defer[s] = (sim->qed[s] && photon) ? pick_defer(sim->fm, kind, mixed, sh) : nullptr;
     for (int ph = 0; ph < NPH; ++ph) {
       kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
       std::vector<Shape>& c = sim->cand[s][ph];
