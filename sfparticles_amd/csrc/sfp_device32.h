@@ -10,7 +10,9 @@
  * not grow with the distance travelled.
  *
  * Same formulas and operation order as the FP64 device code (sfp_device.h),
- * which follows fields.py, inline.py and qed/; only the precision differs.
+ * which follows fields.py, inline.py and qed/; only the precision differs,
+ * and FP32 quotients are products with the hardware reciprocal (sfp32_div,
+ * about 2 ulp: a tenth of the instructions of the IEEE division sequence).
  */
 #ifndef SFP_DEVICE32_H
 #define SFP_DEVICE32_H
@@ -26,6 +28,9 @@ __device__ inline float sfp32_reduce(double ph) {
 }
 
 __device__ inline float sfp32_exp(float x) { return __expf(x); }
+
+/* a / b as a times the hardware reciprocal of b (1 ulp), about 2 ulp */
+__device__ inline float sfp32_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 /* fields.py:69-118 */
 __device__ inline void sfp32_field_simple(const sfp_term_dev* d, double x, double y, double z,
@@ -60,12 +65,12 @@ __device__ inline void sfp32_field_gaussian(const sfp_term_dev* d, double x, dou
   const float xz = xf * (float)c[FG_RZR];
   const float wx2 = w0 * w0 * (1.0f + xz * xz);
   const float gouy = atanf(xz);
-  const float curv = (x == 0.0) ? 0.0f : k0 * r2 * xf / (2.0f * (xf * xf + zR * zR));
+  const float curv = (x == 0.0) ? 0.0f : sfp32_div(k0 * r2 * xf, 2.0f * (xf * xf + zR * zR));
   const double phl = c[FG_K0] * (x - c[FG_X0] - c[FG_DC] * t);
   const double ph_d = phl + (double)(curv - gouy);
   const float phi = (float)ph_d;
   const float E = (float)c[FG_E0] * __cosf(sfp32_reduce(ph_d + c[FG_CEP])) *
-                  __frsqrt_rn(1.0f + xz * xz) * sfp32_exp(-r2 / wx2) *
+                  __frsqrt_rn(1.0f + xz * xz) * sfp32_exp(sfp32_div(-r2, wx2)) *
                   sfp32_exp(-(phi * phi) * (float)c[FG_RKCT2]);
   const float Ey = E * (float)c[FG_COSP];
   const float Ez = E * (float)c[FG_SINP];
@@ -90,20 +95,22 @@ __device__ inline void sfp32_field_focused(const sfp_term_dev* d, double x, doub
   const double xd = c[FF_DIR] * (x - c[FF_XFOC]);
   const float xf = (float)xd;
   const float r2 = (float)(y * y + z * z);
-  const float xz = xf / zR;
+  const float xz = sfp32_div(xf, zR);
   const float wx = w0 * sqrtf(1.0f + xz * xz);
-  const float psiG = atanf(xf / zR);
-  const float psiR = k0 * xf * r2 / 2.0f / (xf * xf + zR * zR);
+  const float psiG = atanf(xz);
+  const float psiR = sfp32_div(k0 * xf * r2 / 2.0f, xf * xf + zR * zR);
   const double psiP = c[FF_K0] * (xd - c[FF_DX0] - ct);
   const double psi_d = (double)(psiG + psiR) + psiP + c[FF_CEP];
   const float pc = (float)(psi_d - c[FF_CEP]);
   const float psi = sfp32_reduce(psi_d);
-  const float rho2 = r2 / (w0 * w0);
-  const float xi = (float)y / w0, nu = (float)z / w0;
-  const float eta = 1.0f / sqrtf(1.0f + xz * xz);
+  const float iw0 = sfp32_div(1.0f, w0);
+  const float rho2 = r2 * (iw0 * iw0);
+  const float xi = (float)y * iw0, nu = (float)z * iw0;
+  const float eta = __frsqrt_rn(1.0f + xz * xz);
   const float rho4 = rho2 * rho2, rho6 = rho4 * rho2, rho8 = rho4 * rho4;
-  const float E = (float)c[FF_E0] * w0 / wx * sfp32_exp(-r2 / (wx * wx)) *
-                  sfp32_exp(-(pc * pc) / (float)c[FF_KCT2]);
+  const float iwx = sfp32_div(1.0f, wx);
+  const float E = (float)c[FF_E0] * w0 * iwx * sfp32_exp(-r2 * (iwx * iwx)) *
+                  sfp32_exp(sfp32_div(-(pc * pc), (float)c[FF_KCT2]));
   float ek[8];
   ek[0] = 1.0f;
   for (int k = 1; k < 8; ++k) ek[k] = ek[k - 1] * eta;
@@ -177,7 +184,7 @@ __device__ inline float sfp32_boris(float& ux, float& uy, float& uz, const float
   const float uxp = uxm + uym * Tz - uzm * Ty;
   const float uyp = uym + uzm * Tx - uxm * Tz;
   const float uzp = uzm + uxm * Ty - uym * Tx;
-  const float Tf = 2.0f / (1.0f + Tx * Tx + Ty * Ty + Tz * Tz);
+  const float Tf = sfp32_div(2.0f, 1.0f + Tx * Tx + Ty * Ty + Tz * Tz);
   const float Sx = Tf * Tx, Sy = Tf * Ty, Sz = Tf * Tz;
   ux = uxm + uyp * Sz - uzp * Sy + efactor * F[0];
   uy = uym + uzp * Sx - uxp * Sz + efactor * F[1];
@@ -194,14 +201,16 @@ __device__ inline float sfp32_boris_tbmt(float& ux, float& uy, float& uz, float&
   const float uxm = ux + efactor * Ex, uym = uy + efactor * Ey, uzm = uz + efactor * Ez;
   const float udB = uxm * Bx + uym * By + uzm * Bz;
   const float gm = sqrtf(1.0f + uxm * uxm + uym * uym + uzm * uzm);
-  const float igm = 1.0f / gm;
-  const float a1 = ae + igm, a2 = ae * gm / (gm + 1.0f), a3 = ae + 1.0f / (gm + 1.0f);
+  const float igm = sfp32_div(1.0f, gm);
+  const float igp = sfp32_div(1.0f, gm + 1.0f);
+  const float a1 = ae + igm, a2 = ae * gm * igp, a3 = ae + igp;
   const float Bxe = a1 * Bx - a2 * udB * uxm - a3 * (uym * Ez - uzm * Ey);
   const float Bye = a1 * By - a2 * udB * uym - a3 * (uzm * Ex - uxm * Ez);
   const float Bze = a1 * Bz - a2 * udB * uzm - a3 * (uxm * Ey - uym * Ex);
   float Tx = bfactor * Bxe, Ty = bfactor * Bye, Tz = bfactor * Bze;
   const float T2 = Tx * Tx + Ty * Ty + Tz * Tz;
-  float Sx = 2 * Tx / (1.0f + T2), Sy = 2 * Ty / (1.0f + T2), Sz = 2 * Tz / (1.0f + T2);
+  const float iT = sfp32_div(2.0f, 1.0f + T2);
+  float Sx = Tx * iT, Sy = Ty * iT, Sz = Tz * iT;
   const float sxp = sx + sy * Tz - sz * Ty;
   const float syp = sy + sz * Tx - sx * Tz;
   const float szp = sz + sx * Ty - sy * Tx;
@@ -217,7 +226,7 @@ __device__ inline float sfp32_boris_tbmt(float& ux, float& uy, float& uz, float&
   const float uxp = uxm + uym * Tz - uzm * Ty;
   const float uyp = uym + uzm * Tx - uxm * Tz;
   const float uzp = uzm + uxm * Ty - uym * Tx;
-  const float Tf = 2.0f / (1.0f + Tx * Tx + Ty * Ty + Tz * Tz);
+  const float Tf = sfp32_div(2.0f, 1.0f + Tx * Tx + Ty * Ty + Tz * Tz);
   Sx = Tf * Tx;
   Sy = Tf * Ty;
   Sz = Tf * Tz;
@@ -227,16 +236,25 @@ __device__ inline float sfp32_boris_tbmt(float& ux, float& uy, float& uz, float&
   return __frsqrt_rn(1.0f + ux * ux + uy * uy + uz * uz);
 }
 
-/* chi in FP64 (inline.py:126-134) from FP32 fields and momentum */
+/* chi in FP64 (inline.py:126-134) from FP32 fields and momentum, with gamma
+ * (|u| for a photon) formed directly: no FP64 division */
+__device__ inline double sfp32_chi_g(const float* F, double ux, double uy, double uz,
+                                     double gamma) {
+  const double factor = SFP_E * SFP_HBAR / ((SFP_ME * SFP_ME) * (SFP_C * SFP_C * SFP_C));
+  const double Ex = F[0], Ey = F[1], Ez = F[2], Bx = F[3], By = F[4], Bz = F[5];
+  const double a = gamma * Ex + (uy * Bz - uz * By) * SFP_C;
+  const double b = gamma * Ey + (uz * Bx - ux * Bz) * SFP_C;
+  const double c = gamma * Ez + (ux * By - uy * Bx) * SFP_C;
+  const double d = ux * Ex + uy * Ey + uz * Ez;
+  return factor * sfp_sqrt(a * a + b * b + c * c - d * d);
+}
 __device__ inline double sfp32_chi(const float* F, float ux, float uy, float uz) {
   const double dux = ux, duy = uy, duz = uz;
-  const double ig = 1.0 / sfp_sqrt(1.0 + dux * dux + duy * duy + duz * duz);
-  return sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], dux, duy, duz, ig);
+  return sfp32_chi_g(F, dux, duy, duz, sfp_sqrt(1.0 + dux * dux + duy * duy + duz * duz));
 }
 __device__ inline double sfp32_chi_photon(const float* F, float ux, float uy, float uz) {
   const double dux = ux, duy = uy, duz = uz;
-  const double ig = 1.0 / sfp_sqrt(dux * dux + duy * duy + duz * duz);
-  return sfp_chi(F[0], F[1], F[2], F[3], F[4], F[5], dux, duy, duz, ig);
+  return sfp32_chi_g(F, dux, duy, duz, sfp_sqrt(dux * dux + duy * duy + duz * duz));
 }
 
 /* inline.py:104-123: LL (g = 1) or CLL (g = g(chi)) in FP32; returns 1/gamma */
@@ -264,7 +282,7 @@ __device__ inline float sfp32_rs_rate(const sfp_rs_node32* nodes, int z_N, float
                                       float delta, bool photon) {
   const float chi_a = delta * chi_parent;
   const float chi_b = chi_parent - chi_a;
-  const float q = photon ? chi_a / chi_parent / chi_b : chi_parent / chi_a / chi_b;
+  const float q = photon ? sfp32_div(chi_a, chi_parent * chi_b) : sfp32_div(chi_parent, chi_a * chi_b);
   const float z = exp2f(2.0f / 3.0f * __log2f(q));
   const float factor = (float)(SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR);
   if (!(z <= z_max)) return (z != z) ? z : 0.0f;
@@ -275,8 +293,9 @@ __device__ inline float sfp32_rs_rate(const sfp_rs_node32* nodes, int z_N, float
   const float ia = nd.ia + nd.ka * dz;
   const float ap = nd.ap + nd.kp * dz;
   const float sz = sqrtf(z);
-  if (photon) return -factor * (ia + (2.0f / z + chi_a * sz) * ap);
-  return factor * (ia + (2.0f / z - chi_parent * sz) * ap);
+  const float tz = sfp32_div(2.0f, z);
+  if (photon) return -factor * (ia + (tz + chi_a * sz) * ap);
+  return factor * (ia + (tz - chi_parent * sz) * ap);
 }
 
 /* sfp_rs_rate_spin in FP32 (photon emission) */
@@ -285,7 +304,7 @@ __device__ inline float sfp32_rs_rate_spin(const sfp_rs_node32* nodes, const flo
                                            float chi_parent, float delta, float sigma) {
   const float chi_a = delta * chi_parent;
   const float chi_b = chi_parent - chi_a;
-  const float q = chi_a / chi_parent / chi_b;
+  const float q = sfp32_div(chi_a, chi_parent * chi_b);
   const float z = exp2f(2.0f / 3.0f * __log2f(q));
   const float factor = (float)(SFP_ALPHA * SFP_ME * (SFP_C * SFP_C) / SFP_HBAR);
   if (!(z <= z_max)) return (z != z) ? z : 0.0f;
@@ -297,13 +316,13 @@ __device__ inline float sfp32_rs_rate_spin(const sfp_rs_node32* nodes, const flo
   const float ia = nd.ia + nd.ka * dz;
   const float ap = nd.ap + nd.kp * dz;
   const float a = an.x + an.y * dz;
-  return -factor * (ia + (2.0f / z + chi_a * sqrtf(z)) * ap + sigma * chi_b * z * a);
+  return -factor * (ia + (sfp32_div(2.0f, z) + chi_a * sqrtf(z)) * ap + sigma * chi_b * z * a);
 }
 
 /* sfp_spin_sigma in FP32 */
 __device__ inline float sfp32_spin_sigma(double q, float ux, float uy, float uz, float ig,
                                          const float* F, float sx, float sy, float sz) {
-  const float g = 1.0f / ig;
+  const float g = sfp32_div(1.0f, ig);
   const float ic = (float)(1.0 / SFP_C);
   const float ex = uy * F[2] - uz * F[1], ey = uz * F[0] - ux * F[2], ez = ux * F[1] - uy * F[0];
   const float bx = uy * F[5] - uz * F[4], by = uz * F[3] - ux * F[5], bz = ux * F[4] - uy * F[3];

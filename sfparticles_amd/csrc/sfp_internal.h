@@ -55,8 +55,10 @@ __device__ inline int wave_lane() { return (int)__lane_id(); }
 // in flight.  The host pass sees plain pointers (same size and layout).
 #ifdef __HIP_DEVICE_COMPILE__
 #define SFP_G __attribute__((address_space(1)))
+#define SFP_L __attribute__((address_space(3)))  // LDS, for pointers to __shared__ arrays
 #else
 #define SFP_G
+#define SFP_L
 #endif
 // a device allocation as the plan holds it (host code)
 template <class T>

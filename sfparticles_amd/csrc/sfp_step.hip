@@ -592,7 +592,7 @@ __device__ inline uint32_t sleep_steps(const Plan& P, const sfp_fieldset* FS, co
     const uint32_t d = end - a.wrel;
     float S = sfp_field_bound_f32_path<FM>(FS, x, y, z, a.t, vx, vy, vz,
                                            a.dt * (double)d * 1.000001);
-    float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+    float chi_b = (float)P.chi_coef * S * sfp32_div(1.0f, (float)ig) * 1.001f;
     if (!qed_possible(P, true, chi_b, ig, a.dt)) return d;
   }
   return 1;
@@ -620,7 +620,7 @@ __device__ inline uint32_t sleep_steps_drawn(const Plan& P, const sfp_fieldset* 
     const uint32_t d = end - a.wrel;
     float S = sfp_field_bound_f32_path<FM>(FS, x, y, z, a.t, vx, vy, vz,
                                            a.dt * (double)d * 1.000001);
-    float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+    float chi_b = (float)P.chi_coef * S * sfp32_div(1.0f, (float)ig) * 1.001f;
     const double pb = rs_pbound(P, true, chi_b, ig, a.dt, false, env_lds);
     const double lim = 2.0 * pb;
     // no bound, or one a window's draws would likely fall under
@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
         // chi <= (e hbar/(m^2 c^3)) gamma (|E| + c|B|) (gamma >= |u|); the FP32
         // bound and this product carry a small margin for their own rounding
         float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
-        float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+        float chi_b = (float)P.chi_coef * S * sfp32_div(1.0f, (float)ig) * 1.001f;
         hot = qed_possible(P, PHOTON, chi_b, ig, a.dt, !PHOTON && P.spin_emission && sp.spin);
       }
       if (QED) {
@@ -940,11 +940,16 @@ __device__ inline bool room_or_halt(const Plan& P, int s, const StepArgs& a, con
 // possible at all (p >= 2^-54, or chi >= chi_min under optical depth: the
 // lepton QED kernel's dense steps decide the next step's mode on it).
 enum { QD_P1 = 0, QD_ZC, QD_CC, QD_NAN, QD_PMAX, QD_LIVE, QD_N };
+// The counters live in LDS and are typed so (SFP_L): through a generic
+// pointer the read below compiled to a flat load, whose wait also waited for
+// every global load in flight -- the next row's state included.
 struct QDiag {
-  uint32_t* sh;  // QD_N LDS words of the workgroup
-  __device__ static void tally(uint32_t* w, bool cond) {
+  SFP_L uint32_t* sh;  // QD_N LDS words of the workgroup
+  __device__ static void tally(SFP_L uint32_t* w, bool cond) {
     const uint64_t m = __ballot(cond);
-    if (m && wave_lane() == __builtin_ctzll(m)) atomicAdd(w, (uint32_t)__popcll(m));
+    if (m && wave_lane() == __builtin_ctzll(m))
+      __hip_atomic_fetch_add(w, (uint32_t)__popcll(m), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __device__ void prob(double p) {
     tally(&sh[QD_P1], p > 1.0);
@@ -954,8 +959,11 @@ struct QDiag {
     // maximum read so far (a stale read is smaller, never larger): after the
     // first rows of a workgroup it is rare, where one per lane on the same
     // LDS word would serialise
-    if (f > 0.0f && __float_as_uint(f) > *(volatile uint32_t*)&sh[QD_PMAX])
-      atomicMax(&sh[QD_PMAX], __float_as_uint(f));
+    if (f > 0.0f &&
+        __float_as_uint(f) > __hip_atomic_load(&sh[QD_PMAX], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+      __hip_atomic_fetch_max(&sh[QD_PMAX], __float_as_uint(f), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   // photon emission: z = (delta / ((1 - delta) chi))^(2/3) > z_max
   __device__ void zcheck(const Plan& P, double chi, double delta) {
@@ -1138,7 +1146,7 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
     }
   }
   if (!PHOTON && event) {
-    if ((float)delta / ig < (float)P.threshold) event = false;
+    if (sfp32_div((float)delta, ig) < (float)P.threshold) event = false;
   }
   return event;
 }
@@ -1445,7 +1453,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
 
   uint32_t alive = 0, proc = 0;
   qdiag_init(sh_qd);
-  QDiag dg{sh_qd};
+  QDiag dg{(SFP_L uint32_t*)sh_qd};
   bool staged = false;
   if constexpr (NETK != 2) {
   const int64_t SPAN = span;
@@ -1501,14 +1509,19 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         if (KIND == KIND_OD) tau = sp.tau[i];
         if (sp.spin) { sx = sp.sx[i]; sy = sp.sy[i]; sz = sp.sz[i]; }
       }
+      // the list index of the row after next loads into a temporary that
+      // becomes i_next only at the end of this row: taken at once, its wait
+      // (the load issued last) was a wait for every load in flight, the next
+      // row's state included, and the rows did not overlap at all
+      uint32_t i_after = 0u;
       if (row + 1 < rows && k + TILE < n_hot) {
         i_cur = i_next;
         nx = sp.x[i_cur]; ny = sp.y[i_cur]; nz = sp.z[i_cur];
         nux = sp.ux[i_cur]; nuy = sp.uy[i_cur]; nuz = sp.uz[i_cur];
         npid = sp.id[i_cur];
         npr = dense ? sp.pruned[i_cur] : (uint8_t)0;
-        i_next = (row + 2 < rows && k + 2 * TILE < n_hot)
-                     ? (dense ? (uint32_t)(k + 2 * TILE) : sp.hot[k + 2 * TILE]) : 0u;
+        if (row + 2 < rows && k + 2 * TILE < n_hot)
+          i_after = dense ? (uint32_t)(k + 2 * TILE) : sp.hot[k + 2 * TILE];
       }
       uint32_t shot = 0;
       const sfp_fieldset* FS = valid ? fields_of<SH>(P, sp, i, shot) : &P.fields;
@@ -1561,6 +1574,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
         if (KIND == KIND_OD && need) sp.tau[i] = tau;
         alive += 1;
       }
+      i_next = i_after;
     }
 
     // event pass, in list order, each decision recomputed from the lepton's
@@ -1755,7 +1769,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
       if (valid && r2 < p) {
         event = true;
         delta = r1_3;
-        if ((float)delta / igf < (float)P.threshold) event = false;  // cpu.py:227-232
+        if (sfp32_div((float)delta, igf) < (float)P.threshold) event = false;  // cpu.py:227-232
       }
     } else if (need) {
       if (MIX) {
@@ -1904,7 +1918,7 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
       delta = (KIND == KIND_RS) ? sfp_od_sample_from0(P.od.photon_cdf, &P.od, r.chi, r.u_tab)
                                 : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, r.chi, r.u_tab);
     // hard-photon cut (cpu.py:227-232)
-    bool event = valid && !((float)delta / igf < (float)P.threshold);
+    bool event = valid && !(sfp32_div((float)delta, igf) < (float)P.threshold);
     if (!dA) event = false;
     const int64_t rank = tile_rank(event, tile, P.flags[s][2], epoch, sh_w, &sh_pref);
     if (event) {
@@ -2231,7 +2245,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
 
   uint32_t cnt[4] = {0, 0, 0, 0};  // alive, processed (live visited), past the bound (id read), multi-step pushes
   qdiag_init(sh_qd);
-  QDiag dg{sh_qd};
+  QDiag dg{(SFP_L uint32_t*)sh_qd};
   bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
@@ -2313,7 +2327,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
       double pb = -1.0;
       if (live && sp.push) {
         float S = sfp_field_bound_f32<FM>(FS, x, y, z, a.t);
-        float chi_b = (float)P.chi_coef * S * (float)(1.0 / ig) * 1.001f;
+        float chi_b = (float)P.chi_coef * S * sfp32_div(1.0f, (float)ig) * 1.001f;
         if (KIND == KIND_RS) {
           pb = rs_pbound(P, true, chi_b, ig, a.dt, false, lds_env);
           hot = pb < 0.0 || !(2.0 * pb < 0x1p-53);
