@@ -274,6 +274,7 @@ __device__ inline int64_t tile_rank(bool pred, int64_t tile, uint64_t* flags, ui
 // launch passes the most rows a tile may take (1..QROWS, a choice of the
 // launch-shape tuner; results do not depend on it).
 static constexpr int QROWS = 16;
+static_assert(QROWS * TILE <= 65536, "event queues hold 16-bit tile positions");
 
 __device__ inline int qed_rows(int64_t n_list, int rcap) {
   // the fewest rounds of tiles (one tile per workgroup per round) that
@@ -294,7 +295,7 @@ __device__ inline int qed_rows(int64_t n_list, int rcap) {
 // q in (row, wave, lane) order, i.e. list order.  Returns the event count.
 // Called by every thread; q and *sh_pref are visible on return.
 __device__ inline uint32_t tile_events(const uint64_t (*mask)[WAVES], int rows, uint64_t* flags,
-                                       int64_t tile, uint32_t epoch, uint32_t* q,
+                                       int64_t tile, uint32_t epoch, uint16_t* q,
                                        uint64_t* sh_pref) {
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
@@ -316,7 +317,8 @@ __device__ inline uint32_t tile_events(const uint64_t (*mask)[WAVES], int rows, 
       total += c;
     }
     const uint64_t m = mask[row][wv];
-    if ((m >> lane) & 1ull) q[off + before + (uint32_t)__popcll(m & lt)] = (uint32_t)(row * TILE) + threadIdx.x;
+    if ((m >> lane) & 1ull)
+      q[off + before + (uint32_t)__popcll(m & lt)] = (uint16_t)((uint32_t)(row * TILE) + threadIdx.x);
     off += total;
   }
   __syncthreads();
@@ -441,6 +443,16 @@ __device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const S
   if (sp.shot) sp.shot[d] = shot;
 }
 
+// the float at or above d (FP32 copies of bound constants stay bounds)
+__device__ inline float f32_up(double d) {
+  float f = (float)d;
+  if ((double)f < d) {
+    const uint32_t b = __float_as_uint(f);
+    f = f > 0.0f ? __uint_as_float(b + 1u) : (f < 0.0f ? __uint_as_float(b - 1u) : 0x1p-149f);
+  }
+  return f;
+}
+
 // Can a QED event happen this step?  chi_b >= chi (bound with a small margin).  Rejection sampling: an event needs r2 < p with r2 >= 2^-53 (the
 // smallest uniform), p = 3 r1^2 w dtau (photon) or w dtau (pair), where the
 // table rate w = (alpha m c^2/hbar) |int Ai(z) + (2/z +- chi' sqrt z) Ai'(z)|
@@ -455,7 +467,7 @@ __device__ inline void spawn_photon(uint32_t shot, const SpeciesDev& sp, const S
 // Rejection sampling: the bound on p itself (< 0: no bound, evaluate; 0: no
 // event possible).
 __device__ inline double rs_pbound(const Plan& P, bool photon, float chi_b, double ig, double dt,
-                                   bool spin, const double2* env_lds = nullptr) {
+                                   bool spin, const float2* env_lds = nullptr) {
   float num;
   if (photon) {
     num = 4.0f;
@@ -472,16 +484,23 @@ __device__ inline double rs_pbound(const Plan& P, bool photon, float chi_b, doub
   if (!(fj >= 1.0f)) return -1.0;                       // first bracket: 2/z unbounded
   int64_t j = (int64_t)fj;
   if (j > P.z_N - 2) return 0.0;
-  // from the workgroup's LDS copy where it has one: a global load issued
-  // after the next row's state would wait for those loads (they complete in
-  // order) and undo the software pipeline
-  double2 e = env_lds ? env_lds[j] : (spin ? P.rs_env_spin[j] : P.rs_env[j]);
+  // from the workgroup's LDS copy where it has one (in FP32, each value
+  // rounded up: still a bound, at half the LDS): a global load issued after
+  // the next row's state would wait for those loads (they complete in order)
+  // and undo the software pipeline
+  double2 e;
+  if (env_lds) {
+    const float2 f = env_lds[j];
+    e = make_double2((double)f.x, (double)f.y);
+  } else {
+    e = spin ? P.rs_env_spin[j] : P.rs_env[j];
+  }
   return P.rate_unit * (e.x + (double)chi_b * e.y) * dt * ig * (photon ? 1.0 : 3.0);
 }
 
 __device__ inline bool qed_possible(const Plan& P, bool photon, float chi_b, double ig,
                                     double dt, bool spin = false,
-                                    const double2* env_lds = nullptr) {
+                                    const float2* env_lds = nullptr) {
   if (P.qed == SFP_QED_OD) return !(chi_b < (float)P.od.chi_min * 0.999f);
   const double pb = rs_pbound(P, photon, chi_b, ig, dt, spin, env_lds);
   return pb < 0.0 || !(2.0 * pb < 0x1p-53);
@@ -611,7 +630,7 @@ template <int FM, bool MIX>
 __device__ inline uint32_t sleep_steps_drawn(const Plan& P, const sfp_fieldset* FS,
                                              const StepArgs& a, double x, double y, double z,
                                              double ux, double uy, double uz, double ig,
-                                             uint64_t pid, uint32_t tag, const double2* env_lds) {
+                                             uint64_t pid, uint32_t tag, const float2* env_lds) {
   const double vx = SFP_C * ig * ux, vy = SFP_C * ig * uy, vz = SFP_C * ig * uz;
   const uint32_t q[2] = {64u, WIN};
   for (int k = 0; k < 2; ++k) {
@@ -1427,7 +1446,7 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[NETK == 2 ? 1 : QROWS][WAVES];
-  __shared__ uint32_t sh_q[NETK == 2 ? 1 : QROWS * TILE];
+  __shared__ uint16_t sh_q[NETK == 2 ? 1 : QROWS * TILE];  // tile positions (< 2^16)
   __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_hot = sp.cnt->n_hot;
@@ -2188,7 +2207,7 @@ __device__ inline void photon_convert(const Plan& P, const SpeciesDev& sp, const
 // FP64 (a 1e6-electron focused-pulse collision: 0.26 -> 0.22 s per 3000
 // steps against 1 wave).
 template <int KIND, int FM, bool MIX, bool SH>
-__global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+__global__ void __launch_bounds__(TILE, MIX ? 4 : 2) k_photon(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
          int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
@@ -2198,12 +2217,12 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
 
   __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
   __shared__ double2 lds_od[KIND == KIND_OD ? 255 : 1];
-  __shared__ double2 lds_env[KIND == KIND_RS ? 1023 : 1];  // rate envelopes of the bound
+  __shared__ float2 lds_env[KIND == KIND_RS ? 1023 : 1];  // rate envelopes of the bound, rounded up
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[QROWS][WAVES];
-  __shared__ uint32_t sh_q[QROWS * TILE];
+  __shared__ uint16_t sh_q[QROWS * TILE];  // tile positions (< 2^16)
   __shared__ uint32_t sh_qd[QD_N];
 
   const int64_t n_list = sp.cnt->n_hot;
@@ -2256,7 +2275,10 @@ __global__ void __launch_bounds__(TILE, MIX ? 3 : 2) k_photon(const Plan* __rest
     if (!staged) {  // tables go to LDS once the workgroup has work
       stage_tables<KIND, true, MIX>(P, lds_rs, lds_od);
       if (KIND == KIND_RS) {
-        for (int64_t k = threadIdx.x; k < P.z_N - 1 && k < 1023; k += TILE) lds_env[k] = P.rs_env[k];
+        for (int64_t k = threadIdx.x; k < P.z_N - 1 && k < 1023; k += TILE) {
+          const double2 e = P.rs_env[k];
+          lds_env[k] = make_float2(f32_up(e.x), f32_up(e.y));
+        }
         __syncthreads();
       }
       staged = true;
@@ -2470,7 +2492,7 @@ __global__ void __launch_bounds__(TILE) k_defer(const Plan* __restrict__ plan, i
   __shared__ int64_t sh_tile;
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[1][WAVES];
-  __shared__ uint32_t sh_q[TILE];
+  __shared__ uint16_t sh_q[TILE];
 
   const int64_t n_list = sp.cnt->n_hot;
   const int64_t n_def = sp.cnt->defer_n;
