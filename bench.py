@@ -442,7 +442,8 @@ def summarize_profile(d):
             res[path] = agg
         return res, disp
 
-    have = {k: os.path.isdir(os.path.join(d, k)) for k in ("fetch", "write", "sq", "cal_f", "cal_w")}
+    have = {k: os.path.isdir(os.path.join(d, k))
+            for k in ("fetch", "write", "sq", "cal_f", "cal_w", "mfma")}
     scale = {}
     for sub, cname in (("cal_f", "FETCH_SIZE"), ("cal_w", "WRITE_SIZE")):
         if have[sub]:
@@ -456,6 +457,8 @@ def summarize_profile(d):
     sqn = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
            "SQ_ACTIVE_INST_VALU", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD"]
     sq = counters("sq", sqn)[0] if have["sq"] else {}
+    # BF16 matrix operations: 512 FLOPs per SQ_INSTS_VALU_MFMA_MOPS_BF16 count
+    mf = counters("mfma", ["SQ_INSTS_VALU_MFMA_MOPS_BF16"])[0] if have["mfma"] else {}
     if scale:
         out += ["Counter scale (counted / known bytes of the copy kernel, 1 GiB read and "
                 "1 GiB written per dispatch): " + ", ".join(f"{k} {v:.3f}" for k, v in scale.items())
@@ -469,16 +472,23 @@ def summarize_profile(d):
                 "",
                 "| kernel | dispatches | mean us | total us | HBM read MB / dispatch | "
                 "HBM written MB / dispatch | wait / wave cycles | VALU active / wave cycles | "
-                "busy cycles / dispatch | VALU insts / dispatch | VMEM read insts / dispatch |",
-                "|---|---|---|---|---|---|---|---|---|---|---|"]
+                "busy cycles / dispatch | VALU insts / dispatch | VMEM read insts / dispatch | "
+                "BF16 MFMA TFLOP/s |",
+                "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+        step_rd = step_wr = 0.0
         for name, v in sorted(t.items(), key=lambda kv: -sum(kv[1])):
-            rd = wr = wt = va = bz = vi = vm = ""
+            rd = wr = wt = va = bz = vi = vm = mt = ""
             f = fetch.get(path, {}).get(name, {}).get("FETCH_SIZE")
             if f and "FETCH_SIZE" in scale:
                 rd = f"{np.mean(f) * 1024 / scale['FETCH_SIZE'] / 1e6:.1f}"
+                step_rd += sum(f) * 1024 / scale["FETCH_SIZE"]
             w = write.get(path, {}).get(name, {}).get("WRITE_SIZE")
             if w and "WRITE_SIZE" in scale:
                 wr = f"{np.mean(w) * 1024 / scale['WRITE_SIZE'] / 1e6:.1f}"
+                step_wr += sum(w) * 1024 / scale["WRITE_SIZE"]
+            mo = mf.get(path, {}).get(name, {}).get("SQ_INSTS_VALU_MFMA_MOPS_BF16")
+            if mo and sum(mo) > 0:
+                mt = f"{np.mean(mo) * 512 / (np.mean(v) * 1e-6) / 1e12:.1f}"
             q = sq.get(path, {}).get(name, {})
             if q.get("SQ_WAVE_CYCLES") and q.get("SQ_WAIT_ANY"):
                 wt = f"{sum(q['SQ_WAIT_ANY']) / max(sum(q['SQ_WAVE_CYCLES']), 1):.2f}"
@@ -491,8 +501,18 @@ def summarize_profile(d):
             if q.get("SQ_INSTS_VMEM_RD"):
                 vm = f"{np.mean(q['SQ_INSTS_VMEM_RD']):.3g}"
             out.append(f"| `{name}` | {len(v)} | {np.mean(v):.1f} | {sum(v):.0f} | {rd} | {wr} | "
-                       f"{wt} | {va} | {bz} | {vi} | {vm} |")
+                       f"{wt} | {va} | {bz} | {vi} | {vm} | {mt} |")
         out.append("")
+        if step_rd or step_wr:
+            # the same window of the same workload in every run: the line's
+            # algorithmic bytes per step apply to the counter runs' windows
+            alg = line["roofline"]["whole_step"]["alg_bytes_per_step"] if path == head else None
+            tr = (step_rd + step_wr) / steps
+            out += [f"HBM traffic of the window, all kernels (counters): {tr / 1e9:.3f} GB per step "
+                    f"({step_rd / steps / 1e9:.3f} read, {step_wr / steps / 1e9:.3f} written)"
+                    + (f"; algorithmic (the survey's bytes per particle-step, the line's "
+                       f"whole_step): {alg / 1e9:.3f} GB per step, traffic / algorithmic "
+                       f"{tr / alg:.2f}." if alg else "."), ""]
         if path == head:
             rf = line["roofline"]
             kn = {"k_hot": "k_hot", "k_photon": "k_photon", "k_sweep": "k_sweep",
@@ -515,6 +535,13 @@ def summarize_profile(d):
                         f"{rf['alg_bytes_per_launch'] / 1e6:.1f} MB algorithmic per launch -> "
                         f"{ach:.0f} GB/s, frac {ach / HBM_PEAK_GBS:.4f} (line: {rf['frac']}).",
                         ""]
+                f = fetch.get(path, {}).get(name, {}).get("FETCH_SIZE")
+                w = write.get(path, {}).get(name, {}).get("WRITE_SIZE")
+                if f and w and "FETCH_SIZE" in scale and "WRITE_SIZE" in scale:
+                    tb = (np.mean(f) / scale["FETCH_SIZE"] + np.mean(w) / scale["WRITE_SIZE"]) * 1024
+                    out += [f"Its HBM traffic (FETCH_SIZE + WRITE_SIZE, scaled): {tb / 1e6:.1f} MB "
+                            f"per launch, {tb / rf['alg_bytes_per_launch']:.2f} x the algorithmic "
+                            f"bytes; {tb / (mean_ms / 1e3) / 1e9:.0f} GB/s while it runs.", ""]
     return "\n".join(out)
 
 
