@@ -17,8 +17,9 @@ sleeping photons to the window's last step.  --warmup steps run first on a
 separate, smaller copy of the workload (kernel load, allocator warm-up).
 
 Prints one JSON line.  Also reported: the dominant kernel's achieved
-algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around its
-launches on every step of the timed window, --timing-every), the whole step's
+algorithmic HBM bandwidth against the 8 TB/s peak (HIP events around the
+electron QED kernel's launches on every step of the timed window, and around
+every kernel's on every --timing-every-th step), the whole step's
 algorithmic bandwidth, and the CPU oracle timed on this machine's cores over
 the same window on a sample of the same state.  --summarize-profile turns
 the rocprofv3 output of a profiled run into the markdown kept under
@@ -59,6 +60,10 @@ HBM_PEAK_GBS = 8000.0
 # the CPU baseline times the oracle over at most this many steps (the
 # collision centre when the GPU window is longer): about 10-30 s of CPU work
 CPU_WINDOW = 60
+# the kernel timed with HIP events at every step of the timed window (the
+# electron QED kernel, which dominates the step); the others are timed on the
+# steps --timing-every selects, and scaled by their own timed launches
+TIMED_KERNEL = (0, 1)
 
 # Algorithmic bytes of the two step kernels of a species (the persistent
 # state read once and written once; transients stay in registers):
@@ -243,7 +248,7 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
     done = 0
     while done < steps:  # chunks, so that progress is visible during long runs
         m = min(chunk, steps - done)
-        sim.run_steps(m, timing=True, timing_every=timing_every)
+        sim.run_steps(m, timing=True, timing_every=timing_every, timing_kernel=TIMED_KERNEL)
         done += m
         print(f"[bench] {precision}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s, "
               f"{[s.N_buffered for s in species]}", file=sys.stderr, flush=True)
@@ -727,11 +732,11 @@ def main():
     ap.add_argument("--field", choices=["gaussian", "focused"], default="gaussian",
                     help="head-on workload: the paraxial Gaussian pulse (the benchmark) or the "
                          "same pulse in the focused expansion")
-    ap.add_argument("--timing-every", type=int, default=1,
-                    help="time the step kernels of every k-th step with HIP events (default: "
-                         "every step, so that the dominant kernel's mean launch time is the "
-                         "mean over the whole window, as a kernel trace of the same run gives "
-                         "it; the events cost about 2%% of the step time)")
+    ap.add_argument("--timing-every", type=int, default=5,
+                    help="time every step kernel with HIP events on every k-th step (default 5, "
+                         "coprime to the photon list's 16-step rebuild period); the electron QED "
+                         "kernel, which dominates the step, is timed at every step.  Events "
+                         "around every launch of every step cost 5%% of the step time")
     ap.add_argument("--chunk", type=int, default=250,
                     help="head-on: steps per Simulation.run_steps call (a progress line each)")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -811,16 +816,17 @@ def main():
         elapsed, stats, ktime, counts, spectrum, start, base, qd = gpu_run(
             n, args.steps, args.warmup, nsteps_config, precision, args.chunk, args.timing_every,
             args.field, cpu_every if precision == "fp64" else 0)
-        # the kernels of every timing_every-th step are timed (HIP events on
-        # the simulation's stream); sums scale to all steps by steps / timed
-        timed = sum(1 for k in range(start, start + args.steps) if k % args.timing_every == 0)
-        scale = args.steps / max(timed, 1)
+        # HIP events on the simulation's stream around the launches of every
+        # timing_every-th step, and of every step for TIMED_KERNEL: a kernel's
+        # time over the window is its mean timed launch times the steps
+        def est(i, p):
+            ms_, n_ = ktime[i][p]
+            return ms_ / n_ * args.steps if n_ else 0.0
         psteps = sum(s["psteps"] for s in stats)
         asleep = stats[2]["slept"]
-        # dominant kernel: largest accumulated time over (species, phase)
-        dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)),
-                      key=lambda ip: ktime[ip[0]][ip[1]][0])
-        ms = ktime[dom][ph][0] * scale
+        # dominant kernel: largest time over (species, phase)
+        dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)), key=lambda ip: est(*ip))
+        ms = est(dom, ph)
         sb = survey_bytes(dom == 2, ph, stats[dom], precision, args.steps)
         per_launch = (sb if sb is not None else kernel_bytes(dom == 2, ph, stats[dom])) / args.steps
         avg_s = ms / 1e3 / args.steps
@@ -831,7 +837,7 @@ def main():
                   (1, True): f"k_photon<RS, {fk} field>",
                   (2, False): f"k_emit<RS, {fk} field>"}[(ph, dom == 2)]
                  + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
-        total_ms = sum(k[p][0] for k in ktime for p in (0, 1, 2)) * scale
+        total_ms = sum(est(i, p) for i in range(3) for p in (0, 1, 2))
         # whole step at the survey's algorithmic bytes per particle-step: e+-
         # 105 B (FP64) or 81 B (mixed: FP64 x, FP32 u), photons 81 B per
         # executed (awake) photon-step; sleeping photons move no bytes
@@ -846,7 +852,7 @@ def main():
                     "alg_bytes": "survey per-particle-step bytes" if sb is not None
                     else "kernel's own state bytes (list, re-reads)",
                     "kernel_time_share": round(ms / max(1e-9, total_ms), 3),
-                    "timed_steps": timed,
+                    "timed_launches": ktime[dom][ph][1],
                     "whole_step": {"alg_bytes_per_step": int(step_bytes / args.steps),
                                    "GBps": round(step_bytes / elapsed / 1e9, 1),
                                    "frac": round(step_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4)}}
@@ -861,7 +867,7 @@ def main():
                   "photon_multistep_pushes": stats[2]["sleeps"],
                   "electron_dense_steps": stats[0]["dense_steps"],
                   "qed_diagnostics": {names[i]: qd[i] for i in range(3)},
-                  "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(ktime[i][p][0] * scale, 2)
+                  "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(est(i, p), 2)
                                 for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
         return detail, roofline, counts, spectrum, base, start

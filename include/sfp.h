@@ -390,6 +390,12 @@ int sfp_sim_reset_timing(void* sim);
  * the timed run closer to an untimed one.  No counterpart in the reference
  * (instrumentation). */
 int sfp_sim_set_timing_every(void* sim, int64_t every);
+/* With SFP_RUN_TIMING, time one kernel -- species, phase (0 sweep, 1 QED,
+ * 2 emission) -- at every step besides the steps sfp_sim_set_timing_every
+ * selects for all kernels (species < 0: none): the mean launch time of the
+ * kernel a caller prices over a whole run, at two events per step instead
+ * of two per launch.  No counterpart in the reference (instrumentation). */
+int sfp_sim_set_timing_kernel(void* sim, int32_t species, int32_t phase);
 /* The stable ranks of appended particles come from a decoupled look-back
  * over per-tile flag words tagged with a 29-bit launch epoch.  The library
  * zeroes the flags and epochs between batches every `steps` steps (default

@@ -2885,6 +2885,7 @@ struct Sim {
   int fm = SFP_FM_ANY;
   bool timing = false;
   int64_t timing_every = 1;         // with timing: the steps whose index it divides
+  int time_s = -1, time_ph = -1;    // with timing: a kernel timed at every step as well
   std::vector<EvRec> ev_pending[SFP_MAX_SPECIES][NPH];
   double ms_acc[SFP_MAX_SPECIES][NPH] = {};
   int64_t launches[SFP_MAX_SPECIES][NPH] = {};
@@ -3366,7 +3367,8 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
   const StepArgs& ah = sim->args_host[j];
   if (tune >= 0 && (ah.wrel % WIN == 0 || ah.rel == 0)) tune = -2;  // run, not timed
-  const bool timed = sim->timing && ah.step % (uint64_t)sim->timing_every == 0;
+  const bool timed = sim->timing && (ah.step % (uint64_t)sim->timing_every == 0 ||
+                                     (s == sim->time_s && ph == sim->time_ph));
   const bool ev = timed || tune >= 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ev) {
@@ -3869,6 +3871,20 @@ extern "C" int sfp_sim_set_timing_every(void* p, int64_t every) {
   if (every < 1) return set_err(SFP_E_ARG, "timing_every must be >= 1, got %lld", (long long)every);
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   sim->timing_every = every;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_timing_kernel(void* p, int32_t species, int32_t phase) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  if (species < 0) {
+    sim->time_s = sim->time_ph = -1;
+    return SFP_OK;
+  }
+  if (species >= sim->nsp || phase < 0 || phase >= NPH)
+    return set_err(SFP_E_ARG, "no kernel (species %d, phase %d)", species, phase);
+  sim->time_s = species;
+  sim->time_ph = phase;
   return SFP_OK;
 }
 

@@ -178,13 +178,17 @@ class Simulation(object):
             p._push_count_to_device()
         self._build()
 
-    def run_steps(self, nsteps: int, timing: bool = False, timing_every: int = 1) -> None:
+    def run_steps(self, nsteps: int, timing: bool = False, timing_every: int = 1,
+                  timing_kernel=None) -> None:
         """Advance nsteps without callbacks, progress or pruning.  With timing,
         the step kernels of the steps whose index timing_every divides are
-        timed with HIP events (kernel_time)."""
+        timed with HIP events (kernel_time), and the kernel timing_kernel =
+        (species, phase) names at every step."""
         self._prepare()
         if timing:
             L.call("sfp_sim_set_timing_every", self._sim, int(timing_every))
+            s_, ph_ = timing_kernel if timing_kernel is not None else (-1, 0)
+            L.call("sfp_sim_set_timing_kernel", self._sim, int(s_), int(ph_))
         t = C.c_double(self.t)
         flags = (L.RUN_TIMING if timing else 0) | (L.RUN_GRAPH if self.graph else 0) | \
             (L.RUN_AUTOTUNE if self.autotune else 0)

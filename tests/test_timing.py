@@ -58,6 +58,40 @@ def test_sampled_timing_counts_the_sampled_steps_and_changes_nothing():
             assert np.array_equal(np.asarray(getattr(a, k)), np.asarray(getattr(b, k)))
 
 
+@pytest.mark.gpu
+def test_timing_kernel_is_timed_at_every_step():
+    """The kernel named by timing_kernel (the electron QED kernel, as the
+    benchmark names it) is timed at every step, the others on the sampled
+    steps only; results unchanged."""
+    case = head_on_case(3000, 80, seed=22, a0=150.0)
+    ref, rs = _make(case)
+    ref.run_steps(80)
+    sim, ss = _make(case)
+    sim.run_steps(80, timing=True, timing_every=5, timing_kernel=(0, 1))
+    ms0, n0 = sim.kernel_time(0, 0)
+    ms1, n1 = sim.kernel_time(0, 1)
+    ms2, n2 = sim.kernel_time(2, 1)
+    print("electron sweep", n0, "launches; electron QED", n1, "; photon QED", n2)
+    assert sim.stats(0)["replays"] == 0
+    assert n1 == 80 and ms1 > 0.0
+    assert n0 == n2 == len(range(0, 80, 5))
+    for a, b in zip(rs, ss):
+        assert a.N_buffered == b.N_buffered
+        for k in KEYS:
+            assert np.array_equal(np.asarray(getattr(a, k)), np.asarray(getattr(b, k)))
+
+
+def test_timing_kernel_rejects_a_missing_kernel():
+    """Checked on the host before anything else (no device needed)."""
+    import os
+    from sfparticles_amd import _lib as L
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("library not built")
+    rc = L.load().sfp_sim_set_timing_kernel(None, 0, 1)
+    with pytest.raises(L.SFPError, match="null simulation"):
+        L.check(rc)
+
+
 def test_timing_every_rejects_zero():
     """Checked on the host before anything else (no device needed)."""
     import os
