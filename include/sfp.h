@@ -463,6 +463,11 @@ int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
  *      results do not change) */
 #define SFP_SIM_NSTATS 17
 int sfp_sim_stats_n(void* sim, int32_t species, int64_t* stats, int32_t n);
+/* Count and next id of one species as the last sfp_sim_run left them (read
+ * by the run at its end: no device access; the same values
+ * sfp_store_get_count returns until the store changes again).  No
+ * counterpart in the reference. */
+int sfp_sim_get_counts(void* sim, int32_t species, int64_t* n, uint64_t* next_id);
 /* Diagnostics of the exact QED evaluations of one species since the last
  * reset (synchronous; no counterpart in the reference, whose behaviour in
  * these cases is undefined or unreported):

@@ -152,6 +152,7 @@ SIGNATURES = {
     "sfp_sim_reset_timing": [P],
     "sfp_sim_set_timing_every": [P, I64],
     "sfp_sim_set_timing_kernel": [P, I32, I32],
+    "sfp_sim_get_counts": [P, I32, P, P],
     "sfp_store_histogram": [P, I32, I32, I32, D, D, P],
     "sfp_sim_set_grid": [P, I32, I32, I32],
     "sfp_sim_set_shape": [P, I32, I32, I32, I32],

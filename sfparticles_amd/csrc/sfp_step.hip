@@ -2903,12 +2903,16 @@ struct Sim {
   int32_t nshots = 0;
 };
 
+// The counts of every species and the halt record, copied behind the step's
+// kernels into pinned host memory with one wait (a synchronous copy each
+// took a host round trip of its own)
 static int sim_read_counts(Sim* sim) {
-  HIPCHK(hipDeviceSynchronize());
   for (int s = 0; s < sim->nsp; ++s)
-    HIPCHK(hipMemcpy(&sim->counts_host[s], sim->stores[s]->cnt, sizeof(Counts),
-                     hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(sim->halt_host, sim->halt, sizeof(Halt), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&sim->counts_host[s], sim->stores[s]->cnt, sizeof(Counts),
+                          hipMemcpyDeviceToHost, sim->stream));
+  HIPCHK(hipMemcpyAsync(sim->halt_host, sim->halt, sizeof(Halt), hipMemcpyDeviceToHost,
+                        sim->stream));
+  HIPCHK(hipStreamSynchronize(sim->stream));
   sim->syncs++;
   return SFP_OK;
 }
@@ -3871,6 +3875,15 @@ extern "C" int sfp_sim_set_timing_every(void* p, int64_t every) {
   if (every < 1) return set_err(SFP_E_ARG, "timing_every must be >= 1, got %lld", (long long)every);
   if (!sim) return set_err(SFP_E_ARG, "null simulation");
   sim->timing_every = every;
+  return SFP_OK;
+}
+
+extern "C" int sfp_sim_get_counts(void* p, int32_t species, int64_t* n, uint64_t* next_id) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  if (species < 0 || species >= sim->nsp) return set_err(SFP_E_ARG, "no species %d", species);
+  if (n) *n = sim->counts_host[species].n;
+  if (next_id) *next_id = (uint64_t)sim->counts_host[species].next_id;
   return SFP_OK;
 }
 

@@ -208,8 +208,12 @@ class Particles(object):
         n = C.c_int64(0)
         nid = C.c_uint64(0)
         L.call("sfp_store_get_count", self._store, C.byref(n), C.byref(nid))
-        self.N_buffered = int(n.value)
-        self.next_id = int(nid.value)
+        self._set_count(int(n.value), int(nid.value))
+
+    def _set_count(self, n: int, next_id: int):
+        """The host's copy of the device count (after a run or a device read)."""
+        self.N_buffered = n
+        self.next_id = next_id
         if self.buffer_size < self.N_buffered:
             self.buffer_size = self.N_buffered
         self._touch()

@@ -195,8 +195,13 @@ class Simulation(object):
         L.call("sfp_sim_run", self._sim, C.byref(t), int(self.step), int(nsteps), flags)
         self.t = t.value
         self.step += int(nsteps)
-        for p in self.all_particles:
-            p._sync_count_from_device()
+        for i, p in enumerate(self.all_particles):
+            if nsteps > 0:  # the counts the run read at its end
+                n, nid = C.c_int64(0), C.c_uint64(0)
+                L.call("sfp_sim_get_counts", self._sim, i, C.byref(n), C.byref(nid))
+                p._set_count(int(n.value), int(nid.value))
+            else:
+                p._sync_count_from_device()
             object.__setattr__(p, "_ig_stale", True)
 
     def kernel_time(self, species: int, phase: int):
