@@ -216,7 +216,7 @@ def sample_state(species, every):
 
 
 def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing_every=1,
-            field="gaussian", cpu_every=0):
+            field="gaussian", cpu_every=0, shared=True):
     from sfparticles_amd import _lib as L
     if warmup > 0:
         wsim, _ = build_gpu(min(n, 100000), nsteps_config, (0, 0, 0), precision, field)
@@ -224,6 +224,8 @@ def gpu_run(n, steps, warmup, nsteps_config, precision="fp64", chunk=250, timing
         del wsim
     reserve = (2 * n + 4096, n + 4096, 28 * n + 4096)
     sim, species = build_gpu(n, nsteps_config, reserve, precision, field)
+    if not shared:
+        sim.set_shared_launches(False)
     # a run shorter than the geometry times the window of `steps` steps
     # centred on the collision (step nsteps_config / 2: the pulse centre
     # meets the bunch centre), after an untimed fast-forward
@@ -356,19 +358,35 @@ def _csv_rows(d, suffix):
 def _by_species(seq):
     """Names of the per-species kernels tagged with their species: a step
     launches the lepton kernels for electrons, then for positrons, then the
-    photon kernels, each step opening with k_step_prologue."""
-    out, seen = [], {}
+    photon kernels, each step opening with k_step_prologue.  With shared QED
+    launches a step sweeps both lepton species and then runs one launch of
+    the QED (and emission) kernel for both: tagged [e-,e+]."""
+    steps, cur = [], []
     for r in seq:
-        name = r[0]
-        if name.startswith("k_step_prologue"):
-            seen = {}
-        elif name.startswith(("k_sweep", "k_hot", "k_emit", "k_defer")):
-            k = seen.get(name, 0)
-            seen[name] = k + 1
-            name = f"{name} [{('e-', 'e+')[min(k, 1)]}]"
-        elif name.startswith(("k_awake", "k_photon")):
-            name = f"{name} [photon]"
-        out.append((name,) + tuple(r[1:]))
+        if r[0].startswith("k_step_prologue") and cur:
+            steps.append(cur)
+            cur = []
+        cur.append(r)
+    if cur:
+        steps.append(cur)
+    out = []
+    for st in steps:
+        n = {}
+        for r in st:
+            n[r[0]] = n.get(r[0], 0) + 1
+        sweeps = sum(v for k, v in n.items() if k.startswith("k_sweep<false"))
+        seen = {}
+        for r in st:
+            name = r[0]
+            if name.startswith(("k_hot", "k_emit")) and n[name] == 1 and sweeps == 2:
+                name = f"{name} [e-,e+]"
+            elif name.startswith(("k_sweep", "k_hot", "k_emit", "k_defer")):
+                k = seen.get(name, 0)
+                seen[name] = k + 1
+                name = f"{name} [{('e-', 'e+')[min(k, 1)]}]"
+            elif name.startswith(("k_awake", "k_photon")):
+                name = f"{name} [photon]"
+            out.append((name,) + tuple(r[1:]))
     return out
 
 
@@ -526,7 +544,8 @@ def summarize_profile(d):
             # the roofline kernel serves one species per launch; the trace mean
             # over its dispatches of the species the line names
             sp_tag = "[photon]" if "(photon)" in rf["kernel"] else (
-                "[e+]" if "(positron)" in rf["kernel"] else "[e-]")
+                "[e+]" if "(positron)" in rf["kernel"] else (
+                    "[e-,e+]" if "positrons" in rf["kernel"] else "[e-]"))
             cand = [n for n in t if key and n.startswith(key) and n.endswith(sp_tag)]
             if cand:
                 name = max(cand, key=lambda n: sum(t[n]))
@@ -727,6 +746,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=float, default=1e7)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--separate-launches", action="store_true",
+                    help="one QED launch per lepton species (default: electrons and "
+                         "positrons share theirs; the results are the same)")
     ap.add_argument("--cpu-every", type=int, default=20,
                     help="CPU baseline: every k-th particle of the GPU state at the window start")
     ap.add_argument("--field", choices=["gaussian", "focused"], default="gaussian",
@@ -815,7 +837,8 @@ def main():
     def measure(precision):
         elapsed, stats, ktime, counts, spectrum, start, base, qd = gpu_run(
             n, args.steps, args.warmup, nsteps_config, precision, args.chunk, args.timing_every,
-            args.field, cpu_every if precision == "fp64" else 0)
+            args.field, cpu_every if precision == "fp64" else 0,
+            shared=not args.separate_launches)
         # HIP events on the simulation's stream around the launches of every
         # timing_every-th step, and of every step for TIMED_KERNEL: a kernel's
         # time over the window is its mean timed launch times the steps
@@ -824,11 +847,24 @@ def main():
             return ms_ / n_ * args.steps if n_ else 0.0
         psteps = sum(s["psteps"] for s in stats)
         asleep = stats[2]["slept"]
+        # shared QED launches: the positrons' QED kernels run inside the
+        # electrons' launches (no launch of their own was timed)
+        shared = stats[1]["hot"] > 0 and ktime[0][1][1] > 0 and ktime[1][1][1] == 0
         # dominant kernel: largest time over (species, phase)
         dom, ph = max(((i, p) for i in range(3) for p in (0, 1, 2)), key=lambda ip: est(*ip))
         ms = est(dom, ph)
-        sb = survey_bytes(dom == 2, ph, stats[dom], precision, args.steps)
-        per_launch = (sb if sb is not None else kernel_bytes(dom == 2, ph, stats[dom])) / args.steps
+        both = shared and dom == 0 and ph > 0  # the launch serves both lepton species
+
+        def launch_bytes(i):
+            sb_ = survey_bytes(i == 2, ph, stats[i], precision, args.steps)
+            return sb_, (sb_ if sb_ is not None else kernel_bytes(i == 2, ph, stats[i]))
+        sb, per_launch = launch_bytes(dom)
+        if both:
+            sb1, _ = launch_bytes(1)
+            sb = sb + sb1 if sb is not None and sb1 is not None else None
+            per_launch = (sb if sb is not None else
+                          kernel_bytes(False, ph, stats[0]) + kernel_bytes(False, ph, stats[1]))
+        per_launch /= args.steps
         avg_s = ms / 1e3 / args.steps
         achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
         fk = "gaussian" if args.field == "gaussian" else "general"
@@ -836,7 +872,8 @@ def main():
                   (1, False): f"k_hot<RS, {fk} field>", (0, True): "k_awake",
                   (1, True): f"k_photon<RS, {fk} field>",
                   (2, False): f"k_emit<RS, {fk} field>"}[(ph, dom == 2)]
-                 + f"{'' if precision == 'fp64' else ', ' + precision} ({names[dom]})")
+                 + f"{'' if precision == 'fp64' else ', ' + precision} "
+                 + ("(electrons and positrons, one launch)" if both else f"({names[dom]})"))
         total_ms = sum(est(i, p) for i in range(3) for p in (0, 1, 2))
         # whole step at the survey's algorithmic bytes per particle-step: e+-
         # 105 B (FP64) or 81 B (mixed: FP64 x, FP32 u), photons 81 B per
@@ -868,7 +905,8 @@ def main():
                   "electron_dense_steps": stats[0]["dense_steps"],
                   "qed_diagnostics": {names[i]: qd[i] for i in range(3)},
                   "kernel_ms": {f"{names[i]}_{('sweep', 'hot', 'emit')[p]}": round(est(i, p), 2)
-                                for i in range(3) for p in (0, 1, 2) if p < 2 or ktime[i][p][1]},
+                                for i in range(3) for p in (0, 1, 2) if p == 0 or ktime[i][p][1]},
+                  "shared_lepton_launches": shared,
                   "host_syncs": stats[0]["syncs"], "capacity_replays": stats[0]["replays"]}
         return detail, roofline, counts, spectrum, base, start
 

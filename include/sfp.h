@@ -403,6 +403,14 @@ int sfp_sim_set_timing_kernel(void* sim, int32_t species, int32_t phase);
  * smaller interval only exercises that clearing (tests).  No counterpart in
  * the reference. */
 int sfp_sim_set_flag_clear_every(void* sim, int64_t steps);
+/* Shared QED launches (on by default): two consecutive lepton species with
+ * the same QED kernels and photon store -- electrons and positrons -- run in
+ * one launch of the lepton QED kernel (and of the emission kernel), the
+ * second species' photons appended after all of the first's, as with one
+ * launch per species (on == 0), whose results are bit-identical.  The second
+ * species' QED launch shapes are then unused.  No counterpart in the
+ * reference (simulation.py:126-137 steps one species after the other). */
+int sfp_sim_set_shared_launches(void* sim, int32_t on);
 /* Ensemble of independent shots (no counterpart in the reference): shot s has
  * its own field, the nterms[s] terms starting at terms + sum_{j<s} nterms[j].
  * Every particle of a species with a SFP_SHOT attribute moves in the field of

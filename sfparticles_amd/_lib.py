@@ -108,6 +108,7 @@ SIGNATURES = {
     "sfp_find_event_index_noalloc": [P, I64, P, P, P],
     "sfp_sim_stats_n": [P, I32, P, I32],
     "sfp_sim_set_flag_clear_every": [P, I64],
+    "sfp_sim_set_shared_launches": [P, I32],
     "sfp_sim_qed_diag": [P, I32, P, C.POINTER(D)],
     "sfp_photon_from_rejection_sampling": [P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],
     "sfp_pair_from_rejection_sampling": [P, P, D, I64, P, P, P, P, U64, U64, U32, U32, P, P],

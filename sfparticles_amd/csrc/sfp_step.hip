@@ -239,6 +239,30 @@ __device__ uint64_t lookback(uint64_t* flags, int64_t tile, uint64_t agg, uint32
   return prefix;
 }
 
+// A workgroup-uniform value read from LDS, marked uniform for the compiler
+// (scalar registers): what is derived from it -- a species index, its
+// store's pointers -- then stays in scalar registers and scalar loads.
+__device__ inline int64_t wg_uniform(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// The total of a look-back chain of ntiles tiles: its last tile's inclusive
+// prefix, waited for until that tile has published it in this epoch.  Safe
+// to wait on from a workgroup that has no tile of the chain left to do: every
+// tile of the chain is then held by a running workgroup, whose own look-back
+// waits only on tiles before it.
+__device__ uint64_t chain_total(const uint64_t* flags, int64_t ntiles, uint32_t epoch) {
+  if (ntiles <= 0) return 0;
+  while (true) {
+    const uint64_t w = ld_flag(&flags[ntiles - 1]);
+    if ((uint32_t)(w >> FLAG_EPOCH_SHIFT) == epoch && ((w >> FLAG_VAL_BITS) & 3u) == 2u)
+      return w & FLAG_VAL_MASK;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 // Stable rank of `pred` among all lanes of all tiles processed so far:
 // exclusive prefix over (tile, wave, lane).  Called by every thread.
 
@@ -1430,60 +1454,100 @@ __device__ inline void finish_appends(const Plan& P, int s, int ph, int64_t ntil
 // ran faster (a 1e6-electron focused-pulse collision: 3.2e10 -> 3.5e10
 // particle-steps/s).  The LDS of a workgroup (37-50 KB) allows at most 4
 // workgroups per CU.
+//
+// One launch serves one lepton species or two (`sel`: the first in bits 0-3,
+// the second plus one in bits 4-7, 0 for none) that share the kernel and the
+// photon store: the tiles of the first species' list, then those of the
+// second's, come from one counter; each species ranks its events in a
+// look-back chain of its own, and the second appends its photons after all
+// of the first's (it waits for the first chain's total before its first
+// append), so the store's order is that of two launches in species order.
+// The second species' tables and launch gap are saved: it is the positrons
+// of a collision, a few thousand particles whose own launch was a chain of
+// memory latencies (about 20 us) with the GPU otherwise idle.
 template <int KIND, int FM, bool MIX, int NETK, bool SH>
-__global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+__global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM == SFP_FM_ANY) ? 2 : 1)) k_hot(const Plan* __restrict__ plan, int sel, const StepArgs* __restrict__ ap,
       int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
-  const SpeciesDev& sp = P.sp[s];
   using Node = typename std::conditional<MIX, sfp_rs_node32, sfp_rs_node>::type;
 
   __shared__ Node lds_rs[KIND == KIND_RS ? 1023 : 1];
   constexpr bool NET = NETK == 1;
   __shared__ double2 lds_od[(KIND == KIND_OD || NET) ? 255 : 1];
   __shared__ int64_t sh_tile;
+  __shared__ int64_t sh_off;
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_pref;
   __shared__ uint64_t sh_mask[NETK == 2 ? 1 : QROWS][WAVES];
   __shared__ uint16_t sh_q[NETK == 2 ? 1 : QROWS * TILE];  // tile positions (< 2^16)
-  __shared__ uint32_t sh_qd[QD_N];
+  __shared__ uint32_t sh_qd[2][QD_N];
 
-  const int64_t n_hot = sp.cnt->n_hot;
-  // tiles of up to rmax (<= QROWS) rows, or of one row for the rate network
-  const int rows = NETK == 2 ? 1 : qed_rows(n_hot, rmax);
-  const int64_t span = (int64_t)rows * TILE;
-  const int64_t ntiles = (n_hot + span - 1) / span;
+  const int nseg = (sel >> 4) ? 2 : 1;
+  const int sid0 = sel & 15, sid1 = nseg == 2 ? (sel >> 4) - 1 : sid0;
+  // per species: tiles of up to rmax (<= QROWS) rows, or of one row for the
+  // rate network
+  int64_t n_hot_g[2] = {0, 0}, ntiles_g[2] = {0, 0};
+  int rows_g[2] = {1, 1};
+  uint32_t epoch_g[2] = {0, 0};
+  for (int g = 0; g < nseg; ++g) {
+    const int sg = g ? sid1 : sid0;
+    n_hot_g[g] = P.sp[sg].cnt->n_hot;
+    rows_g[g] = NETK == 2 ? 1 : qed_rows(n_hot_g[g], rmax);
+    const int64_t sp_g = (int64_t)rows_g[g] * TILE;
+    ntiles_g[g] = (n_hot_g[g] + sp_g - 1) / sp_g;
+    epoch_g[g] = next_epoch(P.scan[sg][1]);
+  }
+  const int64_t ntiles_all = ntiles_g[0] + ntiles_g[1];
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
-  const uint32_t nwg = participants(ntiles);
+  const uint32_t nwg = participants(ntiles_all);
   if (blockIdx.x >= nwg) return;
-  const SpeciesDev* dA = sp.photon_dst >= 0 ? &P.sp[sp.photon_dst] : nullptr;
-  if (!room_or_halt(P, s, a, dA, nullptr, n_hot)) return;
-  const int64_t baseA = dA ? dA->cnt->n : 0;
-  const int64_t idA = dA ? dA->cnt->next_id : 0;
-  ScanState* scan = P.scan[s][1];
-  const uint32_t epoch = next_epoch(scan);
-  // dense step (set by this step's sweep): the list is the identity over the
-  // store, holes included, and every particle gets the exact evaluation (a
-  // particle the sweep's bound would have culled gives "no event" there, and
-  // is pushed as the sweep pushes it)
-  const bool dense = sp.cnt->dense != 0;
+  const SpeciesDev* dA = P.sp[sid0].photon_dst >= 0 ? &P.sp[P.sp[sid0].photon_dst] : nullptr;
+  if (!room_or_halt(P, sid0, a, dA, nullptr, n_hot_g[0] + n_hot_g[1])) return;
+  const int64_t base0 = dA ? dA->cnt->n : 0;
+  const int64_t id0 = dA ? dA->cnt->next_id : 0;
+  ScanState* scan = P.scan[sid0][1];
 
-  uint32_t alive = 0, proc = 0;
-  qdiag_init(sh_qd);
-  QDiag dg{(SFP_L uint32_t*)sh_qd};
+  // per species: the workgroup's pushed and processed counts (its tiles of
+  // the first species all come before those of the second)
+  uint32_t alive = 0, proc = 0, alive0 = 0, proc0 = 0;
+  bool second = false;
+  // the first species' appends, where the second's start (waited for once)
+  int64_t off = 0;
+  bool have_off = false;
+  qdiag_init(&sh_qd[0][0]);
+  qdiag_init(&sh_qd[1][0]);
   bool staged = false;
   if constexpr (NETK != 2) {
-  const int64_t SPAN = span;
   const int lane = wave_lane();
   const int wv = threadIdx.x / 64;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
-    const int64_t tile = sh_tile;
+    const int64_t vt = wg_uniform(sh_tile);
     __syncthreads();
-    if (tile >= ntiles) break;
+    if (vt >= ntiles_all) break;
+    const int g = vt >= ntiles_g[0] ? 1 : 0;
+    if (g && !second) {
+      alive0 = alive; proc0 = proc;
+      alive = 0; proc = 0;
+      second = true;
+    }
+    const int s = g ? sid1 : sid0;
+    const SpeciesDev& sp = P.sp[s];
+    const int64_t tile = vt - (g ? ntiles_g[0] : 0);
+    const int64_t n_hot = g ? n_hot_g[1] : n_hot_g[0];
+    const int rows = g ? rows_g[1] : rows_g[0];
+    const int64_t SPAN = (int64_t)rows * TILE;
+    const uint32_t epoch = g ? epoch_g[1] : epoch_g[0];
+    QDiag dg{(SFP_L uint32_t*)&sh_qd[g][0]};
+    // dense step (set by this step's sweep): the list is the identity over
+    // the store, holes included, and every particle gets the exact evaluation
+    // (a particle the sweep's bound would have culled gives "no event" there,
+    // and is pushed as the sweep pushes it)
+    const bool dense = sp.cnt->dense != 0;
     // tables go to LDS once the workgroup has work (a short list leaves most
     // of the persistent grid without a tile)
     if (!staged) {
@@ -1605,6 +1669,13 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
     // photon and pushes the lepton (queue order = index order, so the
     // appends keep their order).
     const uint32_t nev = tile_events(sh_mask, rows, P.flags[s][1], tile, epoch, sh_q, &sh_pref);
+    if (!NET && g && !have_off) {  // the second species appends after the first
+      if (threadIdx.x == 0) sh_off = (int64_t)chain_total(P.flags[sid0][1], ntiles_g[0], epoch_g[0]);
+      __syncthreads();
+      off = wg_uniform(sh_off);
+      have_off = true;
+    }
+    const int64_t baseA = base0 + off, idA = id0 + off;
     for (uint32_t e = threadIdx.x; e < nev; e += TILE) {
       const int64_t k = tile * SPAN + (int64_t)sh_q[e];
       const int64_t rank = (int64_t)sh_pref + (int64_t)e;
@@ -1689,9 +1760,22 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
-    const int64_t tile = sh_tile;
+    const int64_t vt = wg_uniform(sh_tile);
     __syncthreads();
-    if (tile >= ntiles) break;
+    if (vt >= ntiles_all) break;
+    const int g = vt >= ntiles_g[0] ? 1 : 0;
+    if (g && !second) {
+      alive0 = alive; proc0 = proc;
+      alive = 0; proc = 0;
+      second = true;
+    }
+    const int s = g ? sid1 : sid0;
+    const SpeciesDev& sp = P.sp[s];
+    const int64_t tile = vt - (g ? ntiles_g[0] : 0);
+    const int64_t n_hot = g ? n_hot_g[1] : n_hot_g[0];
+    const uint32_t epoch = g ? epoch_g[1] : epoch_g[0];
+    QDiag dg{(SFP_L uint32_t*)&sh_qd[g][0]};
+    const bool dense = sp.cnt->dense != 0;
     // tables go to LDS once the workgroup has work (a short list leaves most
     // of the persistent grid without a tile)
     if (!staged) {
@@ -1803,6 +1887,13 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
     if (!dA) event = false;
 
     int64_t rank = tile_rank(event, tile, P.flags[s][1], epoch, sh_w, &sh_pref);
+    if (g && !have_off) {  // the second species appends after the first
+      if (threadIdx.x == 0) sh_off = (int64_t)chain_total(P.flags[sid0][1], ntiles_g[0], epoch_g[0]);
+      __syncthreads();
+      off = wg_uniform(sh_off);
+      have_off = true;
+    }
+    const int64_t baseA = base0 + off, idA = id0 + off;
     if (event) {
       if (MIX) {
         const float df = (float)delta;
@@ -1845,26 +1936,40 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
   }
   }
 
-  const uint32_t live = qdiag_flush(sp.cnt, sh_qd);
-  uint32_t cnt[2] = {alive, proc};
-  uint64_t tot[2];
+  if (!second) {
+    alive0 = alive; proc0 = proc;
+    alive = 0; proc = 0;
+  }
+  uint32_t cnt[4] = {alive0, proc0, alive, proc};
+  uint64_t tot[4];
   block_sum(cnt, sh_w, tot);
-  if (threadIdx.x == 0) {
-    count_add(&sp.cnt->psteps, tot[0]);
-    count_add(&sp.cnt->processed, tot[1]);
-    // dense step: the particles with a possible event decide the next
-    // step's mode (the sweep counts its list otherwise)
-    if (dense) count_add(&sp.cnt->dense_listed[a.step & 1u], live);
+  for (int g = 0; g < nseg; ++g) {
+    const SpeciesDev& sp = P.sp[g ? sid1 : sid0];
+    const uint32_t live = qdiag_flush(sp.cnt, &sh_qd[g][0]);
+    if (threadIdx.x == 0) {
+      count_add(&sp.cnt->psteps, tot[2 * g]);
+      count_add(&sp.cnt->processed, tot[2 * g + 1]);
+      // dense step: the particles with a possible event decide the next
+      // step's mode (the sweep counts its list otherwise)
+      if (sp.cnt->dense != 0) count_add(&sp.cnt->dense_listed[a.step & 1u], live);
+    }
   }
   kernel_exit(scan, nwg, [&]() {
-    if (NET) {
-      const uint64_t c = ntiles > 0 ? (ld_flag(&P.flags[s][1][ntiles - 1]) & FLAG_VAL_MASK) : 0;
-      sp.cnt->n_emit = (int64_t)c;
-      sp.cnt->emit_total += (int64_t)c;
-    } else {
-      finish_appends(P, s, 1, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
+    int64_t done = 0;  // the first species' appends
+    for (int g = 0; g < nseg; ++g) {
+      const int s = g ? sid1 : sid0;
+      const SpeciesDev& sp = P.sp[s];
+      const int64_t nt = ntiles_g[g];
+      const uint64_t c = nt > 0 ? (ld_flag(&P.flags[s][1][nt - 1]) & FLAG_VAL_MASK) : 0;
+      if (NET) {
+        sp.cnt->n_emit = (int64_t)c;
+        sp.cnt->emit_total += (int64_t)c;
+      } else {
+        finish_appends(P, s, 1, nt, sp, dA, nullptr, base0 + done, 0, id0 + done, 0);
+      }
+      done += (int64_t)c;
+      P.scan[s][1]->epoch = epoch_g[g];
     }
-    scan->epoch = epoch;
   });
 }
 
@@ -1877,35 +1982,59 @@ __global__ void __launch_bounds__(TILE, (MIX && NETK == 2) ? 4 : ((!MIX && FM ==
 // (cpu.py:78-88) and push of the lepton.  The room for the appends was
 // checked by k_hot (one per listed particle).
 template <int KIND, int FM, bool SH>
-__global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, int s,
+__global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, int sel,
                                                const StepArgs* __restrict__ ap, int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
-  const SpeciesDev& sp = P.sp[s];
   __shared__ uint4 lds_w[2 * 4096 * 2 / 16];
   __shared__ int64_t sh_tile;
+  __shared__ int64_t sh_off;
   __shared__ uint32_t sh_w[2 * WAVES];
   __shared__ uint64_t sh_pref;
-  const int64_t n_c = sp.cnt->n_emit;
-  const int64_t ntiles = (n_c + TILE - 1) / TILE;
+  // one species or two, as in k_hot: the second's candidates follow the
+  // first's from one tile counter, and its photons follow the first's
+  const int nseg = (sel >> 4) ? 2 : 1;
+  const int sid0 = sel & 15, sid1 = nseg == 2 ? (sel >> 4) - 1 : sid0;
+  int64_t n_c_g[2] = {0, 0}, ntiles_g[2] = {0, 0};
+  uint32_t epoch_g[2] = {0, 0};
+  for (int g = 0; g < nseg; ++g) {
+    const int sg = g ? sid1 : sid0;
+    n_c_g[g] = P.sp[sg].cnt->n_emit;
+    ntiles_g[g] = (n_c_g[g] + TILE - 1) / TILE;
+    epoch_g[g] = next_epoch(P.scan[sg][2]);
+  }
+  const int64_t ntiles_all = ntiles_g[0] + ntiles_g[1];
   // workgroups without work leave before reading the destination counts,
   // which the exit of this launch changes
-  const uint32_t nwg = participants(ntiles);
+  const uint32_t nwg = participants(ntiles_all);
   if (blockIdx.x >= nwg) return;
-  const SpeciesDev* dA = sp.photon_dst >= 0 ? &P.sp[sp.photon_dst] : nullptr;
-  const int64_t baseA = dA ? dA->cnt->n : 0;
-  const int64_t idA = dA ? dA->cnt->next_id : 0;
-  ScanState* scan = P.scan[s][2];
-  const uint32_t epoch = next_epoch(scan);
-  uint32_t alive = 0;
+  const SpeciesDev* dA = P.sp[sid0].photon_dst >= 0 ? &P.sp[P.sp[sid0].photon_dst] : nullptr;
+  const int64_t base0 = dA ? dA->cnt->n : 0;
+  const int64_t id0 = dA ? dA->cnt->next_id : 0;
+  ScanState* scan = P.scan[sid0][2];
+  uint32_t alive = 0, alive0 = 0;
+  bool second = false;
+  int64_t off = 0;  // the first species' appends (waited for once)
+  bool have_off = false;
   bool staged = false;
   while (true) {
     if (threadIdx.x == 0) sh_tile = (int64_t)atomicAdd(&scan->tile_ctr, 1u);
     __syncthreads();
-    const int64_t tile = sh_tile;
+    const int64_t vt = wg_uniform(sh_tile);
     __syncthreads();
-    if (tile >= ntiles) break;
+    if (vt >= ntiles_all) break;
+    const int g = vt >= ntiles_g[0] ? 1 : 0;
+    if (g && !second) {
+      alive0 = alive;
+      alive = 0;
+      second = true;
+    }
+    const int s = g ? sid1 : sid0;
+    const SpeciesDev& sp = P.sp[s];
+    const int64_t tile = vt - (g ? ntiles_g[0] : 0);
+    const int64_t n_c = g ? n_c_g[1] : n_c_g[0];
+    const uint32_t epoch = g ? epoch_g[1] : epoch_g[0];
     if (!staged) {
       const uint4* w2 = (const uint4*)P.net->W2p;
       const uint4* w3 = (const uint4*)P.net->W3p;
@@ -1940,10 +2069,16 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
     bool event = valid && !(sfp32_div((float)delta, igf) < (float)P.threshold);
     if (!dA) event = false;
     const int64_t rank = tile_rank(event, tile, P.flags[s][2], epoch, sh_w, &sh_pref);
+    if (g && !have_off) {  // the second species appends after the first
+      if (threadIdx.x == 0) sh_off = (int64_t)chain_total(P.flags[sid0][2], ntiles_g[0], epoch_g[0]);
+      __syncthreads();
+      off = wg_uniform(sh_off);
+      have_off = true;
+    }
     if (event) {
       const float df = (float)delta;
-      spawn_photon(shot, *dA, a, baseA + rank, (uint64_t)(idA + rank), x, y, z, df * fx, df * fy,
-                   df * fz);
+      spawn_photon(shot, *dA, a, base0 + off + rank, (uint64_t)(id0 + off + rank), x, y, z,
+                   df * fx, df * fy, df * fz);
     }
     if (valid) {
       if (event && sp.rr == SFP_RR_PHOTON) {
@@ -1961,13 +2096,24 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
       alive += 1;
     }
   }
-  uint32_t cnt[1] = {alive};
-  uint64_t tot[1];
+  if (!second) {
+    alive0 = alive;
+    alive = 0;
+  }
+  uint32_t cnt[2] = {alive0, alive};
+  uint64_t tot[2];
   block_sum(cnt, sh_w, tot);
-  if (threadIdx.x == 0) count_add(&sp.cnt->psteps, tot[0]);
+  if (threadIdx.x == 0)
+    for (int g = 0; g < nseg; ++g) count_add(&P.sp[g ? sid1 : sid0].cnt->psteps, tot[g]);
   kernel_exit(scan, nwg, [&]() {
-    finish_appends(P, s, 2, ntiles, sp, dA, nullptr, baseA, 0, idA, 0);
-    scan->epoch = epoch;
+    int64_t done = 0;  // the first species' appends
+    for (int g = 0; g < nseg; ++g) {
+      const int s = g ? sid1 : sid0;
+      const int64_t nt = ntiles_g[g];
+      finish_appends(P, s, 2, nt, P.sp[s], dA, nullptr, base0 + done, 0, id0 + done, 0);
+      done += nt > 0 ? (int64_t)(ld_flag(&P.flags[s][2][nt - 1]) & FLAG_VAL_MASK) : 0;
+      P.scan[s][2]->epoch = epoch_g[g];
+    }
   });
 }
 
@@ -2834,6 +2980,12 @@ struct Sim {
   kfn hot[SFP_MAX_SPECIES] = {};
   kfn emit[SFP_MAX_SPECIES] = {};
   kfn_defer defer[SFP_MAX_SPECIES] = {};  // photons: deferred conversions
+  // two lepton species in one QED launch (k_hot, k_emit): fuse[s] is the
+  // species launched with s (-1: none), inner[s] marks a species whose QED
+  // kernels run in the launch of the species before it; fuse_on: allowed
+  int fuse[SFP_MAX_SPECIES] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  bool inner[SFP_MAX_SPECIES] = {};
+  bool fuse_on = true;
   // launch shapes: candidate shapes of each (species, phase) kernel, the one
   // in use, and a grid and tile rows the caller pinned (0: tuned)
   std::vector<Shape> cand[SFP_MAX_SPECIES][NPH];
@@ -3083,6 +3235,25 @@ static int sim_pick_kernels(Sim* sim) {
       }
     }
   }
+  // Two consecutive lepton species with the same QED kernels and photon store
+  // share their QED launches (the electrons and positrons of a collision):
+  // the second's launch shapes are unused
+  for (int s = 0; s < SFP_MAX_SPECIES; ++s) {
+    sim->fuse[s] = -1;
+    sim->inner[s] = false;
+  }
+  if (sim->fuse_on)
+    for (int s = 0; s + 1 < sim->nsp; ++s) {
+      const int b = s + 1;
+      if (sim->inner[s] || sim->plan.sp[s].photon || sim->plan.sp[b].photon || !sim->hot[s] ||
+          sim->hot[b] != sim->hot[s] || sim->emit[b] != sim->emit[s] ||
+          sim->desc[b].photon_dst != sim->desc[s].photon_dst)
+        continue;
+      sim->fuse[s] = b;
+      sim->inner[b] = true;
+      sim->cand[b][1].clear();
+      sim->cand[b][2].clear();
+    }
   sim->tuned = false;
   sim->graph_stale = true;
   return SFP_OK;
@@ -3380,8 +3551,10 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
     e1 = sim_event(sim);
     if (e0) HIPCHK(hipEventRecord(e0, sim->stream));
   }
+  // the QED kernels take a second species in bits 4-7 (plus one)
+  const int sel = (ph > 0 && sim->fuse[s] >= 0) ? (s | ((sim->fuse[s] + 1) << 4)) : s;
   hipLaunchKernelGGL(fn, dim3(cand[c].grid), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev,
-                     s, (const StepArgs*)(sim->args_dev + j), cand[c].rows);
+                     sel, (const StepArgs*)(sim->args_dev + j), cand[c].rows);
   HIPCHK(hipGetLastError());
   if (ev && e0 && e1) {
     HIPCHK(hipEventRecord(e1, sim->stream));
@@ -3398,8 +3571,14 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
     HIPCHK(hipGetLastError());
   }
   for (int s = s0; s < sim->nsp; ++s) {
+    if (sim->inner[s]) continue;  // launched with the species before it
     int first = (s == s0) ? ph0 : 0;
-    if (first <= 0) CHK(launch(sim, s, 0, j, tune));
+    const int b = sim->fuse[s];
+    if (first <= 0) {
+      CHK(launch(sim, s, 0, j, tune));
+      // a partner's sweep makes its QED list before the shared QED launch
+      if (b >= 0) CHK(launch(sim, b, 0, j, tune));
+    }
     if (first <= 1 && sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
     if (first <= 2 && sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
   }
@@ -3447,7 +3626,9 @@ static int sim_recover(Sim* sim, int s, bool* deferred) {
   const sfp_species_desc& d = sim->desc[s];
   const Counts& c = sim->counts_host[s];
   *deferred = c.defer_n > 0;
-  const int64_t extra = *deferred ? c.defer_n : c.n_hot;
+  // a shared QED launch asks for room for both species' lists
+  const int64_t extra = *deferred ? c.defer_n
+                                  : c.n_hot + (sim->fuse[s] >= 0 ? sim->counts_host[sim->fuse[s]].n_hot : 0);
   for (int dst : {d.photon_dst, d.pair_e_dst, d.pair_p_dst}) {
     if (dst < 0) continue;
     Store* st = sim->stores[dst];
@@ -3601,6 +3782,13 @@ extern "C" int sfp_sim_set_flag_clear_every(void* p, int64_t steps) {
     return set_err(SFP_E_ARG, "flag clear interval must be in [1, 2^28] steps");
   sim->clear_every = steps;
   return SFP_OK;
+}
+
+extern "C" int sfp_sim_set_shared_launches(void* p, int32_t on) {
+  Sim* sim = (Sim*)p;
+  if (!sim) return set_err(SFP_E_ARG, "null simulation");
+  sim->fuse_on = on != 0;
+  return sim_pick_kernels(sim);
 }
 
 extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, int32_t flags) {

@@ -256,6 +256,13 @@ class Simulation(object):
         self._prepare()
         L.call("sfp_sim_set_flag_clear_every", self._sim, int(steps))
 
+    def set_shared_launches(self, on: bool):
+        """Run two consecutive lepton species that share QED kernels and a
+        photon store (electrons, positrons) in one QED launch (the default),
+        or one launch per species; the results are the same."""
+        self._prepare()
+        L.call("sfp_sim_set_shared_launches", self._sim, int(bool(on)))
+
     def set_shape(self, species: int, phase: int, grid: int, rows: int = 0):
         """Pin the workgroup count and the most rows of 256 particles per tile
         of a kernel (rows 0: the kernel's most); grid 0 returns both to the

@@ -71,10 +71,10 @@ def _species_list(case, make):
 
 
 def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sampler=None,
-                 grids=None, reserve=None, **sim_kw):
+                 grids=None, reserve=None, shared=True, **sim_kw):
     """Run a case through sfparticles_amd.Simulation; returns per-species arrays."""
     sim, sp = build_gpu_case(case, precision=precision, sampler=sampler, grids=grids,
-                             reserve=reserve, **sim_kw)
+                             reserve=reserve, shared=shared, **sim_kw)
     steps = case["nsteps"] if nsteps is None else nsteps
     per = steps // chunks
     done = 0
@@ -85,8 +85,10 @@ def run_gpu_case(case, nsteps=None, chunks=1, timing=False, precision="fp64", sa
     return _collect(sp), sim
 
 
-def build_gpu_case(case, precision="fp64", sampler=None, grids=None, reserve=None, **sim_kw):
-    """The species and Simulation of a case, not yet run: (sim, [species])."""
+def build_gpu_case(case, precision="fp64", sampler=None, grids=None, reserve=None, shared=True,
+                   **sim_kw):
+    """The species and Simulation of a case, not yet run: (sim, [species]).
+    shared=False: one QED launch per lepton species."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
     from sfparticles_amd.fields import Fields
@@ -121,6 +123,8 @@ def build_gpu_case(case, precision="fp64", sampler=None, grids=None, reserve=Non
             sim.set_shape(si, ph, *shape)
         else:
             sim.set_grid(si, ph, shape)
+    if not shared:
+        sim.set_shared_launches(False)
     # capacities reserved up front (stores otherwise start at the seed count
     # and grow through capacity halts)
     for s, cap in zip(sp, reserve or ()):
