@@ -2849,10 +2849,12 @@ __device__ inline void catchup_general(const SpeciesDev& sp, int64_t i, uint32_t
 }
 
 // The photons k_catchup marked, with the general repeated addition, and the
-// last photon of an odd count: one wave per mask entry, lane l taking the
-// pair 64 j + l as k_catchup's lane l did (after a long run most photons are
-// marked, and one thread per entry walked 128 of them in turn: 59 ms for
-// 2.5e8 photons).
+// last photon of an odd count.  A wave reads 64 mask entries at once (lane l
+// entry jb + l), then takes the entries with a mark one after the other with
+// all its lanes, lane l the pair 64 j + l as k_catchup's lane l had it: after
+// a long run most photons are marked (one thread per entry walked 128 of them
+// in turn: 59 ms for 2.5e8 photons), after a short one almost none (a wave
+// per entry read the 1.2e6 entries of 1.5e8 photons one by one: 69 us).
 __global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ plan, int s,
                                                       uint32_t wend, double cdt_half) {
   const SpeciesDev& sp = plan->sp[s];
@@ -2862,12 +2864,20 @@ __global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ p
   const int lane = wave_lane();
   const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / 64;
-  for (int64_t j = w0; j < nm; j += nw) {
-    const uint4 mk = masks[j];
-    const uint64_t m0 = (uint64_t)mk.x | ((uint64_t)mk.y << 32);
-    const uint64_t m1 = (uint64_t)mk.z | ((uint64_t)mk.w << 32);
-    if ((m0 >> lane) & 1ull) catchup_general(sp, 2 * (64 * j + lane), wend, cdt_half);
-    if ((m1 >> lane) & 1ull) catchup_general(sp, 2 * (64 * j + lane) + 1, wend, cdt_half);
+  for (int64_t jb = 64 * w0; jb < nm; jb += 64 * nw) {
+    const uint4 mk = jb + lane < nm ? masks[jb + lane] : make_uint4(0u, 0u, 0u, 0u);
+    uint64_t todo = __ballot((mk.x | mk.y | mk.z | mk.w) != 0u);
+    while (todo) {
+      const int l = (int)__builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const uint64_t m0 = (uint64_t)(uint32_t)__shfl((int)mk.x, l) |
+                          ((uint64_t)(uint32_t)__shfl((int)mk.y, l) << 32);
+      const uint64_t m1 = (uint64_t)(uint32_t)__shfl((int)mk.z, l) |
+                          ((uint64_t)(uint32_t)__shfl((int)mk.w, l) << 32);
+      const int64_t j = jb + l;
+      if ((m0 >> lane) & 1ull) catchup_general(sp, 2 * (64 * j + lane), wend, cdt_half);
+      if ((m1 >> lane) & 1ull) catchup_general(sp, 2 * (64 * j + lane) + 1, wend, cdt_half);
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) catchup_general(sp, n - 1, wend, cdt_half);
 }
