@@ -468,8 +468,12 @@ int sfp_sim_stats(void* sim, int32_t species, int64_t* stats);
  *      bound listed at least 7/8 of the particles; results do not change)
  * [16] photons: steps on which conversions found no room in the pair
  *      stores and were made after the stores grew (deferred conversions;
- *      results do not change) */
-#define SFP_SIM_NSTATS 17
+ *      results do not change)
+ * [17] run calls that kept the photon visit lists and window caches of the
+ *      call before (no rebuild at their first step: they continue it, no
+ *      store changed from the host and no other simulation stepped the
+ *      stores in between; results do not change) */
+#define SFP_SIM_NSTATS 18
 int sfp_sim_stats_n(void* sim, int32_t species, int64_t* stats, int32_t n);
 /* Count and next id of one species as the last sfp_sim_run left them (read
  * by the run at its end: no device access; the same values

@@ -140,6 +140,9 @@ struct Store {
   // count, a writable pointer handed out): a simulation keeps photon sleep
   // state across run calls only while it stays the same
   uint64_t version = 0;
+  // run calls that stepped the store (any simulation): a simulation keeps
+  // its photon visit lists across its own consecutive run calls only
+  uint64_t runs = 0;
   void* p[SFP_NATTR] = {};
   Counts* cnt = nullptr;  // device
   // A simulation whose sleeping photons hold the positions of the steps they

@@ -173,8 +173,10 @@ struct StepArgs {
   uint64_t step;
   uint32_t rel, rel_end;  // step index within this run call, and the run's length
   // step on the photon sleep clock (steps since the simulation's wake origin):
-  // QED photons sleep on it, across run calls that continue each other
-  uint32_t wrel, pad;
+  // QED photons sleep on it, across run calls that continue each other;
+  // fresh: the first step of a run call whose photon visit lists are rebuilt
+  // (the lists and window caches of the last call are not kept)
+  uint32_t wrel, fresh;
 };
 
 // --------------------------------------------------------- look-back scan --
@@ -588,9 +590,10 @@ __device__ inline uint32_t next_epoch(const ScanState* scan) {
 // --------------------------------------------------- multi-step photon push --
 // The photon visit list is rebuilt (and photons fall asleep and wake) on the
 // steps of the sleep clock that are multiples of WIN, and on the first step of
-// every run call.
+// a run call that does not continue the last one (or follows a change of the
+// stores): a call that does keeps the last call's lists and window caches.
 __device__ inline bool list_rebuild(const StepArgs& a) {
-  return (a.wrel % WIN) == 0 || a.rel == 0;
+  return (a.wrel % WIN) == 0 || a.fresh != 0;
 }
 
 // A QED photon asleep until this step holds the position of the step it fell
@@ -2249,7 +2252,7 @@ __global__ void __launch_bounds__(TILE) k_awake(const Plan* __restrict__ plan, i
       const int j = __builtin_ctz(m);
       const int64_t i = i0 + j;
       sh_list[lq] = (uint32_t)i;
-      sh_cont[lq++] = (uint8_t)(a.rel != 0 && ((cont >> j) & 1u) && i < prev_end);
+      sh_cont[lq++] = (uint8_t)(a.fresh == 0 && ((cont >> j) & 1u) && i < prev_end);
     }
     __syncthreads();
     // ... and are written out one per thread (coalesced).  A photon awake
@@ -2796,15 +2799,18 @@ __device__ inline int catchup_binade(uint32_t w, uint32_t& s0, uint8_t pr, doubl
 // the general loop inlined as the fallback, the few lanes that took it held
 // whole waves in it and the kernel ran at half the speed; a list with one
 // atomic per wave on a shared counter was slower still.)  The masks live in
-// the visit list, which is unused between run calls and holds 8 bytes per
-// photon of capacity against the masks' 1/8.  All arrays are allocated for
-// the store's capacity, and a full pair lies below the count.
+// the visit list's old-position array (oldk: written and read only on the
+// steps that rebuild the list, so free between run calls, while the list
+// itself is kept for a call that continues this one), whose 4 bytes per
+// photon of capacity (in whole multiples of 64 entries, so 16-byte aligned)
+// hold the masks' 1/8 byte.  All arrays are allocated for the store's
+// capacity, and a full pair lies below the count.
 __global__ void __launch_bounds__(256) k_catchup(const Plan* __restrict__ plan, int s,
                                                  uint32_t wend, double cdt_half) {
   const SpeciesDev& sp = plan->sp[s];
   const int64_t n = sp.cnt->n;
   const int64_t npair = (n + 1) / 2;
-  uint4* masks = (uint4*)sp.hot;
+  uint4* masks = (uint4*)sp.oldk;
   // g of a wave's first lane is a multiple of 64 (workgroups of 256, a
   // stride of whole workgroups): the wave covers pairs g .. g + 63
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < npair;
@@ -2860,7 +2866,7 @@ __global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ p
   const SpeciesDev& sp = plan->sp[s];
   const int64_t n = sp.cnt->n;
   const int64_t nm = ((n + 1) / 2 + 63) / 64;
-  const uint4* masks = (const uint4*)sp.hot;
+  const uint4* masks = (const uint4*)sp.oldk;
   const int lane = wave_lane();
   const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int64_t nw = (int64_t)gridDim.x * blockDim.x / 64;
@@ -3028,6 +3034,11 @@ struct Sim {
   uint64_t wake_origin = 0, wake_end = 0;
   bool wake_valid = false;
   int64_t clock_restarts = 0;
+  // the photon visit lists and window caches of the last run call hold for a
+  // call that continues it (list_kept, with the stores' run counts at its end)
+  bool list_kept = false;
+  std::vector<uint64_t> list_runs;
+  int64_t list_keeps = 0;
   // sleeping photons hold the positions of their sleep start until settled
   // (by the next run that continues this one, or before any host access)
   bool lag = false;
@@ -3117,9 +3128,10 @@ static int sim_refresh_plan(Sim* sim) {
     if (sim->qed[s] && st->cap > sim->hot_cap[s]) {
       // photons keep their visit list from step to step: grown inside a run,
       // the list keeps its entries; the photons' old cache positions (oldk)
-      // follow the list in the same block; at least 64 entries, so that the
-      // catch-up's masks (16 bytes per 128 photons) fit in a small store's
-      int64_t want = std::max<int64_t>(st->cap, 64);
+      // follow the list in the same block; whole multiples of 64 entries, so
+      // that the catch-up's masks (16 bytes per 128 photons, in oldk) are
+      // 16-byte aligned and fit a small store's
+      int64_t want = (std::max<int64_t>(st->cap, 64) + 63) & ~(int64_t)63;
       uint32_t* h = nullptr;
       HIPCHK(hipMalloc(&h, sizeof(uint32_t) * 2 * want));
       if (sim->hot_mem[s]) {
@@ -3389,6 +3401,7 @@ extern "C" int sfp_sim_create(const sfp_species_desc* species, int32_t nspecies,
   sim->wake_mem.assign(nspecies, nullptr);
   sim->since_mem.assign(nspecies, nullptr);
   sim->wake_version.assign(nspecies, 0);
+  sim->list_runs.assign(nspecies, 0);
   sim->wake_cap.assign(nspecies, 0);
   sim->cache_mem.assign(nspecies, nullptr);
   sim->cache_cap.assign(nspecies, 0);
@@ -3543,7 +3556,7 @@ static hipEvent_t sim_event(Sim* sim) {
 // Launch the kernel of (species s, phase ph) for the step in argument slot j.
 // tune >= 0: a tuning launch of the batch's step `tune`, run with candidate
 // shape tune mod candidates and timed for the tuner -- except on the photon
-// list's rebuild steps (every WIN-th step and a run's first), which cost
+// list's rebuild steps (every WIN-th step and a fresh run's first), which cost
 // several times a step in between: the few of them in a batch would fall to
 // some candidates only, so the tuner compares the steps in between.
 static int launch(Sim* sim, int s, int ph, int j, int tune) {
@@ -3551,7 +3564,7 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
   const std::vector<Shape>& cand = sim->cand[s][ph];
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
   const StepArgs& ah = sim->args_host[j];
-  if (tune >= 0 && (ah.wrel % WIN == 0 || ah.rel == 0)) tune = -2;  // run, not timed
+  if (tune >= 0 && (ah.wrel % WIN == 0 || ah.fresh)) tune = -2;  // run, not timed
   const bool timed = sim->timing && (ah.step % (uint64_t)sim->timing_every == 0 ||
                                      (s == sim->time_s && ph == sim->time_ph));
   const bool ev = timed || tune >= 0;
@@ -3819,6 +3832,26 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   for (int s = 0; s < sim->nsp; ++s)
     if (sim->stores[s]->version != sim->wake_version[s]) keep = false;
   sim->wake_valid = false;
+  // the visit lists go on as well when no other simulation stepped the
+  // stores since, every QED photon species moves (photons that do not sleep
+  // to the end of each call) and no window cache is reallocated below
+  bool keep_list = keep && sim->list_kept;
+  for (int s = 0; s < sim->nsp; ++s) {
+    if (sim->stores[s]->runs != sim->list_runs[s]) keep_list = false;
+    if (sim->plan.sp[s].photon && sim->qed[s] && !sim->plan.sp[s].push) keep_list = false;
+  }
+  // ... and only where a rebuild at the call's first step could not pay for
+  // itself.  A rebuild drops from the lists the photons appended since the
+  // last one that can sleep; that pays when half a window of steps or more
+  // follows it before the next rebuild of the clock (at the collision centre
+  // of the 1e7-electron benchmark a rebuild at a call's first step cost
+  // 0.62 ms and made the 14 steps after it 1.05 ms faster), so such a call
+  // rebuilds, while shorter ones (a step or a few per host callback) keep
+  // the lists.  Results are the same either way.
+  if (keep_list && nsteps >= (int64_t)(WIN / 2) &&
+      WIN - (step0 - sim->wake_origin) % WIN >= WIN / 2)
+    keep_list = false;
+  sim->list_kept = false;
   if (!keep) CHK(sim_settle(sim));  // the restarted clock needs current positions
   // From here the run tracks every photon itself: this simulation's store
   // callbacks (installed at the end of the last run) come off, so that a
@@ -3860,6 +3893,7 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
         HIPCHK(hipMalloc(&sim->cache_mem[s], (size_t)cc * (size_t)per));
         sim->cache_cap[s] = cc;
         dirty = true;
+        keep_list = false;
       }
     }
   }
@@ -3896,7 +3930,7 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
       a.rel = (uint32_t)(k + j);
       a.rel_end = (uint32_t)nsteps;
       a.wrel = (uint32_t)(a.step - sim->wake_origin);
-      a.pad = 0;
+      a.fresh = (k + j == 0 && !keep_list) ? 1u : 0u;
       tt = tt + dt;
     }
     const bool tuning = autotune && sim_needs_tune(sim);
@@ -3968,6 +4002,11 @@ extern "C" int sfp_sim_run(void* p, double* t, uint64_t step0, int64_t nsteps, i
   sim->wake_end = step0 + (uint64_t)nsteps;
   for (int s = 0; s < sim->nsp; ++s) sim->wake_version[s] = sim->stores[s]->version;
   sim->wake_valid = true;
+  if (nsteps > 0) {
+    if (keep_list) sim->list_keeps++;
+    for (int s = 0; s < sim->nsp; ++s) sim->list_runs[s] = ++sim->stores[s]->runs;
+  }
+  sim->list_kept = nsteps > 0 || keep_list;
   return SFP_OK;
 }
 
@@ -4142,6 +4181,7 @@ extern "C" int sfp_sim_stats_n(void* p, int32_t species, int64_t* stats, int32_t
   CHK(sfp_sim_stats(p, species, v));
   v[15] = sim->counts_host[species].dense_steps;
   v[16] = sim->counts_host[species].deferrals;
+  v[17] = sim->list_keeps;
   for (int32_t k = 0; k < n && k < SFP_SIM_NSTATS; ++k) stats[k] = v[k];
   return SFP_OK;
 }
@@ -4177,5 +4217,6 @@ extern "C" int sfp_sim_reset_stats(void* p) {
   sim->replays = 0;
   sim->graph_batches = sim->captures = sim->tune_batches = 0;
   sim->clock_restarts = 0;
+  sim->list_keeps = 0;
   return SFP_OK;
 }

@@ -285,15 +285,15 @@ class Simulation(object):
         particles they appended, current buffered count, QED-listed
         particles, and the run's host synchronisations, capacity replays,
         photon sleep counts, graph and tuner batches, dense steps, deferred
-        conversions."""
-        v = (C.c_int64 * 17)()
-        L.call("sfp_sim_stats_n", self._sim, int(species), v, 17)
+        conversions, run calls that kept the photon visit lists."""
+        v = (C.c_int64 * 18)()
+        L.call("sfp_sim_stats_n", self._sim, int(species), v, 18)
         return dict(psteps=int(v[0]), processed=int(v[1]), spawned=int(v[2]), n=int(v[3]),
                     hot=int(v[4]), syncs=int(v[5]), replays=int(v[6]), capacity=int(v[7]),
                     slept=int(v[8]), sleeps=int(v[9]), graph_batches=int(v[10]),
                     captures=int(v[11]), tune_batches=int(v[12]), emit=int(v[13]),
                     clock_restarts=int(v[14]), dense_steps=int(v[15]),
-                    deferrals=int(v[16]))
+                    deferrals=int(v[16]), list_keeps=int(v[17]))
 
     def qed_diag(self, species: int) -> dict:
         """Diagnostics of the exact QED evaluations of a species since the last

@@ -336,9 +336,10 @@ def test_envelope_bounds_are_upper_bounds(which):
 def test_photon_sleep_carries_across_run_calls(qed, graph):
     """Run calls that continue each other keep QED photons asleep across the
     call boundary (no restart of the sleep clock, no visit of every photon
-    at the first step of a call); positions read between calls are those of
-    that step, and the end state still equals the oracle's bit for bit (the
-    optical depth of sleeping photons included; also with graph replays)."""
+    at the first step of a call) and keep the photon visit lists and window
+    caches; positions read between calls are those of that step, and the end
+    state still equals the oracle's bit for bit (the optical depth of
+    sleeping photons included; also with graph replays)."""
     import sfparticles_amd as sfa
     from sfparticles_amd import qed as Q
     from sfparticles_amd.fields import Fields
@@ -357,6 +358,10 @@ def test_photon_sleep_carries_across_run_calls(qed, graph):
         sim = sfa.Simulation(e, p, g, dt=case["dt"], fields=Fields(terms=case["field"]),
                              print_every=None, photon_threshold=case["threshold"],
                              seed=case["seed"], graph=graph)
+        # the photon store reserved up front: a window cache sized once (one
+        # grown at the start of a call rebuilds the lists there)
+        g._ensure_device()
+        g._extend(1 << 20)
         return sim, (e, p, g)
 
     # calls of uneven length, none a multiple of the 16-step list window; the
@@ -371,10 +376,18 @@ def test_photon_sleep_carries_across_run_calls(qed, graph):
     for m in (37, 64, 83) if not graph else (37, 128, 19):  # graph replays need 64-step batches
         sim.run_steps(m)
     mid = photons(sp[2])
-    sim.start(56)
+    # calls of one step, and one that ends the run 5 steps before the clock's
+    # next rebuild, keep the visit lists and window caches of the call before
+    # (the first of them across the host read above, whose catch-up leaves
+    # them be); the longer calls before rebuild theirs at their first step
+    for _ in range(3):
+        sim.run_steps(1)
+    sim.start(53)
     st = sim.stats(2)
-    print("clock restarts", st["clock_restarts"], "slept", st["slept"], "sleeps", st["sleeps"])
+    print("clock restarts", st["clock_restarts"], "slept", st["slept"], "sleeps", st["sleeps"],
+          "visit lists kept", st["list_keeps"])
     assert st["clock_restarts"] == 1 and st["slept"] > 0
+    assert st["list_keeps"] == 4
     # positions between calls: a single call stopped at the same step, read
     # after its simulation is gone (which brings sleeping photons up to date)
     sim1, sp1 = make()
@@ -393,5 +406,6 @@ def test_photon_sleep_carries_across_run_calls(qed, graph):
         assert sim.stats(0)["graph_batches"] >= 2
     compare_final(got, run_oracle_case(case))
     # a change of a store from the host (here the prune) restarts the clock
+    # and rebuilds the lists
     sim.run_steps(8)
-    assert sim.stats(2)["clock_restarts"] == 2
+    assert sim.stats(2)["clock_restarts"] == 2 and sim.stats(2)["list_keeps"] == 4
