@@ -355,15 +355,35 @@ def _csv_rows(d, suffix):
     return rows
 
 
+_LEPTON = ("k_sweep<false", "k_hot", "k_emit", "k_step_prologue")
+
+
+def _step_opens(seq):
+    """Indices of the dispatches that open a step: k_step_prologue, or, where
+    the first species' sweep takes the step's count snapshot itself (the
+    electrons of this benchmark), a lepton sweep that follows no lepton
+    kernel (the positrons' sweep follows the electrons' sweep, or their QED
+    kernels when each species has its own)."""
+    out = []
+    for i, r in enumerate(seq):
+        name = r[0]
+        if name.startswith("k_step_prologue"):
+            out.append(i)
+        elif name.startswith("k_sweep<false") and not (i and seq[i - 1][0].startswith(_LEPTON)):
+            out.append(i)
+    return out
+
+
 def _by_species(seq):
     """Names of the per-species kernels tagged with their species: a step
     launches the lepton kernels for electrons, then for positrons, then the
-    photon kernels, each step opening with k_step_prologue.  With shared QED
-    launches a step sweeps both lepton species and then runs one launch of
-    the QED (and emission) kernel for both: tagged [e-,e+]."""
+    photon kernels (_step_opens).  With shared QED launches a step sweeps both
+    lepton species and then runs one launch of the QED (and emission) kernel
+    for both: tagged [e-,e+]."""
     steps, cur = [], []
-    for r in seq:
-        if r[0].startswith("k_step_prologue") and cur:
+    opens = set(_step_opens(seq))
+    for i, r in enumerate(seq):
+        if i in opens and cur:
             steps.append(cur)
             cur = []
         cur.append(r)
@@ -393,9 +413,9 @@ def _by_species(seq):
 def _windows(seq, steps):
     """Timed windows of a dispatch sequence [(name, ...)]: the dispatches after
     one settle (k_catchup and the k_catchup_slow after it) up to and
-    including the next, when exactly `steps` step prologues lie between them
-    (the settle before the timed steps, the timed steps, the settle that
-    ends them)."""
+    including the next, when exactly `steps` steps open between them (the
+    settle before the timed steps, the timed steps, the settle that ends
+    them)."""
     cu = [i for i, r in enumerate(seq) if r[0] == "k_catchup"]
 
     def end(i):  # the settle's last dispatch
@@ -403,7 +423,7 @@ def _windows(seq, steps):
     out = []
     for a, b in zip(cu, cu[1:]):
         seg = seq[end(a) + 1:end(b) + 1]
-        if sum(1 for r in seg if r[0].startswith("k_step_prologue")) == steps:
+        if len(_step_opens(seg)) == steps:
             out.append(seg)
     return out
 
@@ -443,8 +463,8 @@ def summarize_profile(d):
         t = collections.defaultdict(list)
         for name, a, b in seg:
             t[name].append((b - a) / 1e3)
-        # from the first step's prologue (host work before it is not step time)
-        first = next(a for name, a, b in seg if name.startswith("k_step_prologue"))
+        # from the first step's start (host work before it is not step time)
+        first = seg[_step_opens(seg)[0]][1]
         trace[path] = (t, (seg[-1][2] - first) / 1e3)
 
     def counters(sub, names):

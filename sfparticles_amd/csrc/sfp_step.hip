@@ -700,19 +700,30 @@ __device__ inline uint32_t sleep_steps_drawn(const Plan& P, const sfp_fieldset* 
 // (104 and 203 VGPRs), and 6 or 4 waves ran slower.
 static constexpr int ROWS = 32;
 
+// sel: the species, and SEL_PROLOGUE when this launch opens the step: its
+// first workgroup then takes the step's snapshot of the counts (what
+// k_step_prologue does otherwise, a launch and a gap less per step), and its
+// workgroups read their species' count itself, which nothing has changed yet
+// in this step.
+static constexpr int SEL_PROLOGUE = 0x100;
+__device__ inline void step_snapshot(const Plan& P, const StepArgs& a);
+
 template <bool PHOTON, bool QED, int FM, bool MIX, bool SH>
-__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int s, const StepArgs* __restrict__ ap,
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int sel, const StepArgs* __restrict__ ap,
         int rmax) {
   const StepArgs a = *ap;
   const Plan& P = *plan;
   if (P.halt->flag) return;
+  const int s = sel & 0xff;
+  const bool opens = (sel & SEL_PROLOGUE) != 0;
   const SpeciesDev& sp = P.sp[s];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
   __shared__ uint64_t sh_mask[ROWS][WAVES];
   __shared__ uint64_t sh_pref;
 
-  const int64_t n_old = sp.cnt->n_old;
+  const int64_t n_old = opens ? sp.cnt->n : sp.cnt->n_old;
+  if (opens && blockIdx.x == 0) step_snapshot(P, a);
   if (!PHOTON && QED) {
     // Dense step: when the last step's bound listed at least 7/8 of the
     // particles, this kernel stands down and the QED kernel visits every
@@ -2697,15 +2708,18 @@ __global__ void __launch_bounds__(TILE) k_defer(const Plan* __restrict__ plan, i
 // prologue of a step: snapshot the counts the species kernels iterate over
 // (and, on the steps where photons fall asleep and wake, restart the count of
 // sleeping photons that the list kernel rebuilds)
-__global__ void k_step_prologue(const Plan* plan, const StepArgs* __restrict__ ap) {
-  const StepArgs a = *ap;
-  if (plan->halt->flag) return;
-  int s = threadIdx.x;
-  if (s < plan->nspecies) {
-    Counts* c = plan->sp[s].cnt;
+__device__ inline void step_snapshot(const Plan& P, const StepArgs& a) {
+  const int s = threadIdx.x;
+  if (s < P.nspecies) {
+    Counts* c = P.sp[s].cnt;
     c->n_old = c->n;
     if (list_rebuild(a)) c->asleep_n = 0;
   }
+}
+__global__ void k_step_prologue(const Plan* plan, const StepArgs* __restrict__ ap) {
+  const StepArgs a = *ap;
+  if (plan->halt->flag) return;
+  step_snapshot(*plan, a);
 }
 
 // end of a step: the recorder's row when this step is one it records
@@ -3559,7 +3573,7 @@ static hipEvent_t sim_event(Sim* sim) {
 // list's rebuild steps (every WIN-th step and a fresh run's first), which cost
 // several times a step in between: the few of them in a batch would fall to
 // some candidates only, so the tuner compares the steps in between.
-static int launch(Sim* sim, int s, int ph, int j, int tune) {
+static int launch(Sim* sim, int s, int ph, int j, int tune, bool opens = false) {
   kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
   const std::vector<Shape>& cand = sim->cand[s][ph];
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
@@ -3574,8 +3588,10 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
     e1 = sim_event(sim);
     if (e0) HIPCHK(hipEventRecord(e0, sim->stream));
   }
-  // the QED kernels take a second species in bits 4-7 (plus one)
-  const int sel = (ph > 0 && sim->fuse[s] >= 0) ? (s | ((sim->fuse[s] + 1) << 4)) : s;
+  // the QED kernels take a second species in bits 4-7 (plus one); a sweep
+  // that opens the step, SEL_PROLOGUE
+  const int sel = (ph > 0 && sim->fuse[s] >= 0) ? (s | ((sim->fuse[s] + 1) << 4))
+                                                : (s | (opens ? SEL_PROLOGUE : 0));
   hipLaunchKernelGGL(fn, dim3(cand[c].grid), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev,
                      sel, (const StepArgs*)(sim->args_dev + j), cand[c].rows);
   HIPCHK(hipGetLastError());
@@ -3588,7 +3604,12 @@ static int launch(Sim* sim, int s, int ph, int j, int tune) {
 
 // Launch the kernels of the step in slot j from (species s0, phase ph0) on.
 static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
-  if (s0 == 0 && ph0 == 0) {
+  // the step's count snapshot: in the first sweep, unless the first species
+  // is a QED photon species (its list kernel adds to a count the snapshot
+  // resets)
+  const bool opens = s0 == 0 && ph0 == 0;
+  const bool merged = opens && !(sim->plan.sp[0].photon && sim->qed[0]);
+  if (opens && !merged) {
     hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, sim->stream,
                        (const Plan*)sim->plan_dev, (const StepArgs*)(sim->args_dev + j));
     HIPCHK(hipGetLastError());
@@ -3598,7 +3619,7 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
     int first = (s == s0) ? ph0 : 0;
     const int b = sim->fuse[s];
     if (first <= 0) {
-      CHK(launch(sim, s, 0, j, tune));
+      CHK(launch(sim, s, 0, j, tune, merged && s == 0));
       // a partner's sweep makes its QED list before the shared QED launch
       if (b >= 0) CHK(launch(sim, b, 0, j, tune));
     }
