@@ -163,21 +163,34 @@ SFP_HD int sfp_mlp_perm(int kk) {
  * the permuted weights, exactly the operand layout of the next.  W2p and W3p
  * may point to LDS or global memory.  out[q] receives the normalised output
  * of sample c of group q in its four lanes. */
-/* The weight pointer passed through an empty asm statement: the compiler
- * can no longer treat the weight loads as loop-invariant, so a caller's loop
- * does not hoist all 32 operand loads into registers (no memory access is
- * made by the statement). */
-__device__ inline const __bf16* sfp_mlp_opaque(const __bf16* p) {
-  asm volatile("" : "+v"(p));
-  return p;
+/* Where the weights are: SFP_W_LDS (staged by the caller) or SFP_W_GLOBAL;
+ * the loads then use that address space (ds_read / global_load), not
+ * generic flat loads, which wait on both the memory and the LDS counters. */
+#define SFP_W_GLOBAL 1
+#define SFP_W_LDS 3
+#ifdef __HIP_DEVICE_COMPILE__
+#define SFP_WAS(n) __attribute__((address_space(n)))
+#else
+#define SFP_WAS(n)
+#endif
+
+/* The weight pointer, in its address space, passed through an empty asm
+ * statement: the compiler can no longer treat the weight loads as
+ * loop-invariant, so a caller's loop does not hoist all 32 operand loads into
+ * registers (no memory access is made by the statement). */
+template <int WAS>
+__device__ inline SFP_WAS(WAS) const __bf16* sfp_mlp_opaque(const __bf16* p) {
+  SFP_WAS(WAS) const __bf16* q = (SFP_WAS(WAS) const __bf16*)p;
+  asm volatile("" : "+v"(q));
+  return q;
 }
 
-template <int NG, int NIN = 2>
-__device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_mlp_dev* M, const __bf16* W2p, const __bf16* W3p,
+template <int NG, int NIN = 2, int WAS = SFP_W_GLOBAL>
+__device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_mlp_dev* M, const __bf16* W2g, const __bf16* W3g,
                                      const float* x0, const float* x1, float* out,
                                      const float* x2 = nullptr) {
-  W2p = sfp_mlp_opaque(W2p);
-  W3p = sfp_mlp_opaque(W3p);
+  SFP_WAS(WAS) const __bf16* W2p = sfp_mlp_opaque<WAS>(W2g);
+  SFP_WAS(WAS) const __bf16* W3p = sfp_mlp_opaque<WAS>(W3g);
   const int l = __lane_id();
   const int c = l & 15, g = l >> 4;
   float h[NG][16];
@@ -196,7 +209,7 @@ __device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_ml
     }
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
-    const __bf16* Wp = layer == 0 ? W2p : W3p;
+    SFP_WAS(WAS) const __bf16* Wp = layer == 0 ? W2p : W3p;
     const float* b = layer == 0 ? M->b2 : M->b3;
     sfp_bf16x8 B[NG][2];
 #pragma unroll
@@ -210,8 +223,9 @@ __device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_ml
         }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const sfp_bf16x8 A0 = *(const sfp_bf16x8*)(Wp + (16 * t + c) * 64 + 8 * g);
-      const sfp_bf16x8 A1 = *(const sfp_bf16x8*)(Wp + (16 * t + c) * 64 + 32 + 8 * g);
+      const sfp_bf16x8 A0 = *(SFP_WAS(WAS) const sfp_bf16x8*)(Wp + (16 * t + c) * 64 + 8 * g);
+      const sfp_bf16x8 A1 =
+          *(SFP_WAS(WAS) const sfp_bf16x8*)(Wp + (16 * t + c) * 64 + 32 + 8 * g);
       float bb[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) bb[e] = b[16 * t + 4 * g + e];
@@ -239,10 +253,11 @@ __device__ inline __attribute__((always_inline)) void sfp_mlp_evalN(const sfp_ml
 }
 
 /* NG = 1: one group of 16 samples */
+template <int WAS = SFP_W_GLOBAL>
 __device__ inline float sfp_mlp_eval16(const sfp_mlp_dev* M, const __bf16* W2p,
                                        const __bf16* W3p, float x0, float x1) {
   float o;
-  sfp_mlp_evalN<1>(M, W2p, W3p, &x0, &x1, &o);
+  sfp_mlp_evalN<1, 2, WAS>(M, W2p, W3p, &x0, &x1, &o);
   return o;
 }
 

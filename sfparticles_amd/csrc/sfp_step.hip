@@ -159,10 +159,10 @@ struct Plan {
   SFP_G Halt* halt;
   const SFP_G sfp_fieldset* shot_fields;  // ensemble: one field set per shot, or null
   uint32_t nshots;
-  const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
-  const sfp_mlp_bins* net_bins;  // its per-bin table fallback
-  const sfp_mlp_dev* rate_net;   // learned spin-resolved emission rate, or null
-  const sfp_mlp_bins* rate_bins;
+  SFP_G const sfp_mlp_dev* net;        // learned photon-energy sampler, or null
+  SFP_G const sfp_mlp_bins* net_bins;  // its per-bin table fallback
+  SFP_G const sfp_mlp_dev* rate_net;   // learned spin-resolved emission rate, or null
+  SFP_G const sfp_mlp_bins* rate_bins;
   Recorder rec;
   int32_t nspecies;
   int32_t qed;
@@ -249,6 +249,11 @@ __device__ inline int64_t wg_uniform(int64_t v) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
+
+// A table pointer of the step plan (generic in the table structs the CPU
+// restatement shares) in the global address space: loads through it are
+// global loads, which unlike generic (flat) ones leave the LDS counter alone.
+__device__ inline SFP_G const double* gtab(const double* p) { return (SFP_G const double*)p; }
 
 // The total of a look-back chain of ntiles tiles: its last tile's inclusive
 // prefix, waited for until that tile has published it in this epoch.  Safe
@@ -1104,7 +1109,7 @@ __device__ inline bool qed_event(const Plan& P, const sfp_fieldset* FS, const Sp
     if (chi >= P.od.chi_min) {
       if (dg) dg->od(chi > P.od.chi_max);
       event = od_update(&tau, ig, chi, a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, 0u, lds_od,
-                        PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
+                        PHOTON ? gtab(P.od.pair_cdf) : gtab(P.od.photon_cdf), P.od, &delta);
     }
   }
   if (!PHOTON && event) {
@@ -1199,7 +1204,7 @@ __device__ inline bool qed_event32(const Plan& P, const sfp_fieldset* FS, const 
     if (chi >= P.od.chi_min) {
       if (dg) dg->od(chi > P.od.chi_max);
       event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag,
-                          lds_od, PHOTON ? P.od.pair_cdf : P.od.photon_cdf, P.od, &delta);
+                          lds_od, PHOTON ? gtab(P.od.pair_cdf) : gtab(P.od.photon_cdf), P.od, &delta);
     }
   }
   if (!PHOTON && event) {
@@ -1278,14 +1283,14 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
     if (!(chi >= P.od.chi_min)) return false;
     if (dg) dg->od(chi > P.od.chi_max);
     event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, lds_od,
-                        P.od.photon_cdf, P.od, &delta);
+                        gtab(P.od.photon_cdf), P.od, &delta);
     if (!event) return false;
     sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 2u);
     // the optical-depth inverse CDF maps its uniform onto [ymin, ymax] of the
     // cumulative rate; on the network's from-zero scale that is
     // (ymin + u (ymax - ymin)) / ymax
     double ymin, ymax;
-    sfp_od_row_ends(P.od.photon_cdf, &P.od, chi, &ymin, &ymax);
+    sfp_od_row_ends(gtab(P.od.photon_cdf), &P.od, chi, &ymin, &ymax);
     const double ud = sfp_u01(d.v[2], d.v[3]);
     u = (float)((ymin + ud * (ymax - ymin)) / ymax);
     od_u = ud;
@@ -1300,9 +1305,9 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
     in0 = sfp_mlp_x0(P.net, lc);
     in1 = 2.0f * u - 1.0f;
   } else if (KIND == KIND_RS) {
-    delta = sfp_od_sample_from0(P.od.photon_cdf, &P.od, chi, (double)u);
+    delta = sfp_od_sample_from0(gtab(P.od.photon_cdf), &P.od, chi, (double)u);
   } else {
-    delta = sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, chi, od_u);
+    delta = sfp_od_inverse_cdf(gtab(P.od.photon_cdf), &P.od, chi, od_u);
   }
   return event;
 }
@@ -1394,7 +1399,7 @@ __device__ inline void net_delta_wave(const Plan& P, const __bf16* W2p, const __
     for (int j = 0; j < cc; ++j) mm &= mm - 1ull;
     const int src = (int)__builtin_ctzll(mm);
     const float a0 = __shfl(in0, src), a1 = __shfl(in1, src);
-    const float o = sfp_mlp_eval16(M, W2p, W3p, a0, a1);
+    const float o = sfp_mlp_eval16<SFP_W_LDS>(M, W2p, W3p, a0, a1);  // weights staged in LDS
     const float res = __shfl(o, slot >= 0 && slot < 16 ? slot : 0);
     if (net && slot >= 0 && slot < 16) delta = sfp_mlp_delta(M, res);
     for (int j = 0; j < cnt; ++j) m &= m - 1ull;
@@ -2077,8 +2082,8 @@ __global__ void __launch_bounds__(TILE) k_emit(const Plan* __restrict__ plan, in
     const bool net = valid && r.net;
     net_delta_wave(P, (const __bf16*)lds_w, (const __bf16*)lds_w + 4096, net, r.in0, r.in1, delta);
     if (net && delta > SFP_MLP_DELTA_CUT)  // the hard tail from the table, same uniform
-      delta = (KIND == KIND_RS) ? sfp_od_sample_from0(P.od.photon_cdf, &P.od, r.chi, r.u_tab)
-                                : sfp_od_inverse_cdf(P.od.photon_cdf, &P.od, r.chi, r.u_tab);
+      delta = (KIND == KIND_RS) ? sfp_od_sample_from0(gtab(P.od.photon_cdf), &P.od, r.chi, r.u_tab)
+                                : sfp_od_inverse_cdf(gtab(P.od.photon_cdf), &P.od, r.chi, r.u_tab);
     // hard-photon cut (cpu.py:227-232)
     bool event = valid && !(sfp32_div((float)delta, igf) < (float)P.threshold);
     if (!dA) event = false;
@@ -3444,8 +3449,8 @@ extern "C" int sfp_sim_set_sampler(void* p, void* mlp) {
     if (nin != 2) return set_err(SFP_E_ARG, "not a photon-energy sampler");
     const void *dev = nullptr, *bins = nullptr;
     CHK(sfp_mlp_device(mlp, &dev, &bins));
-    P.net = (const sfp_mlp_dev*)dev;
-    P.net_bins = (const sfp_mlp_bins*)bins;
+    P.net = gptr((const sfp_mlp_dev*)dev);
+    P.net_bins = gptr((const sfp_mlp_bins*)bins);
   } else {
     P.net = nullptr;
     P.net_bins = nullptr;
@@ -3471,8 +3476,8 @@ extern "C" int sfp_sim_set_rate_net(void* p, void* mlp) {
     if (nin != 3) return set_err(SFP_E_ARG, "not an emission-rate network");
     const void *dev = nullptr, *bins = nullptr;
     CHK(sfp_mlp_device(mlp, &dev, &bins));
-    P.rate_net = (const sfp_mlp_dev*)dev;
-    P.rate_bins = (const sfp_mlp_bins*)bins;
+    P.rate_net = gptr((const sfp_mlp_dev*)dev);
+    P.rate_bins = gptr((const sfp_mlp_bins*)bins);
   } else {
     P.rate_net = nullptr;
     P.rate_bins = nullptr;
