@@ -1232,83 +1232,93 @@ __device__ inline bool qed_event32_net(const Plan& P, const sfp_fieldset* FS,
                                        const SpeciesDev& sp, const StepArgs& a,
                                        const sfp_rs_node32* lds_rs, const double2* lds_od,
                                        double x, double y, double z, float ux, float uy, float uz,
-                                       float ig, uint64_t pid, double& tau, double& delta,
-                                       bool& net, float& in0, float& in1, double& chi_out,
-                                       double& u_table, QDiag* dg = nullptr) {
-  bool event = false;
-  delta = 0.0;
-  net = false;
+                                       float ig, uint64_t pid, double& tau, double& delta_out,
+                                       bool& net_out, float& in0_out, float& in1_out,
+                                       double& chi_out, double& u_table_out, QDiag* dg = nullptr) {
+  // The outputs are written once, at the one exit: stores to two of them at
+  // the ends of two branches were merged into one store to a selected
+  // address, which kept both of the caller's variables on the stack.
+  bool event = false, net = false;
+  double delta = 0.0, u_table = 0.0;
+  float in0 = 0.0f, in1 = 0.0f;
   float F[6];
   sfp32_field_eval<FM>(FS, x, y, z, a.t, F);
   const double chi = sfp32_chi(F, ux, uy, uz);
-  chi_out = chi;
   if (dg) dg->nan_chi(chi);
   const sfp_mlp_bins& nb = *P.net_bins;
-  float u = 0.0f;     // uniform that selects delta, on the network's from-zero scale
-  double od_u = 0.0;  // optical depth: the uniform of its own inverse CDF
-  if (KIND == KIND_RS) {
-    if (chi == 0.0) return false;
-    sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
-    const float r1 = sfp32_u01(d.v[0]), r2 = sfp32_u01(d.v[2]);
-    const float dtau = (float)a.dt * ig;
-    if (!(chi >= P.od.chi_min)) {
-      const float r1_3 = r1 * r1 * r1;
-      const float p = 3.0f * (r1 * r1) *
-                      sfp32_rs_rate(lds_rs, (int)P.z_N, (float)P.z_min, (float)P.z_max,
-                                    (float)P.z_delta, (float)(1.0 / P.z_delta), (float)chi, r1_3,
-                                    true) *
-                      dtau;
-      if (dg) {
-        dg->prob(p);
-        dg->zcheck(P, chi, r1_3);
+  do {
+    float u = 0.0f;     // uniform that selects delta, on the network's from-zero scale
+    double od_u = 0.0;  // optical depth: the uniform of its own inverse CDF
+    if (KIND == KIND_RS) {
+      if (chi == 0.0) break;
+      sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 0u);
+      const float r1 = sfp32_u01(d.v[0]), r2 = sfp32_u01(d.v[2]);
+      const float dtau = (float)a.dt * ig;
+      if (!(chi >= P.od.chi_min)) {
+        const float r1_3 = r1 * r1 * r1;
+        const float p = 3.0f * (r1 * r1) *
+                        sfp32_rs_rate(lds_rs, (int)P.z_N, (float)P.z_min, (float)P.z_max,
+                                      (float)P.z_delta, (float)(1.0 / P.z_delta), (float)chi, r1_3,
+                                      true) *
+                        dtau;
+        if (dg) {
+          dg->prob(p);
+          dg->zcheck(P, chi, r1_3);
+        }
+        if (r2 < p) { event = true; delta = r1_3; }
+        break;
       }
-      if (r2 < p) { event = true; delta = r1_3; }
-      return event;
+      float lc = __log10f((float)chi);
+      const float lmax = (float)P.od.log_chi_max, lmin = (float)P.od.log_chi_min;
+      if (dg) dg->clamp(lc > lmax);
+      if (lc > lmax) lc = lmax;
+      const float ld = (float)P.od.log_chi_delta;
+      int idx = (int)((lc - lmin) / ld);
+      if (idx > (int)P.od.chi_N - 2) idx = (int)P.od.chi_N - 2;
+      const double2 nd = lds_od[idx];
+      const float W = (float)nd.x + (float)nd.y * (lc - (lmin + (float)idx * ld));
+      if (dg) dg->prob(W * dtau);  // composition: the event probability of the step
+      if (!(r2 < W * dtau)) break;
+      event = true;
+      u = r1;
+      u_table = (double)r1;
+    } else {
+      if (!(chi >= P.od.chi_min)) break;
+      if (dg) dg->od(chi > P.od.chi_max);
+      event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag,
+                          lds_od, gtab(P.od.photon_cdf), P.od, &delta);
+      if (!event) break;
+      sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 2u);
+      // the optical-depth inverse CDF maps its uniform onto [ymin, ymax] of the
+      // cumulative rate; on the network's from-zero scale that is
+      // (ymin + u (ymax - ymin)) / ymax
+      double ymin, ymax;
+      sfp_od_row_ends(gtab(P.od.photon_cdf), &P.od, chi, &ymin, &ymax);
+      const double ud = sfp_u01(d.v[2], d.v[3]);
+      u = (float)((ymin + ud * (ymax - ymin)) / ymax);
+      od_u = ud;
+      u_table = ud;
     }
-    float lc = __log10f((float)chi);
-    const float lmax = (float)P.od.log_chi_max, lmin = (float)P.od.log_chi_min;
-    if (dg) dg->clamp(lc > lmax);
-    if (lc > lmax) lc = lmax;
-    const float ld = (float)P.od.log_chi_delta;
-    int idx = (int)((lc - lmin) / ld);
-    if (idx > (int)P.od.chi_N - 2) idx = (int)P.od.chi_N - 2;
-    const double2 nd = lds_od[idx];
-    const float W = (float)nd.x + (float)nd.y * (lc - (lmin + (float)idx * ld));
-    if (dg) dg->prob(W * dtau);  // composition: the event probability of the step
-    if (!(r2 < W * dtau)) return false;
-    event = true;
-    u = r1;
-    u_table = (double)r1;
-  } else {
-    if (!(chi >= P.od.chi_min)) return false;
-    if (dg) dg->od(chi > P.od.chi_max);
-    event = od_update32(&tau, ig, chi, (float)a.dt, P.seed, pid, a.step, (uint32_t)sp.tag, lds_od,
-                        gtab(P.od.photon_cdf), P.od, &delta);
-    if (!event) return false;
-    sfp_u32x4 d = sfp_draw4(P.seed, pid, a.step, (uint32_t)sp.tag, 2u);
-    // the optical-depth inverse CDF maps its uniform onto [ymin, ymax] of the
-    // cumulative rate; on the network's from-zero scale that is
-    // (ymin + u (ymax - ymin)) / ymax
-    double ymin, ymax;
-    sfp_od_row_ends(gtab(P.od.photon_cdf), &P.od, chi, &ymin, &ymax);
-    const double ud = sfp_u01(d.v[2], d.v[3]);
-    u = (float)((ymin + ud * (ymax - ymin)) / ymax);
-    od_u = ud;
-    u_table = ud;
-  }
-  // delta: the network, or the table in a fallback bin / outside the range
-  const float lc = __log10f((float)chi);
-  const float fb = (lc - nb.lc_min) / (nb.lc_max - nb.lc_min) * (float)SFP_MLP_BINS;
-  const int bin = fb < 0.0f ? 0 : (fb >= (float)SFP_MLP_BINS ? SFP_MLP_BINS - 1 : (int)fb);
-  if (lc >= nb.lc_min && lc <= nb.lc_max && !((nb.fallback >> bin) & 1ull)) {
-    net = true;
-    in0 = sfp_mlp_x0(P.net, lc);
-    in1 = 2.0f * u - 1.0f;
-  } else if (KIND == KIND_RS) {
-    delta = sfp_od_sample_from0(gtab(P.od.photon_cdf), &P.od, chi, (double)u);
-  } else {
-    delta = sfp_od_inverse_cdf(gtab(P.od.photon_cdf), &P.od, chi, od_u);
-  }
+    // delta: the network, or the table in a fallback bin / outside the range
+    const float lc = __log10f((float)chi);
+    const float fb = (lc - nb.lc_min) / (nb.lc_max - nb.lc_min) * (float)SFP_MLP_BINS;
+    const int bin = fb < 0.0f ? 0 : (fb >= (float)SFP_MLP_BINS ? SFP_MLP_BINS - 1 : (int)fb);
+    if (lc >= nb.lc_min && lc <= nb.lc_max && !((nb.fallback >> bin) & 1ull)) {
+      net = true;
+      in0 = sfp_mlp_x0(P.net, lc);
+      in1 = 2.0f * u - 1.0f;
+    } else if (KIND == KIND_RS) {
+      delta = sfp_od_sample_from0(gtab(P.od.photon_cdf), &P.od, chi, (double)u);
+    } else {
+      delta = sfp_od_inverse_cdf(gtab(P.od.photon_cdf), &P.od, chi, od_u);
+    }
+  } while (false);
+  delta_out = delta;
+  net_out = net;
+  in0_out = in0;
+  in1_out = in1;
+  chi_out = chi;
+  u_table_out = u_table;
   return event;
 }
 
