@@ -355,7 +355,7 @@ def _csv_rows(d, suffix):
     return rows
 
 
-_LEPTON = ("k_sweep<false", "k_hot", "k_emit", "k_step_prologue")
+_LEPTON = ("k_sweep<false", "k_sweep_pair", "k_hot", "k_emit", "k_step_prologue")
 
 
 def _step_opens(seq):
@@ -369,7 +369,8 @@ def _step_opens(seq):
         name = r[0]
         if name.startswith("k_step_prologue"):
             out.append(i)
-        elif name.startswith("k_sweep<false") and not (i and seq[i - 1][0].startswith(_LEPTON)):
+        elif (name.startswith(("k_sweep<false", "k_sweep_pair"))
+              and not (i and seq[i - 1][0].startswith(_LEPTON))):
             out.append(i)
     return out
 
@@ -377,9 +378,9 @@ def _step_opens(seq):
 def _by_species(seq):
     """Names of the per-species kernels tagged with their species: a step
     launches the lepton kernels for electrons, then for positrons, then the
-    photon kernels (_step_opens).  With shared QED launches a step sweeps both
-    lepton species and then runs one launch of the QED (and emission) kernel
-    for both: tagged [e-,e+]."""
+    photon kernels (_step_opens).  With shared launches one launch sweeps both
+    lepton species (k_sweep_pair) and one launch of the QED (and emission)
+    kernel serves both: tagged [e-,e+]."""
     steps, cur = [], []
     opens = set(_step_opens(seq))
     for i, r in enumerate(seq):
@@ -394,11 +395,13 @@ def _by_species(seq):
         n = {}
         for r in st:
             n[r[0]] = n.get(r[0], 0) + 1
-        sweeps = sum(v for k, v in n.items() if k.startswith("k_sweep<false"))
+        sweeps = sum(v for k, v in n.items() if k.startswith("k_sweep<false")) + 2 * sum(
+            v for k, v in n.items() if k.startswith("k_sweep_pair"))
         seen = {}
         for r in st:
             name = r[0]
-            if name.startswith(("k_hot", "k_emit")) and n[name] == 1 and sweeps == 2:
+            if (name.startswith("k_sweep_pair")
+                    or name.startswith(("k_hot", "k_emit")) and n[name] == 1 and sweeps == 2):
                 name = f"{name} [e-,e+]"
             elif name.startswith(("k_sweep", "k_hot", "k_emit", "k_defer")):
                 k = seen.get(name, 0)

@@ -713,14 +713,11 @@ static constexpr int ROWS = 32;
 static constexpr int SEL_PROLOGUE = 0x100;
 __device__ inline void step_snapshot(const Plan& P, const StepArgs& a);
 
+// One species' sweep; opens: the launch opens the step (the count is read
+// from n), snap: this call takes the step's snapshot.
 template <bool PHOTON, bool QED, int FM, bool MIX, bool SH>
-__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int sel, const StepArgs* __restrict__ ap,
-        int rmax) {
-  const StepArgs a = *ap;
-  const Plan& P = *plan;
-  if (P.halt->flag) return;
-  const int s = sel & 0xff;
-  const bool opens = (sel & SEL_PROLOGUE) != 0;
+__device__ inline void sweep_species(const Plan& P, const StepArgs& a, int s, bool opens, bool snap,
+                                     int rmax) {
   const SpeciesDev& sp = P.sp[s];
   __shared__ int64_t sh_tile;
   __shared__ uint32_t sh_w[4 * WAVES];
@@ -728,7 +725,7 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
   __shared__ uint64_t sh_pref;
 
   const int64_t n_old = opens ? sp.cnt->n : sp.cnt->n_old;
-  if (opens && blockIdx.x == 0) step_snapshot(P, a);
+  if (snap && blockIdx.x == 0) step_snapshot(P, a);
   if (!PHOTON && QED) {
     // Dense step: when the last step's bound listed at least 7/8 of the
     // particles, this kernel stands down and the QED kernel visits every
@@ -968,6 +965,35 @@ __global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restr
       sp.cnt->n_hot = 0;
     }
   });
+}
+
+template <bool PHOTON, bool QED, int FM, bool MIX, bool SH>
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep(const Plan* __restrict__ plan, int sel, const StepArgs* __restrict__ ap,
+        int rmax) {
+  const StepArgs a = *ap;
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const bool opens = (sel & SEL_PROLOGUE) != 0;
+  sweep_species<PHOTON, QED, FM, MIX, SH>(P, a, sel & 0xff, opens, opens, rmax);
+}
+
+// The sweeps of two lepton species with QED that share their QED launches
+// (sel: the first species in bits 0-3, the second plus one in bits 4-7), one
+// after the other in one launch: a launch and its gap less per step.  The two
+// lists and scans are separate, so the second sweep waits on nothing of the
+// first; a workgroup takes the second species' tiles once it has left the
+// first's (workgroups past the first's participants at once), so every tile
+// a look-back waits on belongs to a running workgroup.
+template <int FM, bool MIX, bool SH>
+__global__ void __launch_bounds__(TILE, MIX ? 5 : 3) k_sweep_pair(const Plan* __restrict__ plan, int sel, const StepArgs* __restrict__ ap,
+        int rmax) {
+  const StepArgs a = *ap;
+  const Plan& P = *plan;
+  if (P.halt->flag) return;
+  const bool opens = (sel & SEL_PROLOGUE) != 0;
+  sweep_species<false, true, FM, MIX, SH>(P, a, sel & 0xf, opens, opens, rmax);
+  __syncthreads();  // the workgroup's LDS is reused
+  sweep_species<false, true, FM, MIX, SH>(P, a, ((sel >> 4) & 0xf) - 1, opens, false, rmax);
 }
 
 // ------------------------------------------------------------- QED front --
@@ -2921,7 +2947,8 @@ __global__ void __launch_bounds__(256) k_catchup_slow(const Plan* __restrict__ p
 typedef void (*kfn)(const Plan*, int, const StepArgs*, int);
 
 template <int FM, bool MIX, bool SH>
-static kfn sweep_fn(bool photon, bool qed) {
+static kfn sweep_fn(bool photon, bool qed, bool pair) {
+  if (pair) return k_sweep_pair<FM, MIX, SH>;
   if (photon) return qed ? k_awake : k_sweep<true, false, FM, false, false>;
   return qed ? k_sweep<false, true, FM, MIX, SH> : k_sweep<false, false, FM, MIX, SH>;
 }
@@ -2938,10 +2965,10 @@ static kfn hot_fn(bool photon, int kind, int netk) {
 }
 
 template <bool MIX, bool SH>
-static kfn pick_sweep_p(int fm, bool photon, bool qed) {
-  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN, MIX, SH>(photon, qed);
-  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE, MIX, SH>(photon, qed);
-  return sweep_fn<SFP_FM_ANY, MIX, SH>(photon, qed);
+static kfn pick_sweep_p(int fm, bool photon, bool qed, bool pair) {
+  if (fm == SFP_FM_GAUSSIAN) return sweep_fn<SFP_FM_GAUSSIAN, MIX, SH>(photon, qed, pair);
+  if (fm == SFP_FM_PLANE) return sweep_fn<SFP_FM_PLANE, MIX, SH>(photon, qed, pair);
+  return sweep_fn<SFP_FM_ANY, MIX, SH>(photon, qed, pair);
 }
 
 template <bool MIX, bool SH>
@@ -2951,11 +2978,13 @@ static kfn pick_hot_p(int fm, bool photon, int kind, int netk) {
   return hot_fn<SFP_FM_ANY, MIX, SH>(photon, kind, netk);
 }
 
-// sh: an ensemble (a field set per shot)
-static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed, bool sh) {
+// sh: an ensemble (a field set per shot); pair: the two-species lepton sweep
+static kfn pick_sweep(int fm, bool photon, bool qed, bool mixed, bool sh, bool pair = false) {
   if (sh)
-    return mixed ? pick_sweep_p<true, true>(fm, photon, qed) : pick_sweep_p<false, true>(fm, photon, qed);
-  return mixed ? pick_sweep_p<true, false>(fm, photon, qed) : pick_sweep_p<false, false>(fm, photon, qed);
+    return mixed ? pick_sweep_p<true, true>(fm, photon, qed, pair)
+                 : pick_sweep_p<false, true>(fm, photon, qed, pair);
+  return mixed ? pick_sweep_p<true, false>(fm, photon, qed, pair)
+               : pick_sweep_p<false, false>(fm, photon, qed, pair);
 }
 
 static kfn pick_hot(int fm, bool photon, int kind, bool mixed, int netk, bool sh) {
@@ -3022,6 +3051,7 @@ struct Sim {
   std::vector<Store*> stores;
   std::vector<int> qed;            // species has a QED process
   kfn sweep[SFP_MAX_SPECIES] = {};
+  kfn sweep2[SFP_MAX_SPECIES] = {};  // leptons with QED: the sweep of this species and the next
   kfn hot[SFP_MAX_SPECIES] = {};
   kfn emit[SFP_MAX_SPECIES] = {};
   kfn_defer defer[SFP_MAX_SPECIES] = {};  // photons: deferred conversions
@@ -3241,6 +3271,8 @@ static int sim_pick_kernels(Sim* sim) {
   for (int s = 0; s < sim->nsp; ++s) {
     const bool photon = sim->plan.sp[s].photon;
     sim->sweep[s] = pick_sweep(sim->fm, photon, sim->qed[s], mixed, sh);
+    sim->sweep2[s] = (!photon && sim->qed[s]) ? pick_sweep(sim->fm, false, true, mixed, sh, true)
+                                              : nullptr;
     // learned models: the rate network for spin-carrying leptons under
     // spin-resolved emission, else the photon-energy sampler
     int netk = 0;
@@ -3302,6 +3334,7 @@ static int sim_pick_kernels(Sim* sim) {
         continue;
       sim->fuse[s] = b;
       sim->inner[b] = true;
+      sim->cand[b][0].clear();
       sim->cand[b][1].clear();
       sim->cand[b][2].clear();
     }
@@ -3589,7 +3622,8 @@ static hipEvent_t sim_event(Sim* sim) {
 // several times a step in between: the few of them in a batch would fall to
 // some candidates only, so the tuner compares the steps in between.
 static int launch(Sim* sim, int s, int ph, int j, int tune, bool opens = false) {
-  kfn fn = ph == 0 ? sim->sweep[s] : (ph == 1 ? sim->hot[s] : sim->emit[s]);
+  const bool pair = sim->fuse[s] >= 0;  // the next species' kernels run in this species' launches
+  kfn fn = ph == 0 ? (pair ? sim->sweep2[s] : sim->sweep[s]) : (ph == 1 ? sim->hot[s] : sim->emit[s]);
   const std::vector<Shape>& cand = sim->cand[s][ph];
   const int c = tune >= 0 ? tune % (int)cand.size() : sim->choice[s][ph];
   const StepArgs& ah = sim->args_host[j];
@@ -3603,10 +3637,9 @@ static int launch(Sim* sim, int s, int ph, int j, int tune, bool opens = false) 
     e1 = sim_event(sim);
     if (e0) HIPCHK(hipEventRecord(e0, sim->stream));
   }
-  // the QED kernels take a second species in bits 4-7 (plus one); a sweep
-  // that opens the step, SEL_PROLOGUE
-  const int sel = (ph > 0 && sim->fuse[s] >= 0) ? (s | ((sim->fuse[s] + 1) << 4))
-                                                : (s | (opens ? SEL_PROLOGUE : 0));
+  // the kernels of two species take the second in bits 4-7 (plus one); a
+  // sweep that opens the step, SEL_PROLOGUE
+  const int sel = (pair ? (s | ((sim->fuse[s] + 1) << 4)) : s) | (opens ? SEL_PROLOGUE : 0);
   hipLaunchKernelGGL(fn, dim3(cand[c].grid), dim3(TILE), 0, sim->stream, (const Plan*)sim->plan_dev,
                      sel, (const StepArgs*)(sim->args_dev + j), cand[c].rows);
   HIPCHK(hipGetLastError());
@@ -3632,12 +3665,8 @@ static int launch_step(Sim* sim, int j, int s0, int ph0, int tune) {
   for (int s = s0; s < sim->nsp; ++s) {
     if (sim->inner[s]) continue;  // launched with the species before it
     int first = (s == s0) ? ph0 : 0;
-    const int b = sim->fuse[s];
-    if (first <= 0) {
-      CHK(launch(sim, s, 0, j, tune, merged && s == 0));
-      // a partner's sweep makes its QED list before the shared QED launch
-      if (b >= 0) CHK(launch(sim, b, 0, j, tune));
-    }
+    // with a partner (fuse), one launch sweeps both species
+    if (first <= 0) CHK(launch(sim, s, 0, j, tune, merged && s == 0));
     if (first <= 1 && sim->hot[s]) CHK(launch(sim, s, 1, j, tune));
     if (first <= 2 && sim->emit[s]) CHK(launch(sim, s, 2, j, tune));
   }

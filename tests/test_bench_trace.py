@@ -1,12 +1,13 @@
 """The benchmark's kernel-trace analysis (bench.py --summarize-profile): steps
 and timed windows found in a dispatch sequence, and the species tags of the
 per-species kernels, for the three launch layouts a step can have -- shared
-lepton QED launches with the count snapshot in the first sweep (the
-default), one QED launch per lepton species, and a step opened by the
-prologue launch."""
+lepton launches (one sweep launch, k_sweep_pair, and one QED launch for both
+species) with the count snapshot in the sweep (the default), one launch of
+each kernel per lepton species, and a step opened by the prologue launch."""
 import bench
 
 SW = "k_sweep<false, true, 1, true, false>"
+SWP = "k_sweep_pair<1, true, false>"
 HOT = "k_hot<0, 1, true, 1, false>"
 EMIT = "k_emit<0, 1, false>"
 PH = ["k_awake", "k_photon<0, 1, true, false>"]
@@ -32,13 +33,13 @@ def _seq(step_kernels, nsteps, settles=True):
 
 
 def test_shared_launches_layout():
-    seq = bench._by_species(_seq([SW, SW, HOT, EMIT] + PH, 3))
+    seq = bench._by_species(_seq([SWP, HOT, EMIT] + PH, 3))
     names = [r[0] for r in seq]
-    assert names[2:8] == [f"{SW} [e-]", f"{SW} [e+]", f"{HOT} [e-,e+]", f"{EMIT} [e-,e+]",
+    assert names[2:7] == [f"{SWP} [e-,e+]", f"{HOT} [e-,e+]", f"{EMIT} [e-,e+]",
                           "k_awake [photon]", "k_photon<0, 1, true, false> [photon]"]
-    assert bench._step_opens(seq) == [2, 8, 14]
+    assert bench._step_opens(seq) == [2, 7, 12]
     wins = bench._windows(seq, 3)
-    assert len(wins) == 1 and len(wins[0]) == 3 * 6 + 2
+    assert len(wins) == 1 and len(wins[0]) == 3 * 5 + 2
     assert bench._windows(seq, 4) == []
 
 
